@@ -1,0 +1,752 @@
+// C ABI of libatac_contact.so (declared in include/atac.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "atac.h"
+#include "atac_compose.h"
+#include "atac_kernels.h"
+#include "atac_types.h"
+
+using namespace atac;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define ATAC_HIP(call)                                                                          \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess) return fail(ATAC_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+int dalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) return ATAC_OK;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e != hipSuccess) return fail(ATAC_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return ATAC_OK;
+}
+
+template <typename T>
+void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+}  // namespace
+
+struct atac_grid_s {
+    GridDev dev;
+    float* d_values = nullptr;
+    int64_t count = 0;
+};
+
+struct atac_points_s {
+    float* d_x = nullptr;
+    float* d_y = nullptr;
+    float* d_z = nullptr;
+    int64_t n = 0;
+    std::vector<int64_t> seg;   // n_segments + 1 offsets
+    std::vector<int64_t> perm;  // sorted position -> original index within the segment
+};
+
+struct atac_batch_s {
+    int device = 0;
+    int n_cu = 0;
+    size_t lds_limit = 0;
+    std::vector<atac_grid> grids;
+    std::vector<atac_points> pts;
+    std::vector<atac_instance> inst;
+    std::vector<InstDev> inst_dev;
+    InstDev* d_inst = nullptr;
+    GridDev* d_grids = nullptr;
+    atac_tuning tune{};
+    std::vector<TileDev> tiles;
+    TileDev* d_tiles = nullptr;
+    int32_t* d_order = nullptr;
+    int32_t n_tiles = 0;
+    int64_t n_slots = 0;
+    uint32_t* d_act = nullptr;
+    CkptDev* d_ck = nullptr;
+    int32_t* d_tile_active = nullptr;
+    double* d_partial = nullptr;
+    int64_t partial_cap = 0;   // doubles
+    double* d_vjp_partial = nullptr;
+    int32_t* d_counters = nullptr;  // detect, contact, vjp
+    int grid_detect = 0, grid_contact = 0, grid_vjp = 0;
+    size_t lds_detect = 0;
+    int wave_seg = 0;
+    bool ckpt_valid = false;
+    bool ckpt_precise = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    hipStream_t cap_stream = nullptr;
+    // optional per-stage timing: event pairs recorded around each kernel launch
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;  // stage, (start, stop)
+};
+
+namespace {
+enum { ST_DETECT = 0, ST_CONTACT = 1, ST_REDUCE = 2, ST_VJP = 3, ST_VJP_REDUCE = 4, ST_COUNT = 5 };
+
+hipEvent_t take_event(atac_batch b) {
+    if (!b->ev_pool.empty()) {
+        hipEvent_t e = b->ev_pool.back();
+        b->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Records a start event on st when timing is enabled; returns it (or nullptr).
+hipEvent_t stage_begin(atac_batch b, hipStream_t st) {
+    if (!b->timing) return nullptr;
+    hipEvent_t e = take_event(b);
+    if (e) (void)hipEventRecord(e, st);
+    return e;
+}
+
+void stage_end(atac_batch b, int stage, hipEvent_t start, hipStream_t st) {
+    if (!b->timing || !start) return;
+    hipEvent_t e = take_event(b);
+    if (!e) return;
+    (void)hipEventRecord(e, st);
+    b->ev_used.push_back({stage, {start, e}});
+}
+}  // namespace
+
+extern "C" {
+
+const char* atac_last_error(void) { return g_err.c_str(); }
+
+int atac_version(void) { return 100; }
+
+int atac_device_init(int32_t device, int32_t* n_cu) {
+    int count = 0;
+    ATAC_HIP(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return fail(ATAC_ERR_INVALID, "atac_device_init: no such device");
+    ATAC_HIP(hipSetDevice(device));
+    if (n_cu) {
+        hipDeviceProp_t prop;
+        ATAC_HIP(hipGetDeviceProperties(&prop, device));
+        *n_cu = prop.multiProcessorCount;
+    }
+    return ATAC_OK;
+}
+
+// ---------------------------------------------------------------- grids
+int atac_grid_create(const float* values, int32_t ni, int32_t nj, int32_t nk, const double min_box[3],
+                     const double max_box[3], const double dxyz[3], atac_grid* out) {
+    if (!values || !out || !min_box || !max_box || !dxyz) return fail(ATAC_ERR_INVALID, "atac_grid_create: null argument");
+    if (ni < 1 || nj < 1 || nk < 1) return fail(ATAC_ERR_INVALID, "atac_grid_create: empty grid");
+    for (int a = 0; a < 3; ++a)
+        if (!(dxyz[a] > 0.0) || !(max_box[a] >= min_box[a]))
+            return fail(ATAC_ERR_INVALID, "atac_grid_create: spacing must be > 0 and max_box >= min_box");
+    auto* g = new atac_grid_s();
+    g->count = (int64_t)ni * nj * nk;
+    int rc = dalloc(&g->d_values, (size_t)g->count);
+    if (rc) { delete g; return rc; }
+    hipError_t e = hipMemcpy(g->d_values, values, sizeof(float) * g->count, hipMemcpyHostToDevice);
+    if (e != hipSuccess) { dfree(g->d_values); delete g; return fail(ATAC_ERR_HIP, hipGetErrorString(e)); }
+    GridDev& G = g->dev;
+    G.v = g->d_values;
+    G.n[0] = ni; G.n[1] = nj; G.n[2] = nk; G.pad = 0;
+    for (int a = 0; a < 3; ++a) {
+        G.minb[a] = min_box[a];
+        G.maxb[a] = max_box[a];
+        G.d[a] = dxyz[a];
+        G.inv_d[a] = 1.0 / dxyz[a];
+    }
+    *out = g;
+    return ATAC_OK;
+}
+
+int atac_grid_destroy(atac_grid g) {
+    if (!g) return ATAC_OK;
+    dfree(g->d_values);
+    delete g;
+    return ATAC_OK;
+}
+
+// ---------------------------------------------------------------- points
+static uint64_t spread3(uint64_t v) {  // 21 bits -> every third bit
+    v &= 0x1fffffull;
+    v = (v | (v << 32)) & 0x1f00000000ffffull;
+    v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+    v = (v | (v << 8)) & 0x100f00f00f00f00full;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+    v = (v | (v << 2)) & 0x1249249249249249ull;
+    return v;
+}
+
+int atac_points_create(const double* xyz, int64_t n, const int64_t* seg_offsets, int32_t n_segments,
+                       double sort_cell, atac_points* out) {
+    if (!out || n < 0 || (n > 0 && !xyz)) return fail(ATAC_ERR_INVALID, "atac_points_create: bad arguments");
+    std::vector<int64_t> seg;
+    if (seg_offsets) {
+        if (n_segments < 1) return fail(ATAC_ERR_INVALID, "atac_points_create: n_segments < 1");
+        seg.assign(seg_offsets, seg_offsets + n_segments + 1);
+        if (seg.front() != 0 || seg.back() != n) return fail(ATAC_ERR_INVALID, "atac_points_create: offsets must span [0, n]");
+        for (int s = 0; s < n_segments; ++s)
+            if (seg[s + 1] < seg[s]) return fail(ATAC_ERR_INVALID, "atac_points_create: offsets not ascending");
+    } else {
+        seg = {0, n};
+    }
+    auto* P = new atac_points_s();
+    P->n = n;
+    P->seg = seg;
+    P->perm.resize(n);
+    std::vector<float> hx(n), hy(n), hz(n);
+    std::vector<std::pair<uint64_t, int64_t>> keys;
+    for (size_t s = 0; s + 1 < seg.size(); ++s) {
+        const int64_t b = seg[s], e = seg[s + 1], m = e - b;
+        if (m == 0) continue;
+        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int64_t i = b; i < e; ++i)
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = std::min(lo[a], xyz[3 * i + a]);
+                hi[a] = std::max(hi[a], xyz[3 * i + a]);
+            }
+        double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+        double cell = sort_cell > 0.0 ? sort_cell : (ext > 0.0 ? ext / 1024.0 : 1.0);
+        keys.resize(m);
+        for (int64_t i = 0; i < m; ++i) {
+            uint64_t q[3];
+            for (int a = 0; a < 3; ++a) {
+                double c = std::floor((xyz[3 * (b + i) + a] - lo[a]) / cell);
+                c = std::min(std::max(c, 0.0), 2097151.0);
+                q[a] = (uint64_t)c;
+            }
+            keys[i] = {spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2), i};
+        }
+        std::sort(keys.begin(), keys.end());
+        for (int64_t i = 0; i < m; ++i) {
+            const int64_t src = b + keys[i].second;
+            hx[b + i] = (float)xyz[3 * src];
+            hy[b + i] = (float)xyz[3 * src + 1];
+            hz[b + i] = (float)xyz[3 * src + 2];
+            P->perm[b + i] = keys[i].second;
+        }
+    }
+    int rc = dalloc(&P->d_x, n);
+    if (!rc) rc = dalloc(&P->d_y, n);
+    if (!rc) rc = dalloc(&P->d_z, n);
+    if (!rc && n > 0) {
+        hipError_t e = hipMemcpy(P->d_x, hx.data(), sizeof(float) * n, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(P->d_y, hy.data(), sizeof(float) * n, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(P->d_z, hz.data(), sizeof(float) * n, hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = fail(ATAC_ERR_HIP, hipGetErrorString(e));
+    }
+    if (rc) {
+        dfree(P->d_x); dfree(P->d_y); dfree(P->d_z);
+        delete P;
+        return rc;
+    }
+    *out = P;
+    return ATAC_OK;
+}
+
+int atac_points_destroy(atac_points p) {
+    if (!p) return ATAC_OK;
+    dfree(p->d_x); dfree(p->d_y); dfree(p->d_z);
+    delete p;
+    return ATAC_OK;
+}
+
+int atac_points_permutation(atac_points p, int32_t segment, int64_t* sorted_to_original, int64_t capacity) {
+    if (!p || !sorted_to_original) return fail(ATAC_ERR_INVALID, "atac_points_permutation: null argument");
+    if (segment < 0 || segment + 1 >= (int32_t)p->seg.size()) return fail(ATAC_ERR_INVALID, "atac_points_permutation: bad segment");
+    const int64_t b = p->seg[segment], m = p->seg[segment + 1] - b;
+    if (capacity < m) return fail(ATAC_ERR_INVALID, "atac_points_permutation: capacity too small");
+    std::copy(p->perm.begin() + b, p->perm.begin() + b + m, sorted_to_original);
+    return ATAC_OK;
+}
+
+// ---------------------------------------------------------------- batches
+static void compose_for(atac_batch b, const atac_instance& s, InstDev& d) {
+    const atac_points_s& P = *b->pts[s.points];
+    const int64_t beg = P.seg[s.segment];
+    compose_instance(s, b->grids[s.grid]->dev, P.d_x + beg, P.d_y + beg, P.d_z + beg, P.seg[s.segment + 1] - beg, d);
+}
+
+static int check_instances(atac_batch b, const atac_instance* inst, int32_t n) {
+    for (int32_t i = 0; i < n; ++i) {
+        const atac_instance& s = inst[i];
+        if (s.grid < 0 || s.grid >= (int32_t)b->grids.size()) return fail(ATAC_ERR_INVALID, "instance " + std::to_string(i) + ": bad grid index");
+        if (s.points < 0 || s.points >= (int32_t)b->pts.size()) return fail(ATAC_ERR_INVALID, "instance " + std::to_string(i) + ": bad point-set index");
+        if (s.segment < 0 || s.segment + 1 >= (int32_t)b->pts[s.points]->seg.size())
+            return fail(ATAC_ERR_INVALID, "instance " + std::to_string(i) + ": bad segment");
+        if (!(s.contact_scale != 0.0)) return fail(ATAC_ERR_INVALID, "instance " + std::to_string(i) + ": contact_scale must be non-zero");
+    }
+    return ATAC_OK;
+}
+
+static void release_plan(atac_batch b) {
+    dfree(b->d_tiles); dfree(b->d_order); dfree(b->d_act); dfree(b->d_ck); dfree(b->d_tile_active);
+    dfree(b->d_partial); dfree(b->d_vjp_partial);
+    b->ckpt_valid = false;
+    if (b->graph_exec) { (void)hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
+    if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+}
+
+static int plan(atac_batch b, const atac_tuning* t) {
+    atac_tuning tu = t ? *t : atac_tuning{};
+    if (tu.tile_points <= 0) tu.tile_points = 4096;
+    if (tu.detect_block <= 0) tu.detect_block = 256;
+    if (tu.tile_points > 65536) return fail(ATAC_ERR_INVALID, "tile_points must be <= 65536");
+    if (tu.detect_block != 128 && tu.detect_block != 256 && tu.detect_block != 512 && tu.detect_block != 1024)
+        return fail(ATAC_ERR_INVALID, "detect_block must be 128, 256, 512 or 1024");
+    if (tu.blocks_per_cu < 0 || tu.order < 0 || tu.order > 1) return fail(ATAC_ERR_INVALID, "bad tuning");
+    {
+        const int B = tu.detect_block;
+        const size_t lds = (size_t)(((tu.tile_points + B - 1) / B) * 64) * (B / 64) * sizeof(uint16_t);
+        if (lds + 1024 > b->lds_limit)
+            return fail(ATAC_ERR_INVALID, "tile_points " + std::to_string(tu.tile_points) + " with detect_block " +
+                                              std::to_string(B) + " needs " + std::to_string(lds) +
+                                              " B of LDS; the device allows " + std::to_string(b->lds_limit) +
+                                              " B per workgroup");
+    }
+    release_plan(b);
+    b->tune = tu;
+    const int T = tu.tile_points;
+    b->tiles.clear();
+    int64_t slot = 0;
+    for (size_t i = 0; i < b->inst_dev.size(); ++i) {
+        InstDev& d = b->inst_dev[i];
+        d.tile_begin = (int32_t)b->tiles.size();
+        for (int64_t o = 0; o < d.pt_count; o += T) {
+            TileDev td;
+            td.inst = (int32_t)i;
+            td.count = (int32_t)std::min<int64_t>(T, d.pt_count - o);
+            td.begin = o;
+            td.slot = slot + o;
+            b->tiles.push_back(td);
+        }
+        d.tile_count = (int32_t)b->tiles.size() - d.tile_begin;
+        slot += d.pt_count;
+    }
+    b->n_slots = slot;
+    b->n_tiles = (int32_t)b->tiles.size();
+    std::vector<int32_t> order(b->n_tiles);
+    std::iota(order.begin(), order.end(), 0);
+    if (tu.order == 1)
+        std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+            return b->inst_dev[b->tiles[x].inst].grid < b->inst_dev[b->tiles[y].inst].grid;
+        });
+    int rc = dalloc(&b->d_tiles, b->n_tiles);
+    if (!rc) rc = dalloc(&b->d_order, b->n_tiles);
+    if (!rc) rc = dalloc(&b->d_tile_active, b->n_tiles);
+    if (!rc) rc = dalloc(&b->d_act, (size_t)b->n_slots);
+    if (!rc) rc = dalloc(&b->d_vjp_partial, (size_t)b->n_tiles * kVjpOut);
+    if (!rc) { b->partial_cap = (int64_t)b->n_tiles * 24; rc = dalloc(&b->d_partial, (size_t)b->partial_cap); }
+    if (rc) return rc;
+    if (b->n_tiles > 0) {
+        ATAC_HIP(hipMemcpy(b->d_tiles, b->tiles.data(), sizeof(TileDev) * b->n_tiles, hipMemcpyHostToDevice));
+        ATAC_HIP(hipMemcpy(b->d_order, order.data(), sizeof(int32_t) * b->n_tiles, hipMemcpyHostToDevice));
+    }
+    // inst_dev carries the tile ranges: re-upload
+    if (!b->inst_dev.empty())
+        ATAC_HIP(hipMemcpy(b->d_inst, b->inst_dev.data(), sizeof(InstDev) * b->inst_dev.size(), hipMemcpyHostToDevice));
+    // persistent launch shapes
+    const int B = tu.detect_block;
+    b->wave_seg = ((T + B - 1) / B) * 64;
+    b->lds_detect = (size_t)b->wave_seg * (B / 64) * sizeof(uint16_t);
+    int nb = 0;
+    ATAC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, detect_kernel_ptr(B), B, b->lds_detect));
+    if (tu.blocks_per_cu > 0) nb = std::min(nb, tu.blocks_per_cu);
+    nb = std::max(nb, 1);
+    b->grid_detect = std::max(1, std::min(b->n_tiles, nb * b->n_cu));
+    int nc = 0;
+    ATAC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nc, contact_kernel_ptr(1, false), 256, 0));
+    b->grid_contact = std::max(1, std::min(b->n_tiles, std::max(nc, 1) * b->n_cu));
+    int nv = 0;
+    ATAC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nv, vjp_kernel_ptr(false), 256, 0));
+    b->grid_vjp = std::max(1, std::min(b->n_tiles, std::max(nv, 1) * b->n_cu));
+    return ATAC_OK;
+}
+
+int atac_batch_create(const atac_grid* grids, int32_t n_grids, const atac_points* points, int32_t n_point_sets,
+                      const atac_instance* inst, int32_t n_inst, const atac_tuning* tuning, atac_batch* out) {
+    if (!out || n_grids < 0 || n_point_sets < 0 || n_inst < 0) return fail(ATAC_ERR_INVALID, "atac_batch_create: bad arguments");
+    if ((n_grids && !grids) || (n_point_sets && !points) || (n_inst && !inst)) return fail(ATAC_ERR_INVALID, "atac_batch_create: null array");
+    for (int i = 0; i < n_grids; ++i) if (!grids[i]) return fail(ATAC_ERR_INVALID, "atac_batch_create: null grid");
+    for (int i = 0; i < n_point_sets; ++i) if (!points[i]) return fail(ATAC_ERR_INVALID, "atac_batch_create: null point set");
+    auto* b = new atac_batch_s();
+    b->grids.assign(grids, grids + n_grids);
+    b->pts.assign(points, points + n_point_sets);
+    int rc = check_instances(b, inst, n_inst);
+    if (rc) { delete b; return rc; }
+    hipError_t e = hipGetDevice(&b->device);
+    hipDeviceProp_t prop;
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, b->device);
+    if (e != hipSuccess) { delete b; return fail(ATAC_ERR_HIP, hipGetErrorString(e)); }
+    b->n_cu = prop.multiProcessorCount;
+    b->lds_limit = prop.sharedMemPerBlock;
+    b->inst.assign(inst, inst + n_inst);
+    b->inst_dev.resize(n_inst);
+    for (int i = 0; i < n_inst; ++i)
+        compose_for(b, inst[i], b->inst_dev[i]);
+    std::vector<GridDev> gd(n_grids);
+    for (int i = 0; i < n_grids; ++i) gd[i] = b->grids[i]->dev;
+    rc = dalloc(&b->d_grids, n_grids);
+    if (!rc) rc = dalloc(&b->d_inst, n_inst);
+    if (!rc) rc = dalloc(&b->d_counters, 4);
+    if (!rc && n_grids) {
+        e = hipMemcpy(b->d_grids, gd.data(), sizeof(GridDev) * n_grids, hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = fail(ATAC_ERR_HIP, hipGetErrorString(e));
+    }
+    if (!rc) rc = plan(b, tuning);
+    if (!rc) {
+        e = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking);
+        if (e != hipSuccess) rc = fail(ATAC_ERR_HIP, hipGetErrorString(e));
+    }
+    if (rc) { atac_batch_destroy(b); return rc; }
+    *out = b;
+    return ATAC_OK;
+}
+
+int atac_batch_destroy(atac_batch b) {
+    if (!b) return ATAC_OK;
+    release_plan(b);
+    for (auto& u : b->ev_used) { (void)hipEventDestroy(u.second.first); (void)hipEventDestroy(u.second.second); }
+    for (auto e : b->ev_pool) (void)hipEventDestroy(e);
+    dfree(b->d_inst); dfree(b->d_grids); dfree(b->d_counters);
+    if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+    delete b;
+    return ATAC_OK;
+}
+
+int atac_batch_set_instances(atac_batch b, const atac_instance* inst, int32_t n_inst, void* stream) {
+    if (!b || (n_inst && !inst)) return fail(ATAC_ERR_INVALID, "atac_batch_set_instances: null argument");
+    if (n_inst != (int32_t)b->inst.size()) return fail(ATAC_ERR_INVALID, "atac_batch_set_instances: instance count differs");
+    for (int i = 0; i < n_inst; ++i)
+        if (inst[i].grid != b->inst[i].grid || inst[i].points != b->inst[i].points || inst[i].segment != b->inst[i].segment)
+            return fail(ATAC_ERR_INVALID, "atac_batch_set_instances: instance " + std::to_string(i) + " changes its grid/points/segment");
+    int rc = check_instances(b, inst, n_inst);
+    if (rc) return rc;
+    for (int i = 0; i < n_inst; ++i) {
+        const int32_t tb = b->inst_dev[i].tile_begin, tc = b->inst_dev[i].tile_count;
+        compose_for(b, inst[i], b->inst_dev[i]);
+        b->inst_dev[i].tile_begin = tb;
+        b->inst_dev[i].tile_count = tc;
+    }
+    b->inst.assign(inst, inst + n_inst);
+    if (n_inst)
+        ATAC_HIP(hipMemcpyAsync(b->d_inst, b->inst_dev.data(), sizeof(InstDev) * n_inst, hipMemcpyHostToDevice, (hipStream_t)stream));
+    return ATAC_OK;
+}
+
+int atac_batch_retune(atac_batch b, const atac_tuning* tuning) {
+    if (!b) return fail(ATAC_ERR_INVALID, "atac_batch_retune: null batch");
+    ATAC_HIP(hipDeviceSynchronize());
+    return plan(b, tuning);
+}
+
+int atac_batch_info(atac_batch b, int64_t* n_points, int32_t* n_tiles, int32_t* persistent_blocks, atac_tuning* tuning) {
+    if (!b) return fail(ATAC_ERR_INVALID, "atac_batch_info: null batch");
+    if (n_points) *n_points = b->n_slots;
+    if (n_tiles) *n_tiles = b->n_tiles;
+    if (persistent_blocks) *persistent_blocks = b->grid_detect;
+    if (tuning) *tuning = b->tune;
+    return ATAC_OK;
+}
+
+static int mode_of(uint32_t flags) { return (flags & ATAC_EVAL_DERIV) ? 2 : ((flags & ATAC_EVAL_P6) ? 1 : 0); }
+
+static int enqueue_eval(atac_batch b, uint32_t flags, const atac_outputs* out, hipStream_t st) {
+    if (flags & ~(ATAC_EVAL_DERIV | ATAC_EVAL_P6 | ATAC_EVAL_CHECKPOINT | ATAC_EVAL_PRECISE))
+        return fail(ATAC_ERR_INVALID, "atac_batch_eval: unknown flag");
+    const int mode = mode_of(flags);
+    const bool precise = (flags & ATAC_EVAL_PRECISE) != 0;
+    const int stride = mode == 2 ? 24 + 288 : 24;
+    if ((int64_t)stride * b->n_tiles > b->partial_cap) {
+        dfree(b->d_partial);
+        b->partial_cap = (int64_t)stride * b->n_tiles;
+        int rc = dalloc(&b->d_partial, (size_t)b->partial_cap);
+        if (rc) return rc;
+    }
+    if ((flags & ATAC_EVAL_CHECKPOINT) && !b->d_ck && b->n_slots > 0) {
+        int rc = dalloc(&b->d_ck, (size_t)b->n_slots);
+        if (rc) return rc;
+    }
+    if (b->n_tiles > 0) {
+        ATAC_HIP(launch_reset(b->d_counters, 2, st));
+        DetectArgs da{};
+        da.tiles = b->d_tiles; da.order = b->d_order; da.n_tiles = b->n_tiles; da.wave_seg = b->wave_seg;
+        da.counter = b->d_counters; da.inst = b->d_inst; da.grids = b->d_grids;
+        da.act_off = b->d_act; da.tile_active = b->d_tile_active;
+        hipEvent_t t0 = stage_begin(b, st);
+        ATAC_HIP(launch_detect(da, b->tune.detect_block, b->grid_detect, b->lds_detect, st));
+        stage_end(b, ST_DETECT, t0, st);
+        ContactArgs ca{};
+        ca.tiles = b->d_tiles; ca.order = b->d_order; ca.n_tiles = b->n_tiles; ca.stride = stride;
+        ca.counter = b->d_counters + 1; ca.inst = b->d_inst; ca.grids = b->d_grids;
+        ca.act_off = b->d_act; ca.tile_active = b->d_tile_active; ca.partial = b->d_partial;
+        ca.ck_dg = (flags & ATAC_EVAL_CHECKPOINT) ? b->d_ck : nullptr;
+        hipEvent_t t1 = stage_begin(b, st);
+        ATAC_HIP(launch_contact(ca, mode, precise, b->grid_contact, st));
+        stage_end(b, ST_CONTACT, t1, st);
+    }
+    ReduceArgs ra{};
+    ra.inst = b->d_inst; ra.n_inst = (int32_t)b->inst.size(); ra.stride = stride; ra.partial = b->d_partial;
+    ra.tile_active = b->d_tile_active; ra.n_active = out ? out->n_active : nullptr;
+    int ns = 0;
+    if (out && out->fm) ra.seg[ns++] = ReduceSeg{0, 12, out->fm};
+    if (out && out->dfm_dp6) {
+        if (mode < 1) return fail(ATAC_ERR_INVALID, "dfm_dp6 requested without ATAC_EVAL_P6 or ATAC_EVAL_DERIV");
+        ra.seg[ns++] = ReduceSeg{12, 12, out->dfm_dp6};
+    }
+    if (out && (out->Km || out->Dm) && mode < 2) return fail(ATAC_ERR_INVALID, "Km/Dm requested without ATAC_EVAL_DERIV");
+    if (out && out->Km) ra.seg[ns++] = ReduceSeg{24, 144, out->Km};
+    if (out && out->Dm) ra.seg[ns++] = ReduceSeg{168, 144, out->Dm};
+    ra.nseg = ns;
+    if (ns || ra.n_active) {
+        hipEvent_t t2 = stage_begin(b, st);
+        ATAC_HIP(launch_reduce(ra, st));
+        stage_end(b, ST_REDUCE, t2, st);
+    }
+    b->ckpt_valid = (flags & ATAC_EVAL_CHECKPOINT) != 0;
+    b->ckpt_precise = precise;
+    return ATAC_OK;
+}
+
+static int enqueue_vjp(atac_batch b, const double* g_fm, const atac_gradients* g, hipStream_t st) {
+    if (!b->ckpt_valid) return fail(ATAC_ERR_STATE, "atac_batch_vjp: the last evaluation kept no checkpoint (ATAC_EVAL_CHECKPOINT)");
+    if (!g_fm) return fail(ATAC_ERR_INVALID, "atac_batch_vjp: g_fm is null");
+    if (b->n_tiles > 0) {
+        ATAC_HIP(launch_reset(b->d_counters + 2, 1, st));
+        VjpArgs va{};
+        va.tiles = b->d_tiles; va.order = b->d_order; va.n_tiles = b->n_tiles; va.counter = b->d_counters + 2;
+        va.inst = b->d_inst; va.grids = b->d_grids; va.tile_active = b->d_tile_active; va.act_off = b->d_act;
+        va.ck_dg = b->d_ck; va.g_fm = g_fm; va.partial = b->d_vjp_partial;
+        hipEvent_t t3 = stage_begin(b, st);
+        ATAC_HIP(launch_vjp(va, b->ckpt_precise, b->grid_vjp, st));
+        stage_end(b, ST_VJP, t3, st);
+    }
+    ReduceArgs ra{};
+    ra.inst = b->d_inst; ra.n_inst = (int32_t)b->inst.size(); ra.stride = kVjpOut; ra.partial = b->d_vjp_partial;
+    int ns = 0;
+    if (g && g->g_q1) ra.seg[ns++] = ReduceSeg{0, 6, g->g_q1};
+    if (g && g->g_q2) ra.seg[ns++] = ReduceSeg{6, 6, g->g_q2};
+    if (g && g->g_phi1) ra.seg[ns++] = ReduceSeg{12, 6, g->g_phi1};
+    if (g && g->g_phi2) ra.seg[ns++] = ReduceSeg{18, 6, g->g_phi2};
+    if (g && g->g_kn) ra.seg[ns++] = ReduceSeg{24, 1, g->g_kn};
+    if (g && g->g_kt) ra.seg[ns++] = ReduceSeg{25, 1, g->g_kt};
+    ra.nseg = ns;
+    if (ns) {
+        hipEvent_t t4 = stage_begin(b, st);
+        ATAC_HIP(launch_reduce(ra, st));
+        stage_end(b, ST_VJP_REDUCE, t4, st);
+    }
+    return ATAC_OK;
+}
+
+int atac_batch_eval(atac_batch b, uint32_t flags, const atac_outputs* out, void* stream) {
+    if (!b) return fail(ATAC_ERR_INVALID, "atac_batch_eval: null batch");
+    return enqueue_eval(b, flags, out, (hipStream_t)stream);
+}
+
+int atac_batch_vjp(atac_batch b, const double* g_fm, const atac_gradients* out, void* stream) {
+    if (!b) return fail(ATAC_ERR_INVALID, "atac_batch_vjp: null batch");
+    return enqueue_vjp(b, g_fm, out, (hipStream_t)stream);
+}
+
+int atac_batch_graph_capture(atac_batch b, uint32_t flags, const atac_outputs* out, const double* g_fm,
+                             const atac_gradients* grads) {
+    if (!b) return fail(ATAC_ERR_INVALID, "atac_batch_graph_capture: null batch");
+    if (g_fm && !(flags & ATAC_EVAL_CHECKPOINT)) return fail(ATAC_ERR_INVALID, "graph with adjoint needs ATAC_EVAL_CHECKPOINT");
+    if ((flags & ATAC_EVAL_CHECKPOINT) && !b->d_ck && b->n_slots > 0) {
+        int rc = dalloc(&b->d_ck, (size_t)b->n_slots);
+        if (rc) return rc;
+    }
+    {   // no allocation may happen while the stream is capturing
+        const int stride = mode_of(flags) == 2 ? 24 + 288 : 24;
+        if ((int64_t)stride * b->n_tiles > b->partial_cap) {
+            dfree(b->d_partial);
+            b->partial_cap = (int64_t)stride * b->n_tiles;
+            int rc = dalloc(&b->d_partial, (size_t)b->partial_cap);
+            if (rc) return rc;
+        }
+    }
+    if (b->graph_exec) { (void)hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
+    if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+    ATAC_HIP(hipStreamSynchronize(b->cap_stream));
+    const bool timing = b->timing;
+    b->timing = false;
+    ATAC_HIP(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_eval(b, flags, out, b->cap_stream);
+    if (!rc && g_fm) rc = enqueue_vjp(b, g_fm, grads, b->cap_stream);
+    b->timing = timing;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return fail(ATAC_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    b->graph = g;
+    ATAC_HIP(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
+    return ATAC_OK;
+}
+
+int atac_batch_graph_launch(atac_batch b, void* stream) {
+    if (!b || !b->graph_exec) return fail(ATAC_ERR_STATE, "atac_batch_graph_launch: no captured graph");
+    ATAC_HIP(hipGraphLaunch(b->graph_exec, (hipStream_t)stream));
+    return ATAC_OK;
+}
+
+int atac_batch_active_ids(atac_batch b, int32_t inst, int64_t* ids, int64_t capacity, int64_t* count) {
+    if (!b || !count) return fail(ATAC_ERR_INVALID, "atac_batch_active_ids: null argument");
+    if (inst < 0 || inst >= (int32_t)b->inst.size()) return fail(ATAC_ERR_INVALID, "atac_batch_active_ids: bad instance");
+    ATAC_HIP(hipDeviceSynchronize());
+    const InstDev& d = b->inst_dev[inst];
+    std::vector<int32_t> ta(d.tile_count);
+    if (d.tile_count)
+        ATAC_HIP(hipMemcpy(ta.data(), b->d_tile_active + d.tile_begin, sizeof(int32_t) * d.tile_count, hipMemcpyDeviceToHost));
+    const atac_points_s& P = *b->pts[b->inst[inst].points];
+    const int64_t seg_begin = P.seg[b->inst[inst].segment];
+    std::vector<int64_t> out;
+    for (int k = 0; k < d.tile_count; ++k) {
+        const TileDev& t = b->tiles[d.tile_begin + k];
+        std::vector<uint32_t> off(ta[k]);
+        if (ta[k]) ATAC_HIP(hipMemcpy(off.data(), b->d_act + t.slot, sizeof(uint32_t) * ta[k], hipMemcpyDeviceToHost));
+        for (uint32_t o : off) out.push_back(P.perm[seg_begin + t.begin + o]);
+    }
+    std::sort(out.begin(), out.end());
+    *count = (int64_t)out.size();
+    if (ids) {
+        if (capacity < (int64_t)out.size()) return fail(ATAC_ERR_INVALID, "atac_batch_active_ids: capacity too small");
+        std::copy(out.begin(), out.end(), ids);
+    }
+    return ATAC_OK;
+}
+
+int atac_batch_touched_values(atac_batch b, int64_t* count) {
+    if (!b || !count) return fail(ATAC_ERR_INVALID, "atac_batch_touched_values: null argument");
+    std::vector<int64_t> off(b->grids.size() + 1, 0);
+    for (size_t g = 0; g < b->grids.size(); ++g) off[g + 1] = off[g] + (b->grids[g]->count + 31) / 32;
+    uint32_t* bits = nullptr;
+    int64_t* d_off = nullptr;
+    unsigned long long* d_cnt = nullptr;
+    int rc = dalloc(&bits, (size_t)std::max<int64_t>(off.back(), 1));
+    if (!rc) rc = dalloc(&d_off, off.size());
+    if (!rc) rc = dalloc(&d_cnt, 1);
+    hipError_t e = hipSuccess;
+    if (!rc) e = hipMemset(bits, 0, sizeof(uint32_t) * std::max<int64_t>(off.back(), 1));
+    if (!rc && e == hipSuccess) e = hipMemset(d_cnt, 0, sizeof(unsigned long long));
+    if (!rc && e == hipSuccess) e = hipMemcpy(d_off, off.data(), sizeof(int64_t) * off.size(), hipMemcpyHostToDevice);
+    if (!rc && e == hipSuccess) {
+        TouchedArgs ta{};
+        ta.inst = b->d_inst; ta.grids = b->d_grids; ta.bits = bits; ta.bit_offset = d_off;
+        e = launch_touched(ta, (int)b->inst.size(), nullptr);
+        if (e == hipSuccess) e = launch_popcount(bits, off.back(), d_cnt, nullptr);
+        unsigned long long c = 0;
+        if (e == hipSuccess) e = hipMemcpy(&c, d_cnt, sizeof c, hipMemcpyDeviceToHost);
+        *count = (int64_t)c;
+    }
+    dfree(bits); dfree(d_off); dfree(d_cnt);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(ATAC_ERR_HIP, hipGetErrorString(e));
+    return ATAC_OK;
+}
+
+int atac_eval_plain(atac_batch b, uint32_t flags, const atac_outputs* out, void* stream) {
+    if (!b || !out || !out->fm || !out->n_active) return fail(ATAC_ERR_INVALID, "atac_eval_plain: fm and n_active are required");
+    if (flags & ~(ATAC_EVAL_DERIV | ATAC_EVAL_P6 | ATAC_EVAL_PRECISE)) return fail(ATAC_ERR_INVALID, "atac_eval_plain: unsupported flag");
+    const bool deriv = (flags & ATAC_EVAL_DERIV) != 0;
+    if (deriv && (!out->Km || !out->Dm)) return fail(ATAC_ERR_INVALID, "atac_eval_plain: Km and Dm are required with ATAC_EVAL_DERIV");
+    hipStream_t st = (hipStream_t)stream;
+    const int n = (int)b->inst.size();
+    ATAC_HIP(launch_zero(out->fm, (int64_t)n * 12 * 2, st));
+    ATAC_HIP(launch_zero(out->n_active, n, st));
+    if (out->dfm_dp6) ATAC_HIP(launch_zero(out->dfm_dp6, (int64_t)n * 12 * 2, st));
+    if (deriv) {
+        ATAC_HIP(launch_zero(out->Km, (int64_t)n * 144 * 2, st));
+        ATAC_HIP(launch_zero(out->Dm, (int64_t)n * 144 * 2, st));
+    }
+    for (int i = 0; i < n; ++i) {
+        PlainArgs pa{};
+        pa.inst = b->d_inst; pa.i = i; pa.grids = b->d_grids;
+        pa.fm = out->fm + 12 * (int64_t)i;
+        pa.p6 = out->dfm_dp6 ? out->dfm_dp6 + 12 * (int64_t)i : nullptr;
+        pa.Km = deriv ? out->Km + 144 * (int64_t)i : nullptr;
+        pa.Dm = deriv ? out->Dm + 144 * (int64_t)i : nullptr;
+        pa.n_active = out->n_active + i;
+        ATAC_HIP(launch_plain(pa, b->inst_dev[i].pt_count, (flags & ATAC_EVAL_PRECISE) != 0, st));
+    }
+    return ATAC_OK;
+}
+
+// ---------------------------------------------------------------- per-point API
+static PointArgs point_args(atac_grid g, const double E2[16], const double* phi2, double col_th) {
+    PointArgs a{};
+    a.grid = g->dev;
+    InstDev& d = a.in;
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) { d.R1[3 * r + c] = (r == c) ? 1.0 : 0.0; d.R2[3 * r + c] = E2[4 * r + c]; }
+        d.p1[r] = 0.0;
+        d.p2[r] = E2[4 * r + 3];
+        d.w1[r] = d.v1[r] = 0.0;
+        d.w2[r] = phi2 ? phi2[r] : 0.0;
+        d.v2[r] = phi2 ? phi2[3 + r] : 0.0;
+    }
+    d.col_th = col_th;
+    d.contact_scale = 1.0;
+    d.kn_chain = 1.0;
+    return a;
+}
+
+int atac_sdf_distance(atac_grid g, const double E2[16], double col_th, const double* xw, int64_t n, double* d, void* stream) {
+    if (!g || !E2 || n < 0 || (n > 0 && (!xw || !d))) return fail(ATAC_ERR_INVALID, "atac_sdf_distance: bad arguments");
+    PointArgs a = point_args(g, E2, nullptr, col_th);
+    a.xw = xw; a.n = n; a.out = d;
+    ATAC_HIP(launch_distance(a, (hipStream_t)stream));
+    return ATAC_OK;
+}
+
+int atac_sdf_collision(atac_grid g, const double E2[16], const double phi2[6], double col_th, uint32_t flags,
+                       const double* xw, const double* xw_dot, int64_t n, double* values, double* derivs, void* stream) {
+    if (!g || !E2 || !phi2 || n < 0 || (n > 0 && (!xw || !xw_dot || !values)))
+        return fail(ATAC_ERR_INVALID, "atac_sdf_collision: bad arguments");
+    if (flags & ~ATAC_EVAL_PRECISE) return fail(ATAC_ERR_INVALID, "atac_sdf_collision: only ATAC_EVAL_PRECISE is accepted");
+    PointArgs a = point_args(g, E2, phi2, col_th);
+    a.xw = xw; a.xwdot = xw_dot; a.n = n; a.out = values; a.out2 = derivs;
+    ATAC_HIP(launch_collision(a, (flags & ATAC_EVAL_PRECISE) != 0, (hipStream_t)stream));
+    return ATAC_OK;
+}
+
+int atac_batch_timing(atac_batch b, int32_t enable, double* ms_sum, int32_t* launches) {
+    if (!b) return fail(ATAC_ERR_INVALID, "atac_batch_timing: null batch");
+    if (ms_sum || launches) {
+        for (int s = 0; s < ST_COUNT; ++s) {
+            if (ms_sum) ms_sum[s] = 0.0;
+            if (launches) launches[s] = 0;
+        }
+        for (auto& u : b->ev_used) {
+            ATAC_HIP(hipEventSynchronize(u.second.second));
+            float ms = 0.0f;
+            ATAC_HIP(hipEventElapsedTime(&ms, u.second.first, u.second.second));
+            if (ms_sum) ms_sum[u.first] += ms;
+            if (launches) launches[u.first] += 1;
+        }
+    }
+    for (auto& u : b->ev_used) { b->ev_pool.push_back(u.second.first); b->ev_pool.push_back(u.second.second); }
+    b->ev_used.clear();
+    b->timing = enable != 0;
+    return ATAC_OK;
+}
+
+}  // extern "C"

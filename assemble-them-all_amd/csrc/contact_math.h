@@ -1,0 +1,584 @@
+// Per-point and per-contact arithmetic of the point-vs-SDF contact path.
+//
+// Everything here is __host__ __device__ so the same code runs inside the gfx950
+// kernels and in the host-side check library the CPU tests load. The functions
+// restate, in their own form, the reference routines:
+//   trilinear query / FD gradient / FD Hessian   Body/BodySDFObj.cpp:123-205
+//   collision values                             Body/BodySDFObj.cpp:239-274
+//   collision derivatives                        Body/BodySDFObj.cpp:278-377
+//   penalty + damping + friction wrench          Force/ForceGeneralSDFContact.cpp:147-219
+//   Km / Dm (as a tangent map)                   Force/ForceGeneralSDFContact.cpp:221-447
+//   contact-scale column of dfm_dp               Force/ForceGeneralSDFContact.cpp:693-699 (+ friction p6 lines)
+//
+// SDF sampling has two paths that agree in exact arithmetic:
+//  * fast (FP32): when every point of the reference's FD stencil (x +- 1e-5 along
+//    each axis, and the half-step mixed stencil) lies in the cell of x, the
+//    central difference of the trilinear interpolant IS its partial derivative
+//    and the mixed stencil IS the bilinear cross term, while the diagonal
+//    stencil is exactly 0. These are read off the 8 corners in FP32.
+//  * slow (FP64): otherwise (x within 1e-5 of a cell face or of the box) the
+//    stencils are evaluated operation for operation in FP64, reproducing the
+//    reference's blend of neighbouring cells and its outside-the-box rule.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cfloat>
+#include <cmath>
+#include "atac_types.h"
+
+#define ATAC_HD __host__ __device__ __forceinline__
+
+namespace atac {
+
+// ------------------------------------------------------------------ small vectors
+template <typename T>
+struct v3 {
+    T x, y, z;
+};
+template <typename T> ATAC_HD v3<T> mk3(T x, T y, T z) { return v3<T>{x, y, z}; }
+template <typename T> ATAC_HD v3<T> operator+(v3<T> a, v3<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <typename T> ATAC_HD v3<T> operator-(v3<T> a, v3<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <typename T> ATAC_HD v3<T> operator-(v3<T> a) { return {-a.x, -a.y, -a.z}; }
+template <typename T> ATAC_HD v3<T> operator*(T s, v3<T> a) { return {s * a.x, s * a.y, s * a.z}; }
+template <typename T> ATAC_HD v3<T> operator*(v3<T> a, T s) { return {s * a.x, s * a.y, s * a.z}; }
+template <typename T> ATAC_HD v3<T>& operator+=(v3<T>& a, v3<T> b) { a.x += b.x; a.y += b.y; a.z += b.z; return a; }
+template <typename T> ATAC_HD T dot(v3<T> a, v3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <typename T> ATAC_HD v3<T> cross(v3<T> a, v3<T> b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <typename T> ATAC_HD T norm(v3<T> a) { return sqrt(dot(a, a)); }
+template <typename T> ATAC_HD v3<T> zero3() { return {T(0), T(0), T(0)}; }
+
+// 3x3, row-major
+template <typename T>
+struct m3 {
+    T a[9];
+};
+template <typename T> ATAC_HD v3<T> mv(const m3<T>& M, v3<T> v) {
+    return {M.a[0] * v.x + M.a[1] * v.y + M.a[2] * v.z, M.a[3] * v.x + M.a[4] * v.y + M.a[5] * v.z,
+            M.a[6] * v.x + M.a[7] * v.y + M.a[8] * v.z};
+}
+template <typename T> ATAC_HD v3<T> mtv(const m3<T>& M, v3<T> v) {
+    return {M.a[0] * v.x + M.a[3] * v.y + M.a[6] * v.z, M.a[1] * v.x + M.a[4] * v.y + M.a[7] * v.z,
+            M.a[2] * v.x + M.a[5] * v.y + M.a[8] * v.z};
+}
+// symmetric 3x3 stored as xx yy zz xy xz yz
+template <typename T> ATAC_HD v3<T> symv(const T H[6], v3<T> v) {
+    return {H[0] * v.x + H[3] * v.y + H[4] * v.z, H[3] * v.x + H[1] * v.y + H[5] * v.z,
+            H[4] * v.x + H[5] * v.y + H[2] * v.z};
+}
+
+// ------------------------------------------------------------------ SDF sampling
+constexpr double kFdEps = 1e-5;        // BodySDFObj.cpp:178,191
+constexpr double kCellMargin = 1e-9;   // cell units kept clear of the stencil edge
+
+ATAC_HD float lerpf(float a, float b, float t) { return (1.0f - t) * a + t * b; }
+ATAC_HD double lerpd(double a, double b, double t) { return (1.0 - t) * a + t * b; }
+
+ATAC_HD int64_t grid_index(const GridDev& G, int i, int j, int k) {
+    return (int64_t)i + (int64_t)G.n[0] * ((int64_t)j + (int64_t)G.n[1] * (int64_t)k);
+}
+
+// Trilinear interpolation at x (SDF body frame) in FP64, in the reference's
+// i -> j -> k order and with its i_pos clamp at the last node
+// (BodySDFObj.cpp:130-155). Caller guarantees x is inside [min_box, max_box].
+ATAC_HD double trilinear64_inside(const GridDev& G, double x0, double x1, double x2) {
+    double ci = (x0 - G.minb[0]) / G.d[0], cj = (x1 - G.minb[1]) / G.d[1], ck = (x2 - G.minb[2]) / G.d[2];
+    int i = (int)floor(ci), j = (int)floor(cj), k = (int)floor(ck);
+    ci = ci - i; cj = cj - j; ck = ck - k;
+    i = i < 0 ? 0 : (i > G.n[0] - 1 ? G.n[0] - 1 : i);
+    j = j < 0 ? 0 : (j > G.n[1] - 1 ? G.n[1] - 1 : j);
+    k = k < 0 ? 0 : (k > G.n[2] - 1 ? G.n[2] - 1 : k);
+    int ip = (i == G.n[0] - 1) ? i : i + 1;
+    int jp = (j == G.n[1] - 1) ? j : j + 1;
+    int kp = (k == G.n[2] - 1) ? k : k + 1;
+    const float* v = G.v;
+    double a = lerpd((double)v[grid_index(G, i, j, k)], (double)v[grid_index(G, ip, j, k)], ci);
+    double b = lerpd((double)v[grid_index(G, i, jp, k)], (double)v[grid_index(G, ip, jp, k)], ci);
+    double c = lerpd((double)v[grid_index(G, i, j, kp)], (double)v[grid_index(G, ip, j, kp)], ci);
+    double d = lerpd((double)v[grid_index(G, i, jp, kp)], (double)v[grid_index(G, ip, jp, kp)], ci);
+    return lerpd(lerpd(a, b, cj), lerpd(c, d, cj), ck);
+}
+
+// query_SDF(x, outside_accurate) in FP64 (BodySDFObj.cpp:123-174)
+ATAC_HD double query_sdf64(const GridDev& G, double x0, double x1, double x2, bool outside_accurate) {
+    bool inside = x0 >= G.minb[0] && x0 <= G.maxb[0] && x1 >= G.minb[1] && x1 <= G.maxb[1] &&
+                  x2 >= G.minb[2] && x2 <= G.maxb[2];
+    if (inside) return trilinear64_inside(G, x0, x1, x2);
+    if (!outside_accurate) return (double)FLT_MAX;
+    double b0 = fmin(fmax(x0, G.minb[0]), G.maxb[0]);
+    double b1 = fmin(fmax(x1, G.minb[1]), G.maxb[1]);
+    double b2 = fmin(fmax(x2, G.minb[2]), G.maxb[2]);
+    double e0 = x0 - b0, e1 = x1 - b1, e2 = x2 - b2;
+    return sqrt(e0 * e0 + e1 * e1 + e2 * e2) + trilinear64_inside(G, b0, b1, b2);
+}
+
+// FP64 value, FD gradient and FD Hessian at x, operation for operation as
+// query_SDF / query_dSDF / query_ddSDF (BodySDFObj.cpp:123-205); the six
+// axis-stencil values are shared between gradient and Hessian diagonal, as they
+// are bitwise the same evaluations in the reference.
+ATAC_HD void sdf_stencil64(const GridDev& G, const double x[3], double& val, double g[3], double H[6]) {
+    const double h = kFdEps, hh = kFdEps / 2;
+    val = query_sdf64(G, x[0], x[1], x[2], false);
+    double xs = query_sdf64(G, x[0], x[1], x[2], true);
+    double px = query_sdf64(G, x[0] + h, x[1], x[2], true), mx = query_sdf64(G, x[0] - h, x[1], x[2], true);
+    double py = query_sdf64(G, x[0], x[1] + h, x[2], true), my = query_sdf64(G, x[0], x[1] - h, x[2], true);
+    double pz = query_sdf64(G, x[0], x[1], x[2] + h, true), mz = query_sdf64(G, x[0], x[1], x[2] - h, true);
+    g[0] = (px - mx) / (2 * h);
+    g[1] = (py - my) / (2 * h);
+    g[2] = (pz - mz) / (2 * h);
+    const double inv = 1.0 / (h * h);
+    H[0] = (px + mx - 2 * xs) / (h * h);
+    H[1] = (py + my - 2 * xs) / (h * h);
+    H[2] = (pz + mz - 2 * xs) / (h * h);
+    H[3] = (query_sdf64(G, (x[0] + hh), (x[1] + hh), x[2], true) + query_sdf64(G, (x[0] - hh), (x[1] - hh), x[2], true) -
+            query_sdf64(G, (x[0] + hh), (x[1] - hh), x[2], true) - query_sdf64(G, (x[0] - hh), (x[1] + hh), x[2], true)) /
+           (h * h);
+    H[4] = (query_sdf64(G, (x[0] + hh), x[1], (x[2] + hh), true) + query_sdf64(G, (x[0] - hh), x[1], (x[2] - hh), true) -
+            query_sdf64(G, (x[0] + hh), x[1], (x[2] - hh), true) - query_sdf64(G, (x[0] - hh), x[1], (x[2] + hh), true)) /
+           (h * h);
+    H[5] = (query_sdf64(G, x[0], (x[1] + hh), (x[2] + hh), true) + query_sdf64(G, x[0], (x[1] - hh), (x[2] - hh), true) -
+            query_sdf64(G, x[0], (x[1] + hh), (x[2] - hh), true) - query_sdf64(G, x[0], (x[1] - hh), (x[2] + hh), true)) /
+           (h * h);
+    (void)inv;
+}
+
+// Cell of x when the whole FD stencil stays inside it (fast path), else false.
+// f: the fractions in FP32, fd: the same in FP64.
+ATAC_HD bool fast_cell(const GridDev& G, const double x[3], int idx[3], float f[3], double fd[3]) {
+    const double h = kFdEps;
+    for (int a = 0; a < 3; ++a) {
+        if (!(x[a] - h >= G.minb[a] && x[a] + h <= G.maxb[a])) return false;
+        double c = (x[a] - G.minb[a]) / G.d[a];
+        double fl = floor(c);
+        if (!(fl >= 0.0 && fl <= (double)(G.n[a] - 2))) return false;
+        double fr = c - fl;
+        double m = h * G.inv_d[a] + kCellMargin;
+        if (fr < m || fr > 1.0 - m) return false;
+        idx[a] = (int)fl;
+        f[a] = (float)fr;
+        fd[a] = fr;
+    }
+    return true;
+}
+
+
+struct Corners {
+    float v000, v100, v010, v110, v001, v101, v011, v111;
+};
+
+ATAC_HD Corners load_corners(const GridDev& G, int i, int j, int k) {
+    const float* v = G.v;
+    int64_t b = grid_index(G, i, j, k);
+    int64_t sj = G.n[0], sk = (int64_t)G.n[0] * G.n[1];
+    Corners c;
+    c.v000 = v[b];
+    c.v100 = v[b + 1];
+    c.v010 = v[b + sj];
+    c.v110 = v[b + sj + 1];
+    c.v001 = v[b + sk];
+    c.v101 = v[b + sk + 1];
+    c.v011 = v[b + sk + sj];
+    c.v111 = v[b + sk + sj + 1];
+    return c;
+}
+
+// FP64 trilinear value of a cell from its FP32 corners, in the reference's
+// i -> j -> k order (BodySDFObj.cpp:143-153).
+ATAC_HD double cell_value64(const Corners& c, const double f[3]) {
+    const double a = lerpd((double)c.v000, (double)c.v100, f[0]), b = lerpd((double)c.v010, (double)c.v110, f[0]);
+    const double cc = lerpd((double)c.v001, (double)c.v101, f[0]), d = lerpd((double)c.v011, (double)c.v111, f[0]);
+    return lerpd(lerpd(a, b, f[1]), lerpd(cc, d, f[1]), f[2]);
+}
+
+// FP32 value, closed-form gradient and mixed partials of the trilinear cell.
+ATAC_HD void sdf_cell32(const GridDev& G, const Corners& c, const float f[3], float& val, float g[3], float H[6]) {
+    const float fx = f[0], fy = f[1], fz = f[2];
+    // value in the reference's i -> j -> k order
+    float a = lerpf(c.v000, c.v100, fx), b = lerpf(c.v010, c.v110, fx);
+    float cc = lerpf(c.v001, c.v101, fx), d = lerpf(c.v011, c.v111, fx);
+    val = lerpf(lerpf(a, b, fy), lerpf(cc, d, fy), fz);
+    const float idx = (float)G.inv_d[0], idy = (float)G.inv_d[1], idz = (float)G.inv_d[2];
+    // axis differences
+    float x00 = c.v100 - c.v000, x10 = c.v110 - c.v010, x01 = c.v101 - c.v001, x11 = c.v111 - c.v011;
+    float y00 = c.v010 - c.v000, y10 = c.v110 - c.v100, y01 = c.v011 - c.v001, y11 = c.v111 - c.v101;
+    float z00 = c.v001 - c.v000, z10 = c.v101 - c.v100, z01 = c.v011 - c.v010, z11 = c.v111 - c.v110;
+    g[0] = lerpf(lerpf(x00, x10, fy), lerpf(x01, x11, fy), fz) * idx;
+    g[1] = lerpf(lerpf(y00, y10, fx), lerpf(y01, y11, fx), fz) * idy;
+    g[2] = lerpf(lerpf(z00, z10, fx), lerpf(z01, z11, fx), fy) * idz;
+    H[0] = 0.0f;
+    H[1] = 0.0f;
+    H[2] = 0.0f;
+    H[3] = lerpf(x10 - x00, x11 - x01, fz) * (idx * idy);
+    H[4] = lerpf(x01 - x00, x11 - x10, fy) * (idx * idz);
+    H[5] = lerpf(y01 - y00, y11 - y10, fx) * (idy * idz);
+}
+
+// Detection test of one point, collision_detection_general_SDF's loop body
+// (CollisionDetection.cpp:135-141 -> BodySDFObj::distance, BodySDFObj.cpp:207-213):
+// the point maps into grid index space through the FP64 affine map
+// c = A xi + b, the cell and fraction are taken in FP64, the 8 corners and the
+// trilinear value in FP32. Returns the value query_SDF(x) + col_th, or +inf when
+// x is outside [min_box, max_box] (the reference returns FLT_MAX + col_th).
+ATAC_HD float detect_value(const GridDev& G, const InstDev& in, float px, float py, float pz) {
+    const double x = px, y = py, z = pz;
+    const double c0 = in.A[0] * x + in.A[1] * y + in.A[2] * z + in.b[0];
+    const double c1 = in.A[3] * x + in.A[4] * y + in.A[5] * z + in.b[1];
+    const double c2 = in.A[6] * x + in.A[7] * y + in.A[8] * z + in.b[2];
+    const bool inside = c0 >= 0.0 && c0 <= in.cmax[0] && c1 >= 0.0 && c1 <= in.cmax[1] && c2 >= 0.0 && c2 <= in.cmax[2];
+    if (!inside) return INFINITY;
+    int i = (int)c0, j = (int)c1, k = (int)c2;  // c >= 0: truncation is floor
+    i = i < G.n[0] - 1 ? i : G.n[0] - 1;
+    j = j < G.n[1] - 1 ? j : G.n[1] - 1;
+    k = k < G.n[2] - 1 ? k : G.n[2] - 1;
+    const float fx = (float)(c0 - (double)i), fy = (float)(c1 - (double)j), fz = (float)(c2 - (double)k);
+    const int64_t sj = G.n[0], sk = (int64_t)G.n[0] * G.n[1];
+    const int64_t di = i < G.n[0] - 1 ? 1 : 0;
+    const int64_t dj = j < G.n[1] - 1 ? sj : 0;
+    const int64_t dk = k < G.n[2] - 1 ? sk : 0;
+    const float* v = G.v + ((int64_t)i + sj * j + sk * k);
+    const float a = lerpf(v[0], v[di], fx), b = lerpf(v[dj], v[dj + di], fx);
+    const float c = lerpf(v[dk], v[dk + di], fx), d = lerpf(v[dk + dj], v[dk + dj + di], fx);
+    return lerpf(lerpf(a, b, fy), lerpf(c, d, fy), fz) + (float)in.col_th;
+}
+
+// ------------------------------------------------------------------ contact math
+template <typename R>
+struct InstR {
+    m3<R> R1, R2;
+    v3<R> w1, v1, w2, v2;
+    R kn, kt, mu, c, cs;
+};
+
+template <typename R>
+ATAC_HD InstR<R> make_inst(const InstDev& in) {
+    InstR<R> I;
+    for (int k = 0; k < 9; ++k) { I.R1.a[k] = (R)in.R1[k]; I.R2.a[k] = (R)in.R2[k]; }
+    I.w1 = mk3<R>((R)in.w1[0], (R)in.w1[1], (R)in.w1[2]);
+    I.v1 = mk3<R>((R)in.v1[0], (R)in.v1[1], (R)in.v1[2]);
+    I.w2 = mk3<R>((R)in.w2[0], (R)in.w2[1], (R)in.w2[2]);
+    I.v2 = mk3<R>((R)in.v2[0], (R)in.v2[1], (R)in.v2[2]);
+    I.kn = (R)in.kn; I.kt = (R)in.kt; I.mu = (R)in.mu; I.c = (R)in.damping; I.cs = (R)in.contact_scale;
+    return I;
+}
+
+// Forward state of one active contact (values of BodySDFObj::collision and of
+// the force loop body, plus what the tangent and adjoint maps reuse).
+template <typename R>
+struct ContactFwd {
+    v3<R> xi1, x;       // contact point (body 1 frame), its SDF-body-frame position
+    R d;                // query_SDF(x) + col_th
+    v3<R> g;            // grad phi (not normalised)
+    R H[6];             // Hessian of phi (xx yy zz xy xz yz)
+    v3<R> u1, xw1dot, e, n, xi2, a2, xdot, s2, vw, tdot, n1, ni1a, ni2a, t1, t2, T1a, T2a;
+    R gn, ddot, lam, fcn, tn, rho;
+    bool fric, stat;
+    bool slow;
+    R f[12];            // wrench contribution [tau1 f1 tau2 f2]
+};
+
+// Sample phi at the contact point xi1 (FP32 as stored) of instance `in`.
+// x is formed as the reference does: xw1 = R1 xi1 + p1 (CollisionDetection.cpp:136),
+// x = R2^T (xw1 - p2) (BodySDFObj.cpp:251).
+template <typename R>
+ATAC_HD void contact_sample(const GridDev& G, const InstDev& in, double px, double py, double pz, ContactFwd<R>& s,
+                           const CkptDev* ck = nullptr) {
+    double xi[3] = {px, py, pz};
+    double xw[3], x[3];
+    for (int r = 0; r < 3; ++r) xw[r] = in.R1[3 * r] * xi[0] + in.R1[3 * r + 1] * xi[1] + in.R1[3 * r + 2] * xi[2] + in.p1[r];
+    double q0 = xw[0] - in.p2[0], q1 = xw[1] - in.p2[1], q2 = xw[2] - in.p2[2];
+    for (int r = 0; r < 3; ++r) x[r] = in.R2[r] * q0 + in.R2[3 + r] * q1 + in.R2[6 + r] * q2;
+    s.xi1 = mk3<R>((R)xi[0], (R)xi[1], (R)xi[2]);
+    s.x = mk3<R>((R)x[0], (R)x[1], (R)x[2]);
+    int idx[3];
+    float f[3];
+    double fd[3];
+    if (fast_cell(G, x, idx, f, fd)) {
+        Corners c = load_corners(G, idx[0], idx[1], idx[2]);
+        float val, g[3], H[6];
+        sdf_cell32(G, c, f, val, g, H);
+        // the penetration depth of an active contact is taken in FP64 from the
+        // same corners: the FP32 interpolant's ~1e-8 absolute error would be
+        // ~1e-5 of a shallow contact's depth
+        double dval = cell_value64(c, fd) + in.col_th;
+        if (ck) {  // forward checkpoint: the same values, read back
+            dval = (double)ck->d_hi + (double)ck->d_lo;
+            g[0] = ck->g0; g[1] = ck->g1; g[2] = ck->g2;
+        }
+        s.d = (R)dval;
+        s.g = mk3<R>((R)g[0], (R)g[1], (R)g[2]);
+        for (int k = 0; k < 6; ++k) s.H[k] = (R)H[k];
+        s.slow = false;
+    } else {
+        double val, g[3], H[6];
+        sdf_stencil64(G, x, val, g, H);
+        s.d = (R)(val + in.col_th);
+        s.g = mk3<R>((R)g[0], (R)g[1], (R)g[2]);
+        for (int k = 0; k < 6; ++k) s.H[k] = (R)H[k];
+        s.slow = true;
+    }
+}
+
+// Values of BodySDFObj::collision (BodySDFObj.cpp:251-273) given s.x, s.d, s.g
+// and the world velocity s.xw1dot of the contact point.
+template <typename R>
+ATAC_HD void collision_forward(const InstR<R>& I, ContactFwd<R>& s) {
+    s.gn = norm(s.g);
+    s.e = s.g * ((R)1 / s.gn);
+    s.n = mv(I.R2, s.e);
+    s.xi2 = s.x - s.d * s.e;
+    s.a2 = mtv(I.R2, s.xw1dot);
+    s.xdot = cross(s.x, I.w2) + s.a2 - I.v2;       // skew(w2)^T x + R2^T xw_dot - v2
+    s.ddot = dot(s.g, s.xdot);
+    s.s2 = cross(I.w2, s.xi2) + I.v2;
+    s.vw = s.xw1dot - mv(I.R2, s.s2);
+    s.tdot = s.vw - s.n * dot(s.n, s.vw);          // (I - n n^T) vw
+}
+
+// Wrench of one contact (ForceGeneralSDFContact.cpp:157-218 with
+// BodySDFObj::collision, :239-274). Writes s.f and, when p6 != nullptr, the
+// reference's contact-scale column contribution.
+template <typename R>
+ATAC_HD void contact_forward(const InstR<R>& I, ContactFwd<R>& s, R* p6) {
+    const R eps = (R)1e-8;  // constants::eps, Utils.h:9
+    s.u1 = cross(I.w1, s.xi1) + I.v1;              // skew(xi1)^T w1 + v1
+    s.xw1dot = mv(I.R1, s.u1);
+    collision_forward(I, s);
+    s.n1 = mtv(I.R1, s.n);
+    s.ni1a = cross(s.xi1, s.n1);                   // GTRT1 n = [xi1 x n1; n1]
+    s.ni2a = cross(s.xi2, s.e);                    // GTRT2 n = [xi2 x e; e]
+    s.lam = -I.kn * s.d + I.c * s.ddot * s.d;
+    s.f[0] = s.lam * s.ni1a.x; s.f[1] = s.lam * s.ni1a.y; s.f[2] = s.lam * s.ni1a.z;
+    s.f[3] = s.lam * s.n1.x;   s.f[4] = s.lam * s.n1.y;   s.f[5] = s.lam * s.n1.z;
+    s.f[6] = -s.lam * s.ni2a.x; s.f[7] = -s.lam * s.ni2a.y; s.f[8] = -s.lam * s.ni2a.z;
+    s.f[9] = -s.lam * s.e.x;    s.f[10] = -s.lam * s.e.y;   s.f[11] = -s.lam * s.e.z;
+    if (p6) {
+        const R k = I.kn * s.d / I.cs;
+        p6[0] = -k * s.ni1a.x; p6[1] = -k * s.ni1a.y; p6[2] = -k * s.ni1a.z;
+        p6[3] = -k * s.n1.x;   p6[4] = -k * s.n1.y;   p6[5] = -k * s.n1.z;
+        p6[6] = k * s.ni2a.x;  p6[7] = k * s.ni2a.y;  p6[8] = k * s.ni2a.z;
+        p6[9] = k * s.e.x;     p6[10] = k * s.e.y;    p6[11] = k * s.e.z;
+    }
+    s.fric = I.mu > eps;
+    s.stat = true;
+    s.fcn = s.tn = s.rho = (R)0;
+    if (s.fric) {
+        R ss = 0;
+        for (int k = 0; k < 6; ++k) ss += s.f[k] * s.f[k];
+        s.fcn = sqrt(ss);                          // fc1.norm()
+        s.tn = norm(s.tdot);
+        s.t1 = mtv(I.R1, s.tdot);
+        s.t2 = mtv(I.R2, s.tdot);
+        s.T1a = cross(s.xi1, s.t1);
+        s.T2a = cross(s.xi2, s.t2);
+        s.stat = I.mu * s.fcn >= I.kt * s.tn - eps;
+        R a1, a2;                                  // friction = -a1 [T1a; t1], +a2 [T2a; t2]
+        if (s.stat) {
+            a1 = I.kt;
+            a2 = I.kt;
+        } else {
+            s.rho = s.fcn / s.tn;
+            a1 = I.mu * s.rho;
+            a2 = I.mu * s.rho;
+        }
+        s.f[0] -= a1 * s.T1a.x; s.f[1] -= a1 * s.T1a.y; s.f[2] -= a1 * s.T1a.z;
+        s.f[3] -= a1 * s.t1.x;  s.f[4] -= a1 * s.t1.y;  s.f[5] -= a1 * s.t1.z;
+        s.f[6] += a2 * s.T2a.x; s.f[7] += a2 * s.T2a.y; s.f[8] += a2 * s.T2a.z;
+        s.f[9] += a2 * s.t2.x;  s.f[10] += a2 * s.t2.y; s.f[11] += a2 * s.t2.z;
+        if (p6) {
+            const R b1 = a1 / I.cs, b2 = a2 / I.cs;
+            p6[0] -= b1 * s.T1a.x; p6[1] -= b1 * s.T1a.y; p6[2] -= b1 * s.T1a.z;
+            p6[3] -= b1 * s.t1.x;  p6[4] -= b1 * s.t1.y;  p6[5] -= b1 * s.t1.z;
+            p6[6] += b2 * s.T2a.x; p6[7] += b2 * s.T2a.y; p6[8] += b2 * s.T2a.z;
+            p6[9] += b2 * s.t2.x;  p6[10] += b2 * s.t2.y; p6[11] += b2 * s.t2.z;
+        }
+    }
+}
+
+// Perturbation direction: right-multiplied pose twists E <- E exp([om; v]) of
+// both bodies (the convention of every reference FD test, e.g.
+// ForceGeneralSDFContact.cpp:1066), body-frame twist increments, and kn.
+template <typename R>
+struct Tangent {
+    v3<R> om1, v1, om2, v2, w1p, v1p, w2p, v2p;
+    R kn;
+};
+
+// Column j of [Km | Dm] (j < 12: q1 then q2; 12 <= j < 24: phi1 then phi2).
+template <typename R>
+ATAC_HD Tangent<R> unit_tangent(int j) {
+    Tangent<R> t;
+    t.om1 = t.v1 = t.om2 = t.v2 = t.w1p = t.v1p = t.w2p = t.v2p = zero3<R>();
+    t.kn = 0;
+    v3<R>* slots[8] = {&t.om1, &t.v1, &t.om2, &t.v2, &t.w1p, &t.v1p, &t.w2p, &t.v2p};
+    v3<R>& s = *slots[j / 3];
+    int c = j % 3;
+    if (c == 0) s.x = 1; else if (c == 1) s.y = 1; else s.z = 1;
+    return t;
+}
+
+// Tangents of the collision values (the 14 derivative blocks of
+// BodySDFObj::collision, BodySDFObj.cpp:314-376, applied to one direction).
+template <typename R>
+struct CollTangent {
+    R dd, dddot;
+    v3<R> de, dn, dxi2, dtdot;
+};
+
+template <typename R>
+ATAC_HD void collision_jvp(const InstR<R>& I, const ContactFwd<R>& s, v3<R> dxw, v3<R> dxwdot, v3<R> om2, v3<R> v2,
+                           v3<R> w2p, v3<R> v2p, CollTangent<R>& o) {
+    v3<R> dx = mtv(I.R2, dxw) + cross(s.x, om2) - v2;          // dx_dxw, dx_dw2 = skew(x), dx_dv2 = -I
+    o.dd = dot(s.g, dx);
+    v3<R> dg = symv(s.H, dx);
+    o.de = (dg - s.e * dot(s.e, dg)) * ((R)1 / s.gn);           // de_ddd_dx * H * dx
+    o.dn = mv(I.R2, o.de + cross(om2, s.e));
+    o.dxi2 = dx - s.e * o.dd - s.d * o.de;
+    v3<R> dxdot = cross(dx, I.w2) + cross(s.x, w2p) + cross(s.a2, om2) + mtv(I.R2, dxwdot) - v2p;
+    o.dddot = dot(dg, s.xdot) + dot(s.g, dxdot);
+    v3<R> dxw2dot = mv(I.R2, cross(om2, s.s2) + cross(I.w2, o.dxi2) + cross(w2p, s.xi2) + v2p);
+    v3<R> dvw = dxwdot - dxw2dot;
+    o.dtdot = dvw - s.n * dot(s.n, dvw) - o.dn * dot(s.n, s.vw) - s.n * dot(s.vw, o.dn);
+}
+
+// Directional derivative of the contact wrench (the tangent form of the
+// reference's Km/Dm assembly, ForceGeneralSDFContact.cpp:240-446).
+template <typename R>
+ATAC_HD void contact_jvp(const InstR<R>& I, const ContactFwd<R>& s, const Tangent<R>& t, R df[12]) {
+    v3<R> dxw1 = mv(I.R1, cross(t.om1, s.xi1) + t.v1);                          // dxw1_dq1
+    v3<R> dxw1dot = mv(I.R1, cross(t.om1, s.u1) + cross(t.w1p, s.xi1) + t.v1p);  // dxw1dot_dq1, _dphi1
+    CollTangent<R> c;
+    collision_jvp(I, s, dxw1, dxw1dot, t.om2, t.v2, t.w2p, t.v2p, c);
+    const R dd = c.dd, dddot = c.dddot;
+    const v3<R> de = c.de, dn = c.dn, dxi2 = c.dxi2, dtdot = c.dtdot;
+    v3<R> dn1 = mtv(I.R1, dn) + cross(s.n1, t.om1);
+    R dlam = -t.kn * s.d - I.kn * dd + I.c * (dddot * s.d + s.ddot * dd);
+    v3<R> a1 = dlam * s.ni1a + s.lam * cross(s.xi1, dn1);
+    v3<R> l1 = dlam * s.n1 + s.lam * dn1;
+    v3<R> a2 = -(dlam * s.ni2a + s.lam * (cross(dxi2, s.e) + cross(s.xi2, de)));
+    v3<R> l2 = -(dlam * s.e + s.lam * de);
+    if (s.fric) {
+        v3<R> dt1 = mtv(I.R1, dtdot) + cross(s.t1, t.om1);
+        v3<R> dt2 = mtv(I.R2, dtdot) + cross(s.t2, t.om2);
+        v3<R> dT1a = cross(s.xi1, dt1);
+        v3<R> dT2a = cross(dxi2, s.t2) + cross(s.xi2, dt2);
+        if (s.stat) {
+            a1 = a1 - I.kt * dT1a; l1 = l1 - I.kt * dt1;
+            a2 = a2 + I.kt * dT2a; l2 = l2 + I.kt * dt2;
+        } else {
+            // fc1 = lam [ni1a; n1]; its tangent is [a1; l1] before friction is added
+            R fc_dot = s.lam * (dot(s.ni1a, a1) + dot(s.n1, l1));
+            R dfcn = fc_dot / s.fcn;
+            R dtn = dot(s.tdot, dtdot) / s.tn;
+            R drho = dfcn / s.tn - s.fcn * dtn / (s.tn * s.tn);
+            a1 = a1 - I.mu * (drho * s.T1a + s.rho * dT1a);
+            l1 = l1 - I.mu * (drho * s.t1 + s.rho * dt1);
+            a2 = a2 + I.mu * (drho * s.T2a + s.rho * dT2a);
+            l2 = l2 + I.mu * (drho * s.t2 + s.rho * dt2);
+        }
+    }
+    df[0] = a1.x; df[1] = a1.y; df[2] = a1.z; df[3] = l1.x; df[4] = l1.y; df[5] = l1.z;
+    df[6] = a2.x; df[7] = a2.y; df[8] = a2.z; df[9] = l2.x; df[10] = l2.y; df[11] = l2.z;
+}
+
+// Adjoint of contact_jvp: given gb = d(loss)/d(wrench), returns
+// out[0:6] = gb^T Km[:, q1], out[6:12] = gb^T Km[:, q2], out[12:18] = gb^T Dm[:, phi1],
+// out[18:24] = gb^T Dm[:, phi2], out[24] = gb^T d f / d kn, out[25] = gb^T d f / d kt
+// (kn, kt effective). Hand-written reverse sweep of the tangent map above.
+template <typename R>
+ATAC_HD void contact_vjp(const InstR<R>& I, const ContactFwd<R>& s, const R gb[12], R out[26]) {
+    const v3<R> f1a = mk3<R>(gb[0], gb[1], gb[2]), f1l = mk3<R>(gb[3], gb[4], gb[5]);
+    const v3<R> f2a = mk3<R>(gb[6], gb[7], gb[8]), f2l = mk3<R>(gb[9], gb[10], gb[11]);
+    v3<R> om1b = zero3<R>(), v1b = zero3<R>(), om2b = zero3<R>(), v2b = zero3<R>();
+    v3<R> w1pb = zero3<R>(), v1pb = zero3<R>(), w2pb = zero3<R>(), v2pb = zero3<R>();
+    v3<R> tdotb = zero3<R>(), xi2b = zero3<R>(), nb = zero3<R>(), eb = zero3<R>();
+    R knb = 0, ktb = 0;
+    v3<R> fc1ab = f1a, fc1lb = f1l;   // adjoint of the normal force on body 1
+    if (s.fric) {
+        v3<R> T1ab, t1b, T2ab, t2b;
+        if (s.stat) {
+            T1ab = -I.kt * f1a; t1b = -I.kt * f1l; T2ab = I.kt * f2a; t2b = I.kt * f2l;
+            ktb += -(dot(f1a, s.T1a) + dot(f1l, s.t1)) + dot(f2a, s.T2a) + dot(f2l, s.t2);
+        } else {
+            R rhob = -I.mu * (dot(f1a, s.T1a) + dot(f1l, s.t1)) + I.mu * (dot(f2a, s.T2a) + dot(f2l, s.t2));
+            T1ab = -I.mu * s.rho * f1a; t1b = -I.mu * s.rho * f1l;
+            T2ab = I.mu * s.rho * f2a;  t2b = I.mu * s.rho * f2l;
+            R fcnb = rhob / s.tn;
+            R tnb = -rhob * s.fcn / (s.tn * s.tn);
+            R k = fcnb * s.lam / s.fcn;            // fc1 = lam [ni1a; n1]
+            fc1ab += k * s.ni1a;
+            fc1lb += k * s.n1;
+            tdotb += (tnb / s.tn) * s.tdot;
+        }
+        t1b += cross(T1ab, s.xi1);
+        tdotb += mv(I.R1, t1b);
+        om1b += cross(t1b, s.t1);
+        xi2b += cross(s.t2, T2ab);
+        t2b += cross(T2ab, s.xi2);
+        tdotb += mv(I.R2, t2b);
+        om2b += cross(t2b, s.t2);
+    }
+    R lamb = dot(fc1ab, s.ni1a) + dot(fc1lb, s.n1) - dot(f2a, s.ni2a) - dot(f2l, s.e);
+    v3<R> n1b = s.lam * (cross(fc1ab, s.xi1) + fc1lb);
+    v3<R> ni2ab = -s.lam * f2a, ni2lb = -s.lam * f2l;
+    xi2b += cross(s.e, ni2ab);
+    eb += cross(ni2ab, s.xi2) + ni2lb;
+    nb += mv(I.R1, n1b);
+    om1b += cross(n1b, s.n1);
+    knb += -lamb * s.d;
+    R db = lamb * (-I.kn + I.c * s.ddot);
+    R ddotb = lamb * I.c * s.d;
+    v3<R> vwb = tdotb - s.n * dot(s.n, tdotb);
+    nb += -dot(s.n, s.vw) * tdotb - dot(s.n, tdotb) * s.vw;
+    v3<R> xw1dotb = vwb;
+    v3<R> z = mtv(I.R2, -vwb);                     // R2^T (adjoint of xw2dot)
+    om2b += cross(s.s2, z);
+    xi2b += cross(z, I.w2);
+    w2pb += cross(s.xi2, z);
+    v2pb += z;
+    v3<R> gtb = ddotb * s.xdot;
+    v3<R> xdotb = ddotb * s.g;
+    v3<R> xb = cross(I.w2, xdotb);
+    w2pb += cross(xdotb, s.x);
+    om2b += cross(xdotb, s.a2);
+    xw1dotb += mv(I.R2, xdotb);
+    v2pb += -xdotb;
+    xb += xi2b;
+    db += -dot(s.e, xi2b);
+    eb += -s.d * xi2b;
+    v3<R> m = mtv(I.R2, nb);
+    eb += m;
+    om2b += cross(s.e, m);
+    gtb += (eb - s.e * dot(s.e, eb)) * ((R)1 / s.gn);
+    xb += symv(s.H, gtb);
+    xb += db * s.g;
+    v3<R> xw1b = mv(I.R2, xb);
+    om2b += cross(xb, s.x);
+    v2b += -xb;
+    v3<R> y = mtv(I.R1, xw1dotb);
+    om1b += cross(s.u1, y);
+    w1pb += cross(s.xi1, y);
+    v1pb += y;
+    v3<R> y2 = mtv(I.R1, xw1b);
+    om1b += cross(s.xi1, y2);
+    v1b += y2;
+    const v3<R> o[8] = {om1b, v1b, om2b, v2b, w1pb, v1pb, w2pb, v2pb};
+    for (int k = 0; k < 8; ++k) { out[3 * k] = o[k].x; out[3 * k + 1] = o[k].y; out[3 * k + 2] = o[k].z; }
+    out[24] = knb;
+    out[25] = ktb;
+}
+
+// Neumaier-compensated FP32 accumulator
+struct Comp32 {
+    float s, c;
+    ATAC_HD void zero() { s = 0.0f; c = 0.0f; }
+    ATAC_HD void add(float v) {
+        float t = s + v;
+        float big = fabsf(s) >= fabsf(v) ? s : v;
+        float small = fabsf(s) >= fabsf(v) ? v : s;
+        c += (big - t) + small;
+        s = t;
+    }
+    ATAC_HD double value() const { return (double)s + (double)c; }
+};
+
+}  // namespace atac

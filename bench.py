@@ -1,0 +1,231 @@
+"""Contact-point SDF query throughput (force + adjoint) on one MI355X.
+
+A step is one pass of the batched contact path over the whole workload with the
+inputs resident in HBM: detection over every (instance, point) query, the contact
+wrench of the active points, per-instance FP64 reduction, then the hand-written
+adjoint (VJP) for a fixed synthetic cotangent g_fm. The step is replayed from a
+captured HIP graph. value = queries per second, a query being one
+(instance, contact point) pair.
+
+Default workload: 4,096 force objects, N ~ log-uniform[1k, 50k] contact points each
+(about 51M queries), 16 distinct 128^3 FP32 SDF grids (cylindrical-hole boxes,
+square-hole boxes, superquadric blobs), pose = nominal assembled pose + s*u with
+s ~ U[0.01, 0.1] along one of 26 cube directions; kn=1e4, damping=10, mu=0.3, kt=1e3.
+
+Printed as one JSON line, with the dominant kernel's HBM roofline (algorithmic bytes
+per launch over its HIP-event-timed launch duration), the FP64 CPU oracle on a
+bounded sample of the same workload, the error against FP64 on that sample and the
+plain per-instance GPU path for comparison.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "assemble-them-all_amd"))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "contact-point SDF queries/s (force+adjoint) at 1 GPU; % HBM roofline; err vs FP64"
+INST_DEV_BYTES = 512        # sizeof(InstDev)
+TILE_BYTES = 24 + 4 + 4     # TileDev + processing-order entry + tile_active write
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_workload(name):
+    from atac import workloads as W
+    if name == "batched":
+        return W.batched()
+    if name == "differentiated":
+        return W.differentiated()
+    if name == "peg_in_hole":
+        return W.peg_in_hole()
+    raise SystemExit(f"unknown workload {name}")
+
+
+def traffic_from_profiles(kernel="detect_kernel"):
+    """HBM bytes per launch of the kernel from the committed rocprofv3 PMC summary, or None."""
+    path = os.path.join(ROOT, "profiles", "detect_kernel_batched_pmc.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(w, gpu_fm, budget_s, threads):
+    """FP64 oracle (reference-mode restatement, computeForceWithDerivative per force
+    object) on the first instances of the workload, until about budget_s of CPU time."""
+    from oracle import oracle as O
+    from atac import workloads as W
+    grids = [O.GridRef.from_grid(g) for g in w.grids]
+    keep = []
+    t_total, q_total, n_done = 0.0, 0, 0
+    errs_sum, errs_net = [], []
+    chunk = 32
+    while n_done < len(w.instances) and t_total < budget_s:
+        insts = w.instances[n_done:n_done + chunk]
+        oi = []
+        for ins in insts:
+            pts = W.round_points_f32(w.point_sets[ins.points])
+            keep.append(pts)
+            oi.append(O.make_instance(ins.grid, pts, ins.E1, ins.phi1, ins.E2, ins.phi2, ins.col_th, ins.kn, ins.kt,
+                                      ins.mu, ins.damping, ins.scale, ins.contact_scale))
+        t0 = time.perf_counter()
+        r = O.eval_batch(grids, oi, O.FLAG_DERIV, threads=threads)
+        t_total += time.perf_counter() - t0
+        q_total += sum(len(w.point_sets[i.points]) for i in insts)
+        for k in range(len(insts)):
+            ref = r["fm"][k]
+            d = np.linalg.norm(gpu_fm[n_done + k] - ref)
+            nref = np.linalg.norm(ref)
+            if nref > 0:
+                errs_net.append(d / nref)
+        n_done += len(insts)
+    # per-contact-normalised error on a few instances (needs the per-contact scale)
+    for k in range(min(n_done, 64)):
+        ins = w.instances[k]
+        pts = W.round_points_f32(w.point_sets[ins.points])
+        res = O.eval_instance(grids, O.make_instance(ins.grid, pts, ins.E1, ins.phi1, ins.E2, ins.phi2, ins.col_th,
+                                                     ins.kn, ins.kt, ins.mu, ins.damping))
+        if res["n_active"]:
+            errs_sum.append(np.linalg.norm(gpu_fm[k] - res["fm"]) / res["scale_fm"])
+    return {
+        "value": q_total / t_total, "unit": "queries/s", "cores": threads, "kind": "port",
+        "sample": f"first {n_done} of {len(w.instances)} force objects ({q_total} queries), FP64 oracle, "
+                  f"detection + computeForceWithDerivative (FD stencils), OpenMP over force objects",
+    }, {
+        "max_rel_err_vs_sum_of_contact_norms": float(max(errs_sum)) if errs_sum else None,
+        "max_rel_err_vs_net": float(max(errs_net)) if errs_net else None,
+        "median_rel_err_vs_net": float(np.median(errs_net)) if errs_net else None,
+        "instances_checked": n_done,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="batched", choices=["batched", "differentiated", "peg_in_hole"])
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU oracle work")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--tile-points", type=int, default=0)
+    ap.add_argument("--detect-block", type=int, default=0)
+    ap.add_argument("--order", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from atac.batch import batch_from_workload, device_init
+
+    n_cu = device_init(0)
+    t0 = time.perf_counter()
+    w = make_workload(args.workload)
+    log(f"workload {w.name}: {len(w.instances)} instances, {w.n_queries} queries, generated in "
+        f"{time.perf_counter() - t0:.1f}s")
+    t0 = time.perf_counter()
+    tuning = {"tile_points": args.tile_points, "detect_block": args.detect_block, "order": args.order}
+    b = batch_from_workload(w, tuning=tuning)
+    info = b.info()
+    log(f"batch ready in {time.perf_counter() - t0:.1f}s: {info}")
+    n_q = info["n_points"]
+
+    g = torch.tensor(np.random.default_rng(7).normal(size=(b.n, 12)), device="cuda")
+    out = b.alloc_outputs()
+    grads = b.alloc_gradients()
+    b.graph_capture(out, checkpoint=True, g_fm=g, grads=grads)
+
+    for _ in range(args.warmup):
+        b.graph_launch()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(args.steps):
+        b.graph_launch()
+    stop.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / args.steps
+    n_active = int(out["n_active"].sum().item())
+    log(f"step {ms:.3f} ms, {n_q / ms * 1e3:.3e} queries/s, active {n_active} ({n_active / n_q:.2%})")
+
+    # per-kernel durations with HIP events on the library's launch stream (eager launches)
+    b.timing(True)
+    for _ in range(args.steps):
+        b.eval(checkpoint=True, out=out)
+        b.vjp(g, out=grads)
+    st = b.timing(False)
+    torch.cuda.synchronize()
+    stage_ms = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in st.items()}
+    log(f"stage ms: {stage_ms}")
+
+    touched = b.touched_values()
+    det_bytes = 12 * n_q + 4 * touched + INST_DEV_BYTES * b.n + TILE_BYTES * info["n_tiles"] + 4 * n_active
+    det_ms = stage_ms["detect"]
+    achieved = det_bytes / (det_ms * 1e-3) / 1e9
+    step_bytes = (det_bytes + 20 * n_active * 2 + 8 * 12 * b.n * 2 + 8 * 26 * b.n)
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
+                "kernel": "detect_kernel", "algorithmic_bytes_per_launch": int(det_bytes),
+                "touched_grid_values": touched, "launch_ms": round(det_ms, 4)}
+
+    plain = None
+    if not args.no_plain:
+        po = b.alloc_outputs(deriv=True)
+        b.eval_plain(deriv=True, out=po)
+        torch.cuda.synchronize()
+        reps = 3
+        s2 = torch.cuda.Event(enable_timing=True)
+        e2 = torch.cuda.Event(enable_timing=True)
+        s2.record()
+        for _ in range(reps):
+            b.eval_plain(deriv=True, out=po)
+        e2.record()
+        torch.cuda.synchronize()
+        pms = s2.elapsed_time(e2) / reps
+        plain = {"ms_per_step": round(pms, 3), "queries_per_s": n_q / pms * 1e3,
+                 "what": "one launch per force object, one lane per point, inline contact math with Km/Dm "
+                         "(the adjoint by contraction), FP64 atomics",
+                 "speedup_batched_vs_plain": round(pms / ms, 1)}
+        log(f"plain path {pms:.2f} ms/step ({pms / ms:.1f}x slower)")
+
+    cpu = None
+    err = None
+    if not args.no_cpu:
+        threads = min(16, len(os.sched_getaffinity(0)))
+        cpu, err = cpu_baseline(w, out["fm"].cpu().numpy(), args.cpu_budget, threads)
+        log(f"cpu baseline {cpu['value']:.3e} queries/s on {threads} threads; err {err}")
+
+    line = {
+        "metric": METRIC, "value": n_q / ms * 1e3, "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": w.name, "instances": b.n, "queries": n_q, "grids": f"{len(w.grids)} x "
+                   f"{'x'.join(str(v) for v in w.grids[0].n)} FP32", "active_fraction": round(n_active / n_q, 5),
+                   "step": "detect + contact wrench + reduce + adjoint (VJP) + reduce, HIP graph replay",
+                   "tile_points": info["tile_points"], "detect_block": info["detect_block"],
+                   "persistent_blocks": info["persistent_blocks"], "n_cu": n_cu, "parallelism": "none"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "err_vs_fp64": err,
+        "plain_path": plain,
+        "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "step_effective_GBps": round(step_bytes / (ms * 1e-3) / 1e9, 1),
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

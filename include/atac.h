@@ -1,0 +1,181 @@
+/*
+ * atac: MI355X (gfx950) point-vs-SDF contact for RedMax-style simulation.
+ *
+ * Plain C ABI of libatac_contact.so. Host buffers are plain pointers and sizes;
+ * "device" buffers are HIP device pointers allocated by the caller; streams are
+ * hipStream_t passed as void* (NULL = the default stream). Every entry point
+ * returns ATAC_OK (0) or a negative status; atac_last_error() returns the message
+ * of the last failure on the calling thread. Handles are not thread-safe.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   simulation/redmax/Body/BodySDFObj.{h,cpp}          the SDF body (grid + queries)
+ *   simulation/redmax/CollisionDetection/CollisionDetection.cpp:124-147
+ *   simulation/redmax/Force/ForceGeneralSDFContact.{h,cpp}
+ *   simulation/redmax/Mesh.cpp:143-198                 SDFMesh distance queries
+ */
+#ifndef ATAC_H
+#define ATAC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ATAC_OK 0
+#define ATAC_ERR_INVALID (-1)   /* bad argument or size */
+#define ATAC_ERR_HIP (-2)       /* HIP runtime error */
+#define ATAC_ERR_NOMEM (-3)     /* allocation failed */
+#define ATAC_ERR_STATE (-4)     /* call not valid in the current state */
+
+typedef struct atac_grid_s* atac_grid;     /* FP32 SDF grid resident in HBM */
+typedef struct atac_points_s* atac_points; /* Morton-sorted SoA FP32 contact points */
+typedef struct atac_batch_s* atac_batch;   /* instance table, tile plan, workspace */
+
+/* One force object: ForceGeneralSDFContact(contact_body, SDF_body, kn, kt, mu, damping)
+ * with its _scale, the contact body's _contact_scale, the SDF body's _col_th and the
+ * two bodies' _E_0i / _phi (ForceGeneralSDFContact.h:14-31, Body.h:48-61,
+ * BodySDFObj.h:13-15). E is 4x4 row-major [R p; 0 1]; phi = [w; v] in the body frame. */
+typedef struct {
+    int32_t grid;     /* index into the grid array given to atac_batch_create */
+    int32_t points;   /* index into the point-set array */
+    int32_t segment;  /* segment of that point set */
+    int32_t reserved;
+    double E1[16];    /* contact body pose */
+    double phi1[6];
+    double E2[16];    /* SDF body pose */
+    double phi2[6];
+    double col_th;    /* BodySDFObj::_col_th */
+    double kn, kt, mu, damping;
+    double scale;         /* ForceGeneralSDFContact::_scale (set_contact_scale) */
+    double contact_scale; /* contact body's _contact_scale */
+} atac_instance;
+
+typedef struct {
+    int32_t tile_points;   /* points per tile (0: 4096); at most 65536 */
+    int32_t detect_block;  /* threads per detection workgroup: 128, 256, 512, 1024 (0: 256) */
+    int32_t blocks_per_cu; /* persistent workgroups per CU (0: occupancy limit) */
+    int32_t order;         /* tile processing order: 0 instance order, 1 grouped by grid */
+} atac_tuning;
+
+/* flags of atac_batch_eval / atac_eval_plain */
+#define ATAC_EVAL_FORCE 0u       /* fm only: ForceGeneralSDFContact::computeForce */
+#define ATAC_EVAL_DERIV 1u       /* + Km, Dm: computeForceWithDerivative */
+#define ATAC_EVAL_P6 2u          /* + contact-scale column of dfm_dp */
+#define ATAC_EVAL_CHECKPOINT 4u  /* keep (d, grad phi) of active points for atac_batch_vjp */
+#define ATAC_EVAL_PRECISE 8u     /* FP64 contact arithmetic (detection stays FP32) */
+
+/* Outputs of one evaluation, device pointers, any may be NULL. Per instance:
+ * fm 12 = [tau1 f1 tau2 f2] (each body's frame, angular first), Km/Dm 144 row-major
+ * (d fm / d [q1 q2], d fm / d [phi1 phi2], pose perturbation E <- E exp([w; v])),
+ * dfm_dp6 12 (d fm / d contact_scale as the reference assembles it), n_active 1. */
+typedef struct {
+    double* fm;
+    double* Km;
+    double* Dm;
+    double* dfm_dp6;
+    int32_t* n_active;
+} atac_outputs;
+
+/* Gradients from atac_batch_vjp, device pointers, any may be NULL. Per instance:
+ * g_q1/g_q2 6 (right-multiplied pose twists), g_phi1/g_phi2 6, g_kn 1, g_kt 1
+ * (with respect to the force's kn and kt parameters). */
+typedef struct {
+    double* g_q1;
+    double* g_q2;
+    double* g_phi1;
+    double* g_phi2;
+    double* g_kn;
+    double* g_kt;
+} atac_gradients;
+
+const char* atac_last_error(void);
+int atac_version(void);
+
+/* Selects the HIP device used by subsequent calls of this thread and reports its
+ * compute-unit count (may be NULL). */
+int atac_device_init(int32_t device, int32_t* n_cu);
+
+/* BodySDFObj's grid (_SDF, _min_box, _max_box, _dx/_dy/_dz; BodySDFObj.h:13-16,
+ * BodySDFObj.cpp:77-83): ni*nj*nk FP32 values in a[i + ni*(j + nj*k)] order. */
+int atac_grid_create(const float* values, int32_t ni, int32_t nj, int32_t nk, const double min_box[3],
+                     const double max_box[3], const double dxyz[3], atac_grid* out);
+int atac_grid_destroy(atac_grid g);
+
+/* Body::_contact_points (Body.h:37,129; BodyMeshObj.cpp:184-238): n points, FP64 xyz
+ * interleaved, in the contact body's frame. seg_offsets (n_segments + 1 entries,
+ * NULL = one segment) splits them into independent sets. Each segment is Morton-
+ * sorted by the cell of edge sort_cell it falls in (<= 0: automatic) and stored
+ * as FP32 SoA. */
+int atac_points_create(const double* xyz, int64_t n, const int64_t* seg_offsets, int32_t n_segments,
+                       double sort_cell, atac_points* out);
+int atac_points_destroy(atac_points p);
+/* For each stored (sorted) point of a segment, its original index within the segment. */
+int atac_points_permutation(atac_points p, int32_t segment, int64_t* sorted_to_original, int64_t capacity);
+
+/* A set of force objects evaluated together in one persistent launch. */
+int atac_batch_create(const atac_grid* grids, int32_t n_grids, const atac_points* points, int32_t n_point_sets,
+                      const atac_instance* inst, int32_t n_inst, const atac_tuning* tuning, atac_batch* out);
+int atac_batch_destroy(atac_batch b);
+/* New poses / parameters for the same (grid, points, segment) structure. */
+int atac_batch_set_instances(atac_batch b, const atac_instance* inst, int32_t n_inst, void* stream);
+/* Re-plans tiles and launch shape (used by the autotuner). */
+int atac_batch_retune(atac_batch b, const atac_tuning* tuning);
+int atac_batch_info(atac_batch b, int64_t* n_points, int32_t* n_tiles, int32_t* persistent_blocks,
+                    atac_tuning* tuning);
+
+/* ForceGeneralSDFContact::computeForce / computeForceWithDerivative for every instance
+ * (ForceGeneralSDFContact.cpp:46-144 with collision_detection_general_SDF,
+ * CollisionDetection.cpp:124-147): detection over all points, then the contact
+ * wrench (and Km, Dm, p6 column) over the active ones. Stream-ordered. */
+int atac_batch_eval(atac_batch b, uint32_t flags, const atac_outputs* out, void* stream);
+
+/* Adjoint of the last checkpointed evaluation: g_fm (device, 12 per instance) ->
+ * gradients. Stream-ordered after that evaluation. */
+int atac_batch_vjp(atac_batch b, const double* g_fm, const atac_gradients* out, void* stream);
+
+/* Captures [reset, detect, contact, reduce] (and the adjoint when g_fm != NULL)
+ * into a HIP graph bound to these outputs; atac_batch_graph_launch replays it. */
+int atac_batch_graph_capture(atac_batch b, uint32_t flags, const atac_outputs* out, const double* g_fm,
+                             const atac_gradients* grads);
+int atac_batch_graph_launch(atac_batch b, void* stream);
+
+/* Per-stage kernel timing with HIP events recorded on the launch stream around
+ * each kernel of atac_batch_eval / atac_batch_vjp (not inside captured graphs).
+ * Reads and clears what was recorded since the last call (ms_sum, launches: 5
+ * entries each for detect, contact, reduce, vjp, vjp-reduce; may be NULL), then
+ * enables (enable != 0) or disables recording. Synchronous when reading. */
+int atac_batch_timing(atac_batch b, int32_t enable, double* ms_sum, int32_t* launches);
+
+/* Active contacts of one instance from the last evaluation (host out): original
+ * point indices (within the instance's segment), ascending. Synchronous. */
+int atac_batch_active_ids(atac_batch b, int32_t inst, int64_t* ids, int64_t capacity, int64_t* count);
+
+/* Distinct grid values read by the detection stencils of all instances (summed
+ * over grids). Synchronous diagnostic used for the algorithmic byte count. */
+int atac_batch_touched_values(atac_batch b, int64_t* count);
+
+/* The plain per-instance path: one launch per instance, one lane per point, the
+ * contact math inline and FP64 atomics; same outputs as atac_batch_eval. */
+int atac_eval_plain(atac_batch b, uint32_t flags, const atac_outputs* out, void* stream);
+
+/* BodySDFObj::distance (BodySDFObj.cpp:207-213) for n world points (device, n x 3):
+ * query_SDF(R2^T (xw - p2)) + col_th, FLT_MAX + col_th outside the box. With E2 = I
+ * and col_th = 0 this is SDFMesh::distance (Mesh.cpp:143-198). */
+int atac_sdf_distance(atac_grid g, const double E2[16], double col_th, const double* xw, int64_t n, double* d,
+                      void* stream);
+
+/* BodySDFObj::collision (BodySDFObj.cpp:239-377) for n world points and velocities
+ * (device, n x 3 each): values (n x 11: d, n[3], ddot, tdot[3], xi2[3]) and, when
+ * derivs != NULL, the 14 derivative blocks (n x 135: dd_dxw 3, dd_dq2 6, dn_dxw 9,
+ * dn_dq2 18, dddot_dxw 3, dddot_dxwdot 3, dddot_dq2 6, dddot_dphi2 6, dtdot_dxw 9,
+ * dtdot_dxwdot 9, dtdot_dq2 18, dtdot_dphi2 18, dxi2_dxw 9, dxi2_dq2 18, row-major). */
+int atac_sdf_collision(atac_grid g, const double E2[16], const double phi2[6], double col_th, uint32_t flags,
+                       const double* xw, const double* xw_dot, int64_t n, double* values, double* derivs,
+                       void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ATAC_H */

@@ -1,0 +1,79 @@
+"""TEST ONLY: ctypes front end of libhostcheck.so, the contact library's device
+arithmetic compiled for the host (see hostcheck.hip)."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "libhostcheck.so")
+_REPO = os.path.dirname(os.path.dirname(_HERE))
+
+
+def build():
+    cmd = ["/opt/rocm/bin/hipcc", "--offload-arch=gfx950", "-O2", "-std=c++17", "-fPIC", "-shared",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           "-I" + os.path.join(_REPO, "assemble-them-all_amd", "csrc"), "-I" + os.path.join(_REPO, "include"),
+           os.path.join(_HERE, "hostcheck.hip"), "-o", _LIB]
+    subprocess.run(cmd, check=True, stderr=subprocess.DEVNULL)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        _lib = C.CDLL(_LIB)
+    return _lib
+
+
+def _grid_args(g):
+    vals = np.ascontiguousarray(g.values, dtype=np.float32)
+    nk, nj, ni = vals.shape
+    n = np.array([ni, nj, nk], dtype=np.int32)
+    return vals, n, np.ascontiguousarray(g.min_box, dtype=np.float64), np.ascontiguousarray(g.max_box, dtype=np.float64), \
+        np.ascontiguousarray(g.dxyz, dtype=np.float64)
+
+
+def _vp(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def eval_instance(grid, pts_f32, inst, deriv=False, precise=False, g_fm=None):
+    """Host run of the device math for one force object. inst: atac._lib.Instance."""
+    vals, n, mn, mx, d = _grid_args(grid)
+    p = np.ascontiguousarray(pts_f32, dtype=np.float32).reshape(-1, 3)
+    px, py, pz = [np.ascontiguousarray(p[:, k]) for k in range(3)]
+    npts = len(p)
+    fm = np.zeros(12); p6 = np.zeros(12)
+    Km = np.zeros(144) if deriv else None
+    Dm = np.zeros(144) if deriv else None
+    na = C.c_int64(0)
+    act = np.zeros(npts, dtype=np.uint8)
+    slow = np.zeros(npts, dtype=np.uint8)
+    g = np.ascontiguousarray(g_fm, dtype=np.float64) if g_fm is not None else None
+    vjp = np.zeros(26) if g is not None else None
+    flags = 8 if precise else 0
+    lib().hc_eval(_vp(vals), _vp(n), _vp(mn), _vp(mx), _vp(d), _vp(px), _vp(py), _vp(pz), C.c_int64(npts), C.byref(inst),
+                  C.c_uint32(flags), _vp(fm), _vp(p6), _vp(Km), _vp(Dm), C.byref(na), _vp(act), _vp(slow), _vp(g), _vp(vjp))
+    out = {"fm": fm, "dfm_dp6": p6, "n_active": int(na.value), "active": act.astype(bool), "slow": slow.astype(bool)}
+    if deriv:
+        out["Km"] = Km.reshape(12, 12)
+        out["Dm"] = Dm.reshape(12, 12)
+    if vjp is not None:
+        out["vjp"] = vjp
+    return out
+
+
+def detect_values(grid, pts_f32, inst):
+    vals, n, mn, mx, d = _grid_args(grid)
+    p = np.ascontiguousarray(pts_f32, dtype=np.float32).reshape(-1, 3)
+    px, py, pz = [np.ascontiguousarray(p[:, k]) for k in range(3)]
+    out = np.zeros(len(p), dtype=np.float32)
+    lib().hc_detect(_vp(vals), _vp(n), _vp(mn), _vp(mx), _vp(d), _vp(px), _vp(py), _vp(pz), C.c_int64(len(p)),
+                    C.byref(inst), _vp(out))
+    return out

@@ -1,0 +1,163 @@
+"""GPU adjoint (VJP) checks: against the oracle's Km/Dm (the reference's analytic
+dfdq), against FP64 central finite differences (right-multiplied exp, active set
+held fixed as in the reference FD tests, ForceGeneralSDFContact.cpp:1048-1110) and
+d/d(kn) against a kn central difference."""
+import numpy as np
+import pytest
+import torch
+
+from atac import workloads as W
+from atac.batch import batch_from_workload
+from helpers import branch_stable_contacts, oracle_workload, rel_err
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def setup(gpu):
+    w = W.peg_in_hole(n_poses=24)
+    ref = oracle_workload(w)
+    b = batch_from_workload(w)
+    return w, ref, b
+
+
+def test_vjp_matches_oracle_jacobians(setup):
+    w, ref, b = setup
+    rng = np.random.default_rng(11)
+    g = rng.normal(size=(b.n, 12))
+    b.eval(checkpoint=True)
+    grads = b.vjp(torch.tensor(g, device="cuda"))
+    torch.cuda.synchronize()
+    gq1, gq2 = grads["g_q1"].cpu().numpy(), grads["g_q2"].cpu().numpy()
+    gp1, gp2 = grads["g_phi1"].cpu().numpy(), grads["g_phi2"].cpu().numpy()
+    for i, r in enumerate(ref):
+        if not r["n_active"]:
+            assert np.all(gq1[i] == 0)
+            continue
+        Kc = g[i] @ r["Km"]
+        Dc = g[i] @ r["Dm"]
+        s = np.linalg.norm(g[i]) * r["scale_Km"]
+        assert rel_err(np.concatenate([gq1[i], gq2[i]]), Kc, s) <= TOL
+        sd = np.linalg.norm(g[i]) * max(r["scale_Dm"], 1e-30)
+        assert rel_err(np.concatenate([gp1[i], gp2[i]]), Dc, sd) <= TOL
+
+
+def _fixed_contact_fm(w, i, ids, E1=None, E2=None, kn=None):
+    ins = w.instances[i]
+    G = O.GridRef.from_grid(w.grids[ins.grid])
+    pts = W.round_points_f32(w.point_sets[ins.points])[ids]
+    return O.force_fixed_contacts(G, pts, ins.E1 if E1 is None else E1, ins.phi1, ins.E2 if E2 is None else E2,
+                                  ins.phi2, ins.col_th, ins.kn if kn is None else kn, ins.kt, ins.mu, ins.damping)["fm"]
+
+
+def multilinear_workload(n_poses=8, n_points=4000, seed=21, mu=W.MU, kt=W.KT):
+    """A ground body whose field f = z + 0.05 x y + 0.03 y z - 0.02 x y z is multilinear, so
+    trilinear interpolation reproduces it in every cell (no kinks at cell faces), and a
+    point patch pressed into it with random tilt, offset and twists."""
+    rng = np.random.default_rng(seed)
+    grid = W.make_grid(lambda x, y, z: z + 0.05 * x * y + 0.03 * y * z - 0.02 * x * y * z, (41, 41, 41),
+                       (-1.0, -1.0, -1.0), (0.05, 0.05, 0.05))
+    pts = np.stack([rng.uniform(-0.4, 0.4, n_points), rng.uniform(-0.4, 0.4, n_points),
+                    rng.uniform(-0.02, 0.0, n_points)], axis=1)
+    inst = []
+    for _ in range(n_poses):
+        R = W.rotation_exp(rng.normal(0, 0.05, 3))
+        t = np.array([rng.uniform(-0.1, 0.1), rng.uniform(-0.1, 0.1), rng.uniform(-0.01, 0.01)])
+        E2 = W.pose(W.rotation_exp(rng.normal(0, 0.02, 3)), rng.normal(0, 0.01, 3))
+        inst.append(W.Instance(0, 0, W.pose(R, t), rng.normal(0, 0.1, 6), E2, rng.normal(0, 0.1, 6), mu=mu, kt=kt))
+    return W.Workload("multilinear_ground", [grid], [pts], inst)
+
+
+def _fd_check(w, ref, h, tol, n_check=4, seed=12):
+    """For up to n_check instances: the GPU adjoint over the contacts a step h cannot
+    switch (friction branch, d = 0) against FP64 central FD of the oracle over the same
+    fixed contacts, and d/d(kn) against a kn central difference."""
+    rng = np.random.default_rng(seed)
+    checked = 0
+    for i, r in enumerate(ref):
+        if not r["n_active"] or checked >= n_check:
+            continue
+        ins = w.instances[i]
+        ids = branch_stable_contacts(w, ins, np.nonzero(r["active"])[0], h=h)
+        if len(ids) < 10:
+            continue
+        print(f"[case] fd h={h} instance {i} with {len(ids)} contacts", flush=True)
+        pts = W.round_points_f32(w.point_sets[ins.points])[ids]
+        sub = W.Workload("fd_subset", [w.grids[ins.grid]], [pts],
+                         [W.Instance(0, 0, ins.E1, ins.phi1, ins.E2, ins.phi2, ins.col_th, ins.kn, ins.kt, ins.mu,
+                                     ins.damping, ins.scale, ins.contact_scale)])
+        g = rng.normal(size=(1, 12))
+        b = batch_from_workload(sub)
+        out = b.eval(checkpoint=True)
+        grads = b.vjp(torch.tensor(g, device="cuda"))
+        torch.cuda.synchronize()
+        assert int(out["n_active"][0].item()) == len(ids)
+        an = np.concatenate([grads["g_q1"][0].cpu().numpy(), grads["g_q2"][0].cpu().numpy()])
+        gkn = float(grads["g_kn"][0].item())
+        all_ids = np.arange(len(ids))
+        fd = np.zeros(12)
+        for k in range(6):
+            dq = np.zeros(6)
+            dq[k] = h
+            fp = _fixed_contact_fm(sub, 0, all_ids, E1=ins.E1 @ O.se3_exp(dq))
+            fmn = _fixed_contact_fm(sub, 0, all_ids, E1=ins.E1 @ O.se3_exp(-dq))
+            fd[k] = g[0] @ (fp - fmn) / (2 * h)
+            fp = _fixed_contact_fm(sub, 0, all_ids, E2=ins.E2 @ O.se3_exp(dq))
+            fmn = _fixed_contact_fm(sub, 0, all_ids, E2=ins.E2 @ O.se3_exp(-dq))
+            fd[6 + k] = g[0] @ (fp - fmn) / (2 * h)
+        G = O.GridRef.from_grid(w.grids[ins.grid])
+        full = O.force_fixed_contacts(G, pts, ins.E1, ins.phi1, ins.E2, ins.phi2, ins.col_th, ins.kn, ins.kt, ins.mu,
+                                      ins.damping, derivs=True)
+        scale = np.linalg.norm(g[0]) * np.abs(full["Km"]).sum()
+        assert rel_err(an, fd, scale) <= tol, (i, rel_err(an, fd, scale))
+        dk = ins.kn * 1e-6
+        fkp = _fixed_contact_fm(sub, 0, all_ids, kn=ins.kn + dk)
+        fkm = _fixed_contact_fm(sub, 0, all_ids, kn=ins.kn - dk)
+        fdk = g[0] @ (fkp - fkm) / (2 * dk)
+        assert abs(gkn - fdk) <= tol * np.linalg.norm(g[0]) * np.abs(full["fm"]).sum() / ins.kn, (i, gkn, fdk)
+        checked += 1
+    assert checked > 0
+
+
+def test_vjp_central_fd_h1e4_multilinear_field(gpu):
+    """Central FD, h = 1e-4, FP64, on a field with no interpolation kinks."""
+    w = multilinear_workload()
+    _fd_check(w, oracle_workload(w), 1e-4, 1e-5)
+
+
+def test_vjp_central_fd_peg_in_hole(setup):
+    """Central FD on the peg-in-hole grid; h = 1e-6 keeps the FD stencil of almost every
+    contact inside its trilinear cell (the interpolant has slope jumps at cell faces)."""
+    w, ref, b = setup
+    _fd_check(w, ref, 1e-6, 1e-4)
+
+
+def test_vjp_requires_checkpoint(setup):
+    w, ref, b = setup
+    b.eval()
+    import atac
+    with pytest.raises(atac.AtacError):
+        b.vjp(torch.zeros((b.n, 12), dtype=torch.float64, device="cuda"))
+
+
+def test_graph_with_adjoint(setup):
+    w, ref, b = setup
+    g = torch.tensor(np.random.default_rng(3).normal(size=(b.n, 12)), device="cuda")
+    b.eval(checkpoint=True)
+    eager = b.vjp(g)
+    torch.cuda.synchronize()
+    eager = {k: v.cpu().numpy() for k, v in eager.items()}
+    out = b.alloc_outputs()
+    grads = b.alloc_gradients()
+    b.graph_capture(out, checkpoint=True, g_fm=g, grads=grads)
+    for rep in range(3):
+        for v in grads.values():
+            v.zero_()
+        b.graph_launch()
+        torch.cuda.synchronize()
+        for k in eager:
+            assert np.array_equal(grads[k].cpu().numpy(), eager[k]), (rep, k)
+        torch.ones(100, device="cuda").sum().item()

@@ -1,0 +1,137 @@
+"""CPU checks of the contact library's arithmetic (contact_math.h, compiled for the
+host) against the FP64 oracle: detection, wrench, Km/Dm tangent columns, the p6
+column, the hand-written adjoint (against the oracle's Km/Dm and against central
+finite differences), and the FP64 slow path near cell faces and the box boundary."""
+import numpy as np
+import pytest
+
+from atac import workloads as W
+from atac.batch import make_instance
+from helpers import band_mask, oracle_workload, rel_err
+from hostcheck import hostcheck as H
+from oracle import oracle as O
+
+TOL = 1e-5
+
+
+def _ai(ins):
+    return make_instance(ins.grid, 0, 0, ins.E1, ins.phi1, ins.E2, ins.phi2, ins.col_th, ins.kn, ins.kt, ins.mu,
+                         ins.damping, ins.scale, ins.contact_scale)
+
+
+@pytest.fixture(scope="module")
+def peg():
+    w = W.peg_in_hole(n_poses=48)
+    return w, oracle_workload(w)
+
+
+@pytest.mark.parametrize("precise", [False, True])
+def test_peg_wrench_jacobians_p6(peg, precise):
+    w, ref = peg
+    pts = W.round_points_f32(w.point_sets[0])
+    for i, r in enumerate(ref):
+        h = H.eval_instance(w.grids[0], pts, _ai(w.instances[i]), deriv=True, precise=precise)
+        diff = h["active"] != r["active"]
+        assert not np.any(diff & ~band_mask(w, w.instances[i]))
+        if not r["n_active"]:
+            assert np.all(h["fm"] == 0)
+            continue
+        assert rel_err(h["fm"], r["fm"], r["scale_fm"]) <= TOL
+        assert rel_err(h["Km"], r["Km"], r["scale_Km"]) <= TOL
+        assert rel_err(h["Dm"], r["Dm"], r["scale_Dm"]) <= TOL
+        assert rel_err(h["dfm_dp6"], r["dfm_dp6"], r["scale_fm"]) <= TOL
+
+
+def test_vjp_equals_jacobian_contraction(peg):
+    w, ref = peg
+    pts = W.round_points_f32(w.point_sets[0])
+    rng = np.random.default_rng(4)
+    for i, r in enumerate(ref[:16]):
+        if not r["n_active"]:
+            continue
+        g = rng.normal(size=12)
+        h = H.eval_instance(w.grids[0], pts, _ai(w.instances[i]), g_fm=g)
+        s = np.linalg.norm(g) * r["scale_Km"]
+        assert rel_err(h["vjp"][:12], g @ r["Km"], s) <= TOL
+        assert rel_err(h["vjp"][12:24], g @ r["Dm"], np.linalg.norm(g) * r["scale_Dm"]) <= TOL
+
+
+def _fixed_fm(w, i, ids, E1=None, E2=None, kn=None, kt=None, phi1=None, phi2=None):
+    ins = w.instances[i]
+    G = O.GridRef.from_grid(w.grids[ins.grid])
+    pts = W.round_points_f32(w.point_sets[ins.points])[ids]
+    return O.force_fixed_contacts(G, pts, ins.E1 if E1 is None else E1, ins.phi1 if phi1 is None else phi1,
+                                  ins.E2 if E2 is None else E2, ins.phi2 if phi2 is None else phi2, ins.col_th,
+                                  ins.kn if kn is None else kn, ins.kt if kt is None else kt, ins.mu,
+                                  ins.damping)["fm"]
+
+
+def _multilinear(mu, kt):
+    import test_gpu_adjoint
+    return test_gpu_adjoint.multilinear_workload(mu=mu, kt=kt)
+
+
+@pytest.mark.parametrize("mu,kt", [(0.3, 1e3), (0.0, 1e3)], ids=["friction", "frictionless"])
+def test_vjp_central_fd_h1e4_multilinear(mu, kt):
+    """The hand-written adjoint (host-compiled) against FP64 central FD with h = 1e-4 on a
+    multilinear field, which trilinear interpolation reproduces in every cell, over the
+    contacts whose friction branch cannot switch inside the step."""
+    from helpers import branch_stable_contacts
+    w = _multilinear(mu, kt)
+    ref = oracle_workload(w)
+    pts = W.round_points_f32(w.point_sets[0])
+    rng = np.random.default_rng(9)
+    h = 1e-4
+    n_static = 0
+    for i, r in enumerate(ref[:4]):
+        ins = w.instances[i]
+        ids = branch_stable_contacts(w, ins, np.nonzero(r["active"])[0])
+        g = rng.normal(size=12)
+        sub = O.force_fixed_contacts(O.GridRef.from_grid(w.grids[0]), pts[ids], ins.E1, ins.phi1, ins.E2, ins.phi2,
+                                     ins.col_th, ins.kn, ins.kt, ins.mu, ins.damping, derivs=True)
+        hv = H.eval_instance(w.grids[0], pts[ids], _ai(ins), g_fm=g, precise=True)
+        assert hv["n_active"] == len(ids)
+        v = hv["vjp"]
+        fd = np.zeros(24)
+        for k in range(6):
+            dq = np.zeros(6)
+            dq[k] = h
+            fd[k] = g @ (_fixed_fm(w, i, ids, E1=ins.E1 @ O.se3_exp(dq)) - _fixed_fm(w, i, ids, E1=ins.E1 @ O.se3_exp(-dq))) / (2 * h)
+            fd[6 + k] = g @ (_fixed_fm(w, i, ids, E2=ins.E2 @ O.se3_exp(dq)) - _fixed_fm(w, i, ids, E2=ins.E2 @ O.se3_exp(-dq))) / (2 * h)
+            fd[12 + k] = g @ (_fixed_fm(w, i, ids, phi1=ins.phi1 + dq) - _fixed_fm(w, i, ids, phi1=ins.phi1 - dq)) / (2 * h)
+            fd[18 + k] = g @ (_fixed_fm(w, i, ids, phi2=ins.phi2 + dq) - _fixed_fm(w, i, ids, phi2=ins.phi2 - dq)) / (2 * h)
+        sK = np.linalg.norm(g) * np.abs(sub["Km"]).sum()
+        sD = np.linalg.norm(g) * max(np.abs(sub["Dm"]).sum(), 1e-30)
+        assert rel_err(v[:12], fd[:12], sK) <= TOL
+        assert rel_err(v[12:24], fd[12:24], sD) <= TOL
+        assert rel_err(v[:12], g @ sub["Km"], sK) <= TOL
+        sf = np.linalg.norm(g) * np.abs(sub["fm"]).sum()
+        dk = ins.kn * 1e-5
+        fdk = g @ (_fixed_fm(w, i, ids, kn=ins.kn + dk) - _fixed_fm(w, i, ids, kn=ins.kn - dk)) / (2 * dk)
+        assert abs(v[24] - fdk) <= 1e-7 * sf / ins.kn
+        dt = ins.kt * 1e-5
+        fdt = g @ (_fixed_fm(w, i, ids, kt=ins.kt + dt) - _fixed_fm(w, i, ids, kt=ins.kt - dt)) / (2 * dt)
+        assert abs(v[25] - fdt) <= 1e-7 * sf / ins.kt
+        n_static += len(ids)
+    assert n_static > 100
+
+
+def test_slow_path_near_faces_and_box():
+    """Points placed within the FD step of cell faces, cell corners and the box boundary
+    take the FP64 stencil path and reproduce the reference's FD blend exactly."""
+    w = W.peg_in_hole(n_points=16, n_poses=1)
+    g = w.grids[0]
+    rng = np.random.default_rng(8)
+    base = -1.575 + 0.05 * rng.integers(4, 60, size=(3000, 3))
+    off = rng.choice([-3e-6, -1e-6, 0.0, 1e-6, 3e-6, 7e-6, 0.0125, 0.025], size=(3000, 3))
+    x = (base + off).astype(np.float32).astype(np.float64)
+    G = O.GridRef.from_grid(g)
+    E1 = np.eye(4)
+    ins = W.Instance(0, 0, E1, np.zeros(6), np.eye(4), np.zeros(6), col_th=-5.0)   # d = phi - 5: every inside point active
+    ref = O.eval_instance([G], O.make_instance(0, x, E1, ins.phi1, ins.E2, ins.phi2, ins.col_th, ins.kn, ins.kt,
+                                               ins.mu, ins.damping), O.FLAG_DERIV)
+    h = H.eval_instance(g, x.astype(np.float32), _ai(ins), deriv=True)
+    assert h["slow"].sum() > 500
+    assert h["n_active"] == ref["n_active"]
+    assert rel_err(h["fm"], ref["fm"], ref["scale_fm"]) <= TOL
+    assert rel_err(h["Km"], ref["Km"], ref["scale_Km"]) <= TOL

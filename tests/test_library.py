@@ -1,0 +1,46 @@
+"""CPU checks of the built artefacts: the contact library loads and exports every
+entry point include/atac.h declares (no GPU call is made)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, "include", "atac.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(atac_[a-z_0-9]+)\s*\(", src)))
+
+
+def test_header_declares_entry_points():
+    syms = declared_symbols()
+    assert "atac_batch_eval" in syms and "atac_batch_vjp" in syms and len(syms) >= 20
+
+
+def test_library_exports_every_declared_symbol():
+    from atac import _lib
+    assert os.path.exists(_lib.LIB_PATH), "run __graft_entry__.build() first"
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [s for s in declared_symbols() if not hasattr(L, s)]
+    assert not missing, missing
+    assert set(_lib.EXPORTED) == set(declared_symbols())
+
+
+def test_library_reports_version_and_errors_without_gpu():
+    from atac import _lib
+    L = _lib.lib()
+    assert L.atac_version() >= 100
+    # a null handle is rejected before any device work
+    rc = L.atac_batch_eval(None, 0, None, None)
+    assert rc == _lib.ATAC_ERR_INVALID
+    assert b"null batch" in L.atac_last_error()
+
+
+def test_package_imports_without_loading_library():
+    import importlib
+    import atac
+    importlib.reload(atac)
+    assert atac.EVAL_DERIV == 1 and atac.EVAL_CHECKPOINT == 4
