@@ -76,16 +76,17 @@ struct atac_batch_s {
     atac_tuning tune{};
     std::vector<TileDev> tiles;
     TileDev* d_tiles = nullptr;
-    int32_t* d_order = nullptr;
+    int32_t* d_order = nullptr;       // tiles, queue-partitioned
+    int32_t* d_tq_begin = nullptr;    // nq + 1
+    int32_t* d_iorder = nullptr;      // instances, queue-partitioned
+    int32_t* d_iq_begin = nullptr;    // nq + 1
+    int32_t nq = 1;
     int32_t n_tiles = 0;
     int64_t n_slots = 0;
     uint32_t* d_act = nullptr;
     CkptDev* d_ck = nullptr;
     int32_t* d_tile_active = nullptr;
-    double* d_partial = nullptr;
-    int64_t partial_cap = 0;   // doubles
-    double* d_vjp_partial = nullptr;
-    int32_t* d_counters = nullptr;  // detect, contact, vjp
+    int32_t* d_counters = nullptr;  // 3 x nq queue heads: detect, contact, vjp
     int grid_detect = 0, grid_contact = 0, grid_vjp = 0;
     size_t lds_detect = 0;
     int wave_seg = 0;
@@ -297,9 +298,11 @@ static int check_instances(atac_batch b, const atac_instance* inst, int32_t n) {
     return ATAC_OK;
 }
 
+constexpr int kXcds = 8;   // MI355X: 8 XCDs, each with its own L2
+
 static void release_plan(atac_batch b) {
     dfree(b->d_tiles); dfree(b->d_order); dfree(b->d_act); dfree(b->d_ck); dfree(b->d_tile_active);
-    dfree(b->d_partial); dfree(b->d_vjp_partial);
+    dfree(b->d_tq_begin); dfree(b->d_iorder); dfree(b->d_iq_begin); dfree(b->d_counters);
     b->ckpt_valid = false;
     if (b->graph_exec) { (void)hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
     if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
@@ -312,7 +315,7 @@ static int plan(atac_batch b, const atac_tuning* t) {
     if (tu.tile_points > 65536) return fail(ATAC_ERR_INVALID, "tile_points must be <= 65536");
     if (tu.detect_block != 128 && tu.detect_block != 256 && tu.detect_block != 512 && tu.detect_block != 1024)
         return fail(ATAC_ERR_INVALID, "detect_block must be 128, 256, 512 or 1024");
-    if (tu.blocks_per_cu < 0 || tu.order < 0 || tu.order > 1) return fail(ATAC_ERR_INVALID, "bad tuning");
+    if (tu.blocks_per_cu < 0 || tu.order < 0 || tu.order > 2) return fail(ATAC_ERR_INVALID, "bad tuning");
     {
         const int B = tu.detect_block;
         const size_t lds = (size_t)(((tu.tile_points + B - 1) / B) * 64) * (B / 64) * sizeof(uint16_t);
@@ -343,19 +346,45 @@ static int plan(atac_batch b, const atac_tuning* t) {
     }
     b->n_slots = slot;
     b->n_tiles = (int32_t)b->tiles.size();
-    std::vector<int32_t> order(b->n_tiles);
+    // work queues: order 0 = grouped by grid and cut into one queue per XCD with
+    // balanced point counts; 1 = one queue in instance order; 2 = one queue grouped by grid
+    b->nq = tu.order == 0 ? kXcds : 1;
+    std::vector<int32_t> order(b->n_tiles), iorder(b->inst_dev.size());
     std::iota(order.begin(), order.end(), 0);
-    if (tu.order == 1)
-        std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
-            return b->inst_dev[b->tiles[x].inst].grid < b->inst_dev[b->tiles[y].inst].grid;
-        });
+    std::iota(iorder.begin(), iorder.end(), 0);
+    if (tu.order != 1) {
+        auto grid_of_tile = [&](int32_t t) { return b->inst_dev[b->tiles[t].inst].grid; };
+        std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return grid_of_tile(x) < grid_of_tile(y); });
+        std::stable_sort(iorder.begin(), iorder.end(), [&](int32_t x, int32_t y) { return b->inst_dev[x].grid < b->inst_dev[y].grid; });
+    }
+    auto cut = [&](const std::vector<int32_t>& items, auto weight) {
+        std::vector<int32_t> begin(b->nq + 1, (int32_t)items.size());
+        begin[0] = 0;
+        double total = 0.0;
+        for (int32_t it : items) total += weight(it);
+        double acc = 0.0;
+        int q = 1;
+        for (size_t k = 0; k < items.size() && q < b->nq; ++k) {
+            acc += weight(items[k]);
+            while (q < b->nq && acc >= total * q / b->nq) begin[q++] = (int32_t)(k + 1);
+        }
+        return begin;
+    };
+    std::vector<int32_t> tq = cut(order, [&](int32_t t) { return (double)b->tiles[t].count; });
+    std::vector<int32_t> iq = cut(iorder, [&](int32_t i) { return (double)b->inst_dev[i].pt_count + 1.0; });
     int rc = dalloc(&b->d_tiles, b->n_tiles);
     if (!rc) rc = dalloc(&b->d_order, b->n_tiles);
+    if (!rc) rc = dalloc(&b->d_tq_begin, b->nq + 1);
+    if (!rc) rc = dalloc(&b->d_iorder, b->inst_dev.size());
+    if (!rc) rc = dalloc(&b->d_iq_begin, b->nq + 1);
+    if (!rc) rc = dalloc(&b->d_counters, 3 * b->nq);
     if (!rc) rc = dalloc(&b->d_tile_active, b->n_tiles);
     if (!rc) rc = dalloc(&b->d_act, (size_t)b->n_slots);
-    if (!rc) rc = dalloc(&b->d_vjp_partial, (size_t)b->n_tiles * kVjpOut);
-    if (!rc) { b->partial_cap = (int64_t)b->n_tiles * 24; rc = dalloc(&b->d_partial, (size_t)b->partial_cap); }
     if (rc) return rc;
+    ATAC_HIP(hipMemcpy(b->d_tq_begin, tq.data(), sizeof(int32_t) * tq.size(), hipMemcpyHostToDevice));
+    ATAC_HIP(hipMemcpy(b->d_iq_begin, iq.data(), sizeof(int32_t) * iq.size(), hipMemcpyHostToDevice));
+    if (!iorder.empty())
+        ATAC_HIP(hipMemcpy(b->d_iorder, iorder.data(), sizeof(int32_t) * iorder.size(), hipMemcpyHostToDevice));
     if (b->n_tiles > 0) {
         ATAC_HIP(hipMemcpy(b->d_tiles, b->tiles.data(), sizeof(TileDev) * b->n_tiles, hipMemcpyHostToDevice));
         ATAC_HIP(hipMemcpy(b->d_order, order.data(), sizeof(int32_t) * b->n_tiles, hipMemcpyHostToDevice));
@@ -372,12 +401,13 @@ static int plan(atac_batch b, const atac_tuning* t) {
     if (tu.blocks_per_cu > 0) nb = std::min(nb, tu.blocks_per_cu);
     nb = std::max(nb, 1);
     b->grid_detect = std::max(1, std::min(b->n_tiles, nb * b->n_cu));
+    const int n_inst = (int)b->inst_dev.size();
     int nc = 0;
     ATAC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nc, contact_kernel_ptr(1, false), 256, 0));
-    b->grid_contact = std::max(1, std::min(b->n_tiles, std::max(nc, 1) * b->n_cu));
+    b->grid_contact = std::max(1, std::min(std::max(n_inst, 1), std::max(nc, 1) * b->n_cu));
     int nv = 0;
     ATAC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nv, vjp_kernel_ptr(false), 256, 0));
-    b->grid_vjp = std::max(1, std::min(b->n_tiles, std::max(nv, 1) * b->n_cu));
+    b->grid_vjp = std::max(1, std::min(std::max(n_inst, 1), std::max(nv, 1) * b->n_cu));
     return ATAC_OK;
 }
 
@@ -406,7 +436,6 @@ int atac_batch_create(const atac_grid* grids, int32_t n_grids, const atac_points
     for (int i = 0; i < n_grids; ++i) gd[i] = b->grids[i]->dev;
     rc = dalloc(&b->d_grids, n_grids);
     if (!rc) rc = dalloc(&b->d_inst, n_inst);
-    if (!rc) rc = dalloc(&b->d_counters, 4);
     if (!rc && n_grids) {
         e = hipMemcpy(b->d_grids, gd.data(), sizeof(GridDev) * n_grids, hipMemcpyHostToDevice);
         if (e != hipSuccess) rc = fail(ATAC_ERR_HIP, hipGetErrorString(e));
@@ -426,7 +455,7 @@ int atac_batch_destroy(atac_batch b) {
     release_plan(b);
     for (auto& u : b->ev_used) { (void)hipEventDestroy(u.second.first); (void)hipEventDestroy(u.second.second); }
     for (auto e : b->ev_pool) (void)hipEventDestroy(e);
-    dfree(b->d_inst); dfree(b->d_grids); dfree(b->d_counters);
+    dfree(b->d_inst); dfree(b->d_grids);
     if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
     delete b;
     return ATAC_OK;
@@ -469,58 +498,48 @@ int atac_batch_info(atac_batch b, int64_t* n_points, int32_t* n_tiles, int32_t* 
 
 static int mode_of(uint32_t flags) { return (flags & ATAC_EVAL_DERIV) ? 2 : ((flags & ATAC_EVAL_P6) ? 1 : 0); }
 
+static QueueDev queues(atac_batch b, bool tiles, int which) {
+    QueueDev q{};
+    q.order = tiles ? b->d_order : b->d_iorder;
+    q.begin = tiles ? b->d_tq_begin : b->d_iq_begin;
+    q.counter = b->d_counters + which * b->nq;
+    q.nq = b->nq;
+    return q;
+}
+
 static int enqueue_eval(atac_batch b, uint32_t flags, const atac_outputs* out, hipStream_t st) {
     if (flags & ~(ATAC_EVAL_DERIV | ATAC_EVAL_P6 | ATAC_EVAL_CHECKPOINT | ATAC_EVAL_PRECISE))
         return fail(ATAC_ERR_INVALID, "atac_batch_eval: unknown flag");
     const int mode = mode_of(flags);
     const bool precise = (flags & ATAC_EVAL_PRECISE) != 0;
-    const int stride = mode == 2 ? 24 + 288 : 24;
-    if ((int64_t)stride * b->n_tiles > b->partial_cap) {
-        dfree(b->d_partial);
-        b->partial_cap = (int64_t)stride * b->n_tiles;
-        int rc = dalloc(&b->d_partial, (size_t)b->partial_cap);
-        if (rc) return rc;
-    }
+    if (out && out->dfm_dp6 && mode < 1) return fail(ATAC_ERR_INVALID, "dfm_dp6 requested without ATAC_EVAL_P6 or ATAC_EVAL_DERIV");
+    if (out && (out->Km || out->Dm) && mode < 2) return fail(ATAC_ERR_INVALID, "Km/Dm requested without ATAC_EVAL_DERIV");
     if ((flags & ATAC_EVAL_CHECKPOINT) && !b->d_ck && b->n_slots > 0) {
         int rc = dalloc(&b->d_ck, (size_t)b->n_slots);
         if (rc) return rc;
     }
+    if (b->inst.empty()) return ATAC_OK;
+    ATAC_HIP(launch_reset(b->d_counters, 2 * b->nq, st));
     if (b->n_tiles > 0) {
-        ATAC_HIP(launch_reset(b->d_counters, 2, st));
         DetectArgs da{};
-        da.tiles = b->d_tiles; da.order = b->d_order; da.n_tiles = b->n_tiles; da.wave_seg = b->wave_seg;
-        da.counter = b->d_counters; da.inst = b->d_inst; da.grids = b->d_grids;
+        da.q = queues(b, true, 0);
+        da.tiles = b->d_tiles; da.wave_seg = b->wave_seg; da.inst = b->d_inst; da.grids = b->d_grids;
         da.act_off = b->d_act; da.tile_active = b->d_tile_active;
         hipEvent_t t0 = stage_begin(b, st);
         ATAC_HIP(launch_detect(da, b->tune.detect_block, b->grid_detect, b->lds_detect, st));
         stage_end(b, ST_DETECT, t0, st);
-        ContactArgs ca{};
-        ca.tiles = b->d_tiles; ca.order = b->d_order; ca.n_tiles = b->n_tiles; ca.stride = stride;
-        ca.counter = b->d_counters + 1; ca.inst = b->d_inst; ca.grids = b->d_grids;
-        ca.act_off = b->d_act; ca.tile_active = b->d_tile_active; ca.partial = b->d_partial;
-        ca.ck_dg = (flags & ATAC_EVAL_CHECKPOINT) ? b->d_ck : nullptr;
-        hipEvent_t t1 = stage_begin(b, st);
-        ATAC_HIP(launch_contact(ca, mode, precise, b->grid_contact, st));
-        stage_end(b, ST_CONTACT, t1, st);
     }
-    ReduceArgs ra{};
-    ra.inst = b->d_inst; ra.n_inst = (int32_t)b->inst.size(); ra.stride = stride; ra.partial = b->d_partial;
-    ra.tile_active = b->d_tile_active; ra.n_active = out ? out->n_active : nullptr;
-    int ns = 0;
-    if (out && out->fm) ra.seg[ns++] = ReduceSeg{0, 12, out->fm};
-    if (out && out->dfm_dp6) {
-        if (mode < 1) return fail(ATAC_ERR_INVALID, "dfm_dp6 requested without ATAC_EVAL_P6 or ATAC_EVAL_DERIV");
-        ra.seg[ns++] = ReduceSeg{12, 12, out->dfm_dp6};
+    ContactArgs ca{};
+    ca.q = queues(b, false, 1);
+    ca.tiles = b->d_tiles; ca.inst = b->d_inst; ca.grids = b->d_grids;
+    ca.act_off = b->d_act; ca.tile_active = b->d_tile_active;
+    ca.ck_dg = (flags & ATAC_EVAL_CHECKPOINT) ? b->d_ck : nullptr;
+    if (out) {
+        ca.fm = out->fm; ca.p6 = out->dfm_dp6; ca.Km = out->Km; ca.Dm = out->Dm; ca.n_active = out->n_active;
     }
-    if (out && (out->Km || out->Dm) && mode < 2) return fail(ATAC_ERR_INVALID, "Km/Dm requested without ATAC_EVAL_DERIV");
-    if (out && out->Km) ra.seg[ns++] = ReduceSeg{24, 144, out->Km};
-    if (out && out->Dm) ra.seg[ns++] = ReduceSeg{168, 144, out->Dm};
-    ra.nseg = ns;
-    if (ns || ra.n_active) {
-        hipEvent_t t2 = stage_begin(b, st);
-        ATAC_HIP(launch_reduce(ra, st));
-        stage_end(b, ST_REDUCE, t2, st);
-    }
+    hipEvent_t t1 = stage_begin(b, st);
+    ATAC_HIP(launch_contact(ca, mode, precise, b->grid_contact, st));
+    stage_end(b, ST_CONTACT, t1, st);
     b->ckpt_valid = (flags & ATAC_EVAL_CHECKPOINT) != 0;
     b->ckpt_precise = precise;
     return ATAC_OK;
@@ -529,31 +548,19 @@ static int enqueue_eval(atac_batch b, uint32_t flags, const atac_outputs* out, h
 static int enqueue_vjp(atac_batch b, const double* g_fm, const atac_gradients* g, hipStream_t st) {
     if (!b->ckpt_valid) return fail(ATAC_ERR_STATE, "atac_batch_vjp: the last evaluation kept no checkpoint (ATAC_EVAL_CHECKPOINT)");
     if (!g_fm) return fail(ATAC_ERR_INVALID, "atac_batch_vjp: g_fm is null");
-    if (b->n_tiles > 0) {
-        ATAC_HIP(launch_reset(b->d_counters + 2, 1, st));
-        VjpArgs va{};
-        va.tiles = b->d_tiles; va.order = b->d_order; va.n_tiles = b->n_tiles; va.counter = b->d_counters + 2;
-        va.inst = b->d_inst; va.grids = b->d_grids; va.tile_active = b->d_tile_active; va.act_off = b->d_act;
-        va.ck_dg = b->d_ck; va.g_fm = g_fm; va.partial = b->d_vjp_partial;
-        hipEvent_t t3 = stage_begin(b, st);
-        ATAC_HIP(launch_vjp(va, b->ckpt_precise, b->grid_vjp, st));
-        stage_end(b, ST_VJP, t3, st);
+    if (b->inst.empty()) return ATAC_OK;
+    ATAC_HIP(launch_reset(b->d_counters + 2 * b->nq, b->nq, st));
+    VjpArgs va{};
+    va.q = queues(b, false, 2);
+    va.tiles = b->d_tiles; va.inst = b->d_inst; va.grids = b->d_grids; va.tile_active = b->d_tile_active;
+    va.act_off = b->d_act; va.ck_dg = b->d_ck; va.g_fm = g_fm;
+    if (g) {
+        va.g_q1 = g->g_q1; va.g_q2 = g->g_q2; va.g_phi1 = g->g_phi1; va.g_phi2 = g->g_phi2; va.g_kn = g->g_kn;
+        va.g_kt = g->g_kt;
     }
-    ReduceArgs ra{};
-    ra.inst = b->d_inst; ra.n_inst = (int32_t)b->inst.size(); ra.stride = kVjpOut; ra.partial = b->d_vjp_partial;
-    int ns = 0;
-    if (g && g->g_q1) ra.seg[ns++] = ReduceSeg{0, 6, g->g_q1};
-    if (g && g->g_q2) ra.seg[ns++] = ReduceSeg{6, 6, g->g_q2};
-    if (g && g->g_phi1) ra.seg[ns++] = ReduceSeg{12, 6, g->g_phi1};
-    if (g && g->g_phi2) ra.seg[ns++] = ReduceSeg{18, 6, g->g_phi2};
-    if (g && g->g_kn) ra.seg[ns++] = ReduceSeg{24, 1, g->g_kn};
-    if (g && g->g_kt) ra.seg[ns++] = ReduceSeg{25, 1, g->g_kt};
-    ra.nseg = ns;
-    if (ns) {
-        hipEvent_t t4 = stage_begin(b, st);
-        ATAC_HIP(launch_reduce(ra, st));
-        stage_end(b, ST_VJP_REDUCE, t4, st);
-    }
+    hipEvent_t t3 = stage_begin(b, st);
+    ATAC_HIP(launch_vjp(va, b->ckpt_precise, b->grid_vjp, st));
+    stage_end(b, ST_VJP, t3, st);
     return ATAC_OK;
 }
 
@@ -574,15 +581,6 @@ int atac_batch_graph_capture(atac_batch b, uint32_t flags, const atac_outputs* o
     if ((flags & ATAC_EVAL_CHECKPOINT) && !b->d_ck && b->n_slots > 0) {
         int rc = dalloc(&b->d_ck, (size_t)b->n_slots);
         if (rc) return rc;
-    }
-    {   // no allocation may happen while the stream is capturing
-        const int stride = mode_of(flags) == 2 ? 24 + 288 : 24;
-        if ((int64_t)stride * b->n_tiles > b->partial_cap) {
-            dfree(b->d_partial);
-            b->partial_cap = (int64_t)stride * b->n_tiles;
-            int rc = dalloc(&b->d_partial, (size_t)b->partial_cap);
-            if (rc) return rc;
-        }
     }
     if (b->graph_exec) { (void)hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
     if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }

@@ -5,63 +5,56 @@
 
 namespace atac {
 
+// Work queues: items order[begin[q] .. begin[q+1]) belong to queue q.
+struct QueueDev {
+    const int32_t* order;
+    const int32_t* begin;   // nq + 1 entries
+    int32_t* counter;       // nq heads (reset before each launch)
+    int32_t nq;
+    int32_t pad;
+};
+
 struct DetectArgs {
+    QueueDev q;                 // tiles
     const TileDev* tiles;
-    const int32_t* order;       // processing order of the tiles
-    int32_t n_tiles;
     int32_t wave_seg;           // LDS queue entries per wavefront
-    int32_t* counter;           // persistent work-queue head (reset before launch)
+    int32_t pad;
     const InstDev* inst;
     const GridDev* grids;
-    uint32_t* act_off;          // active offsets, tile slice starts at tile.begin
+    uint32_t* act_off;          // active offsets, tile slice starts at tile.slot
     int32_t* tile_active;       // n_tiles
 };
 
 struct ContactArgs {
+    QueueDev q;                 // instances
     const TileDev* tiles;
-    const int32_t* order;
-    int32_t n_tiles;
-    int32_t stride;             // doubles per tile partial
-    int32_t* counter;
     const InstDev* inst;
     const GridDev* grids;
     const uint32_t* act_off;
     const int32_t* tile_active;
-    double* partial;            // n_tiles x stride
     CkptDev* ck_dg;             // per active point (d, grad phi); nullptr: not kept
+    double* fm;                 // outputs per instance (any may be nullptr)
+    double* p6;
+    double* Km;
+    double* Dm;
+    int32_t* n_active;
 };
 
 struct VjpArgs {
+    QueueDev q;                 // instances
     const TileDev* tiles;
-    const int32_t* order;
-    int32_t n_tiles;
-    int32_t pad;
-    int32_t* counter;
     const InstDev* inst;
     const GridDev* grids;
     const int32_t* tile_active;
     const uint32_t* act_off;
     const CkptDev* ck_dg;
     const double* g_fm;         // n_inst x 12
-    double* partial;            // n_tiles x kVjpOut
-};
-
-struct ReduceSeg {
-    int32_t off;
-    int32_t count;
-    double* dst;
-};
-
-struct ReduceArgs {
-    const InstDev* inst;
-    int32_t n_inst;
-    int32_t stride;
-    const double* partial;
-    ReduceSeg seg[6];
-    int32_t nseg;
-    int32_t pad;
-    const int32_t* tile_active;
-    int32_t* n_active;
+    double* g_q1;
+    double* g_q2;
+    double* g_phi1;
+    double* g_phi2;
+    double* g_kn;
+    double* g_kt;
 };
 
 struct PlainArgs {
@@ -101,7 +94,6 @@ hipError_t launch_vjp(const VjpArgs& a, bool precise, int grid, hipStream_t st);
 const void* vjp_kernel_ptr(bool precise);
 hipError_t launch_reset(int32_t* c, int n, hipStream_t st);
 hipError_t launch_zero(void* p, int64_t n_words, hipStream_t st);
-hipError_t launch_reduce(const ReduceArgs& a, hipStream_t st);
 hipError_t launch_plain(const PlainArgs& a, int64_t n_points, bool precise, hipStream_t st);
 hipError_t launch_distance(const PointArgs& a, hipStream_t st);
 hipError_t launch_collision(const PointArgs& a, bool precise, hipStream_t st);

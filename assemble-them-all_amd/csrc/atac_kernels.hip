@@ -8,11 +8,12 @@
 //                  corners and compacts the points with d < 0 (per-wave LDS
 //                  segments filled by ballot + popcount, deterministic order)
 //                  into the tile's slice of a global active list.
-//   contact_kernel the full contact (and optionally the Km/Dm tangent columns)
-//                  of the active points with every lane busy, compensated FP32
-//                  sums, DPP wave reductions, one FP64 partial per tile.
-//   vjp_kernel    the adjoint over the checkpointed active contacts only.
-//   reduce_kernel per-instance FP64 sum of the tile partials in tile order.
+//   contact_kernel one force object per work item: the full contact (and
+//                  optionally the Km/Dm tangent columns) of its active points
+//                  with every lane busy, compensated FP32 sums, DPP wave
+//                  reductions, the FP64 result written in a fixed order.
+//   vjp_kernel     the adjoint over the checkpointed active contacts only.
+// Work items come from per-XCD queues (next_item).
 // Plain path (the one-launch-per-instance baseline): plain_kernel, one lane per
 // point, inline contact math, FP64 atomics.
 #include <hip/hip_runtime.h>
@@ -75,6 +76,23 @@ struct Acc<double> {
 
 __device__ __forceinline__ float stream_load(const float* p) { return __builtin_nontemporal_load(p); }
 
+// ------------------------------------------------------------------ work queues
+// Items (tiles or instances) are split into nq contiguous queues; workgroup b
+// starts on queue b % nq (the XCD it was dispatched to, with round-robin
+// dispatch over the XCDs) and moves to the next queue when its own is drained.
+// Called by one thread; qid/tried persist in that thread across calls. Returns
+// -1 when every queue is empty, so every workgroup's loop ends.
+__device__ __forceinline__ int next_item(const QueueDev& Q, int& qid, int& tried) {
+    while (tried < Q.nq) {
+        const int s = atomicAdd(Q.counter + qid, 1);
+        const int b = Q.begin[qid], len = Q.begin[qid + 1] - b;
+        if (s < len) return Q.order[b + s];
+        qid = (qid + 1 == Q.nq) ? 0 : qid + 1;
+        ++tried;
+    }
+    return -1;
+}
+
 // ------------------------------------------------------------------ batched forward
 // Phase 1, streaming: every point of the tile is tested; active offsets are
 // compacted into per-wave LDS segments (ballot + popcount, deterministic order)
@@ -83,16 +101,16 @@ template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void detect_kernel(DetectArgs a) {
     constexpr int NW = BLOCK / 64;
     extern __shared__ uint16_t s_queue[];   // tile offsets (tiles hold at most 65536 points)
-    __shared__ int s_slot;
+    __shared__ int s_item;
     __shared__ int s_wcount[NW];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int seg = a.wave_seg;
+    int qid = blockIdx.x % a.q.nq, tried = 0;
     for (;;) {
-        if (tid == 0) s_slot = atomicAdd(a.counter, 1);
+        if (tid == 0) s_item = next_item(a.q, qid, tried);
         __syncthreads();
-        const int slot = s_slot;
-        if (slot >= a.n_tiles) break;
-        const int tile = a.order[slot];
+        const int tile = s_item;
+        if (tile < 0) break;
         const TileDev t = a.tiles[tile];
         const InstDev& in = a.inst[t.inst];
         const GridDev G = a.grids[in.grid];
@@ -124,33 +142,65 @@ __global__ __launch_bounds__(BLOCK) void detect_kernel(DetectArgs a) {
     }
 }
 
+// The active contacts of one instance, flattened over its tiles: an exclusive
+// prefix of the tiles' active counts in LDS maps a flat index to (tile, entry).
+// Instances with more than kMaxTilesPerPass tiles are walked in passes.
+constexpr int kMaxTilesPerPass = 256;
+
+template <int BLOCK>
+struct ActiveList {
+    int32_t* pre;     // LDS, kMaxTilesPerPass + 1 entries
+    int t0, nt;       // tiles [t0, t0 + nt) of this pass
+    int total;
+    // fills pre[] for tiles t0 .. t0+nt-1 of the instance (all threads call)
+    __device__ void load(const int32_t* tile_active, int tile_first, int pass_t0, int pass_nt) {
+        t0 = pass_t0;
+        nt = pass_nt;
+        const int tid = threadIdx.x;
+        for (int k = tid; k < kMaxTilesPerPass + 1; k += BLOCK) pre[k] = 0;
+        __syncthreads();
+        for (int k = tid; k < nt; k += BLOCK) pre[k + 1] = tile_active[tile_first + t0 + k];
+        __syncthreads();
+        if (tid == 0)
+            for (int k = 0; k < nt; ++k) pre[k + 1] += pre[k];
+        __syncthreads();
+        total = pre[nt];
+    }
+    // tile index (relative to t0) holding flat entry q
+    __device__ int tile_of(int q) const {
+        int lo = 0, hi = nt - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (pre[mid] <= q) lo = mid; else hi = mid - 1;
+        }
+        return lo;
+    }
+};
+
 // Phase 2: full contact (wrench, p6 column, optionally Km/Dm tangent columns)
-// of the active points of each tile; one FP64 partial per tile.
+// of the active points of one instance per work item; final FP64 results are
+// written directly (fixed summation order: tile order, then queue order).
 template <int BLOCK, int MODE, typename R>
 __global__ __launch_bounds__(BLOCK) void contact_kernel(ContactArgs a) {
     constexpr int NW = BLOCK / 64;
     constexpr bool P6 = MODE >= 1;
     constexpr bool DERIV = MODE >= 2;
     constexpr int NACC = P6 ? 24 : 12;
-    __shared__ int s_slot;
+    __shared__ int s_item;
+    __shared__ int32_t s_pre[kMaxTilesPerPass + 1];
     __shared__ double s_red[NW][NACC];
     __shared__ double s_km[DERIV ? NW : 1][DERIV ? 288 : 1];
+    __shared__ int s_cnt;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    int qid = blockIdx.x % a.q.nq, tried = 0;
+    ActiveList<BLOCK> L;
+    L.pre = s_pre;
     for (;;) {
-        if (tid == 0) s_slot = atomicAdd(a.counter, 1);
+        if (tid == 0) s_item = next_item(a.q, qid, tried);
         __syncthreads();
-        const int slot = s_slot;
-        if (slot >= a.n_tiles) break;
-        const int tile = a.order[slot];
-        const TileDev t = a.tiles[tile];
-        const int cnt = a.tile_active[tile];
-        double* out = a.partial + (int64_t)tile * a.stride;
-        if (cnt == 0) {
-            for (int k = tid; k < a.stride; k += BLOCK) out[k] = 0.0;
-            __syncthreads();
-            continue;
-        }
-        const InstDev& in = a.inst[t.inst];
+        const int i = s_item;
+        if (i < 0) break;
+        const InstDev& in = a.inst[i];
         const GridDev G = a.grids[in.grid];
         if (DERIV)
             for (int k = lane; k < 288; k += 64) s_km[DERIV ? w : 0][DERIV ? k : 0] = 0.0;
@@ -158,115 +208,129 @@ __global__ __launch_bounds__(BLOCK) void contact_kernel(ContactArgs a) {
 #pragma unroll
         for (int k = 0; k < NACC; ++k) acc[k].zero();
         const InstR<R> I = make_inst<R>(in);
-        for (int base = 0; base < cnt; base += BLOCK) {
-            const int qi = base + tid;
-            const bool have = qi < cnt;
-            ContactFwd<R> s;
-            if (have) {
-                const uint32_t o = a.act_off[t.slot + qi];
-                const int64_t p = t.begin + o;
-                contact_sample<R>(G, in, (double)in.px[p], (double)in.py[p], (double)in.pz[p], s);
-                R p6[12];
-                contact_forward<R>(I, s, P6 ? p6 : nullptr);
+        int n_active = 0;
+        for (int pt0 = 0; pt0 < in.tile_count; pt0 += kMaxTilesPerPass) {
+            L.load(a.tile_active, in.tile_begin, pt0, min(kMaxTilesPerPass, in.tile_count - pt0));
+            n_active += L.total;
+            for (int base = 0; base < L.total; base += BLOCK) {
+                const int qi = base + tid;
+                const bool have = qi < L.total;
+                ContactFwd<R> s;
+                if (have) {
+                    const int tt = L.tile_of(qi);
+                    const TileDev t = a.tiles[in.tile_begin + L.t0 + tt];
+                    const int64_t slot = t.slot + (qi - L.pre[tt]);
+                    const int64_t p = t.begin + a.act_off[slot];
+                    contact_sample<R>(G, in, (double)in.px[p], (double)in.py[p], (double)in.pz[p], s);
+                    R p6[12];
+                    contact_forward<R>(I, s, P6 ? p6 : nullptr);
 #pragma unroll
-                for (int k = 0; k < 12; ++k) acc[k].add(s.f[k]);
-                if (P6) {
+                    for (int k = 0; k < 12; ++k) acc[k].add(s.f[k]);
+                    if (P6) {
 #pragma unroll
-                    for (int k = 0; k < 12; ++k) acc[P6 ? 12 + k : 0].add(p6[k]);
+                        for (int k = 0; k < 12; ++k) acc[P6 ? 12 + k : 0].add(p6[k]);
+                    }
+                    if (a.ck_dg) {
+                        const float hi = (float)s.d;
+                        a.ck_dg[slot] = CkptDev{hi, (float)((double)s.d - (double)hi), (float)s.g.x, (float)s.g.y,
+                                                (float)s.g.z};
+                    }
                 }
-                if (a.ck_dg) {
-                    const float hi = (float)s.d;
-                    a.ck_dg[t.slot + qi] = CkptDev{hi, (float)((double)s.d - (double)hi), (float)s.g.x, (float)s.g.y,
-                                                   (float)s.g.z};
-                }
-            }
-            if (DERIV) {
-                // columns of Km (j < 12) and Dm (j >= 12); every lane takes part in the sums
-                for (int j = 0; j < 24; ++j) {
-                    R col[12];
+                if (DERIV) {
+                    // columns of Km (j < 12) and Dm (j >= 12); every lane takes part in the sums
+                    for (int j = 0; j < 24; ++j) {
+                        R col[12];
 #pragma unroll
-                    for (int r = 0; r < 12; ++r) col[r] = (R)0;
-                    if (have) contact_jvp<R>(I, s, unit_tangent<R>(j), col);
-                    const int obase = (j < 12) ? j : 144 + (j - 12);
+                        for (int r = 0; r < 12; ++r) col[r] = (R)0;
+                        if (have) contact_jvp<R>(I, s, unit_tangent<R>(j), col);
+                        const int obase = (j < 12) ? j : 144 + (j - 12);
 #pragma unroll
-                    for (int r = 0; r < 12; ++r) {
-                        const R v = wave_sum(col[r]);
-                        if (lane == 0) s_km[DERIV ? w : 0][DERIV ? obase + 12 * r : 0] += (double)v;
+                        for (int r = 0; r < 12; ++r) {
+                            const R v = wave_sum(col[r]);
+                            if (lane == 0) s_km[DERIV ? w : 0][DERIV ? obase + 12 * r : 0] += (double)v;
+                        }
                     }
                 }
             }
+            __syncthreads();   // s_pre is reloaded by the next pass
         }
 #pragma unroll
         for (int k = 0; k < NACC; ++k) {
             const double v = wave_sum(acc[k].value());
             if (lane == 0) s_red[w][k] = v;
         }
+        if (tid == 0) s_cnt = n_active;
         __syncthreads();
         if (tid < NACC) {
             double sum = 0.0;
 #pragma unroll
             for (int k = 0; k < NW; ++k) sum += s_red[k][tid];
-            out[tid] = sum;
+            double* dst = tid < 12 ? a.fm : a.p6;
+            if (dst) dst[(int64_t)i * 12 + (tid % 12)] = sum;
         }
         if (DERIV) {
             for (int k = tid; k < 288; k += BLOCK) {
                 double sum = 0.0;
 #pragma unroll
                 for (int ww = 0; ww < NW; ++ww) sum += s_km[DERIV ? ww : 0][DERIV ? k : 0];
-                out[24 + k] = sum;
+                double* dst = k < 144 ? a.Km : a.Dm;
+                if (dst) dst[(int64_t)i * 144 + (k % 144)] = sum;
             }
         }
+        if (tid == 0 && a.n_active) a.n_active[i] = s_cnt;
         __syncthreads();
     }
 }
 
 // ------------------------------------------------------------------ batched adjoint
+// One instance per work item over its checkpointed active contacts.
 template <int BLOCK, typename R>
 __global__ __launch_bounds__(BLOCK) void vjp_kernel(VjpArgs a) {
     constexpr int NW = BLOCK / 64;
-    __shared__ int s_slot;
+    __shared__ int s_item;
+    __shared__ int32_t s_pre[kMaxTilesPerPass + 1];
     __shared__ double s_red[NW][kVjpOut];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    int qid = blockIdx.x % a.q.nq, tried = 0;
+    ActiveList<BLOCK> L;
+    L.pre = s_pre;
     for (;;) {
-        if (tid == 0) s_slot = atomicAdd(a.counter, 1);
+        if (tid == 0) s_item = next_item(a.q, qid, tried);
         __syncthreads();
-        const int slot = s_slot;
-        if (slot >= a.n_tiles) break;
-        const int tile = a.order[slot];
-        const TileDev t = a.tiles[tile];
-        const int cnt = a.tile_active[tile];
-        double* out = a.partial + (int64_t)tile * kVjpOut;
-        if (cnt == 0) {
-            if (tid < kVjpOut) out[tid] = 0.0;
-            __syncthreads();
-            continue;
-        }
-        const InstDev& in = a.inst[t.inst];
+        const int i = s_item;
+        if (i < 0) break;
+        const InstDev& in = a.inst[i];
         const GridDev G = a.grids[in.grid];
         const InstR<R> I = make_inst<R>(in);
         R gb[12];
 #pragma unroll
-        for (int k = 0; k < 12; ++k) gb[k] = (R)a.g_fm[(int64_t)t.inst * 12 + k];
-        Acc<R> acc[kVjpOut];
+        for (int k = 0; k < 12; ++k) gb[k] = (R)a.g_fm[(int64_t)i * 12 + k];
+        R acc[kVjpOut];
 #pragma unroll
-        for (int k = 0; k < kVjpOut; ++k) acc[k].zero();
-        for (int qi = tid; qi < cnt; qi += BLOCK) {
-            const uint32_t o = a.act_off[t.slot + qi];
-            const CkptDev ck = a.ck_dg[t.slot + qi];
-            const int64_t p = t.begin + o;
-            ContactFwd<R> s;
-            contact_sample<R>(G, in, (double)in.px[p], (double)in.py[p], (double)in.pz[p], s, &ck);
-            contact_forward<R>(I, s, nullptr);
-            R v[kVjpOut];
-            contact_vjp<R>(I, s, gb, v);
-            v[24] *= (R)in.kn_chain;   // d/d kn = kn_chain * d/d kn_eff (ForceGeneralSDFContact.cpp:155-156)
-            v[25] *= (R)in.kn_chain;
+        for (int k = 0; k < kVjpOut; ++k) acc[k] = (R)0;
+        for (int pt0 = 0; pt0 < in.tile_count; pt0 += kMaxTilesPerPass) {
+            L.load(a.tile_active, in.tile_begin, pt0, min(kMaxTilesPerPass, in.tile_count - pt0));
+            for (int qi = tid; qi < L.total; qi += BLOCK) {
+                const int tt = L.tile_of(qi);
+                const TileDev t = a.tiles[in.tile_begin + L.t0 + tt];
+                const int64_t slot = t.slot + (qi - L.pre[tt]);
+                const int64_t p = t.begin + a.act_off[slot];
+                const CkptDev ck = a.ck_dg[slot];
+                ContactFwd<R> s;
+                contact_sample<R>(G, in, (double)in.px[p], (double)in.py[p], (double)in.pz[p], s, &ck);
+                contact_forward<R>(I, s, nullptr);
+                R v[kVjpOut];
+                contact_vjp<R>(I, s, gb, v);
+                v[24] *= (R)in.kn_chain;   // d/d kn = kn_chain * d/d kn_eff (ForceGeneralSDFContact.cpp:155-156)
+                v[25] *= (R)in.kn_chain;
 #pragma unroll
-            for (int k = 0; k < kVjpOut; ++k) acc[k].add(v[k]);
+                for (int k = 0; k < kVjpOut; ++k) acc[k] += v[k];
+            }
+            __syncthreads();
         }
 #pragma unroll
         for (int k = 0; k < kVjpOut; ++k) {
-            const double v = wave_sum(acc[k].value());
+            const double v = wave_sum((double)acc[k]);
             if (lane == 0) s_red[w][k] = v;
         }
         __syncthreads();
@@ -274,7 +338,15 @@ __global__ __launch_bounds__(BLOCK) void vjp_kernel(VjpArgs a) {
             double sum = 0.0;
 #pragma unroll
             for (int k = 0; k < NW; ++k) sum += s_red[k][tid];
-            out[tid] = sum;
+            double* dst = nullptr;
+            int col = 0, width = 6;
+            if (tid < 6) { dst = a.g_q1; col = tid; }
+            else if (tid < 12) { dst = a.g_q2; col = tid - 6; }
+            else if (tid < 18) { dst = a.g_phi1; col = tid - 12; }
+            else if (tid < 24) { dst = a.g_phi2; col = tid - 18; }
+            else if (tid == 24) { dst = a.g_kn; col = 0; width = 1; }
+            else { dst = a.g_kt; col = 0; width = 1; }
+            if (dst) dst[(int64_t)i * width + col] = sum;
         }
         __syncthreads();
     }
@@ -287,26 +359,6 @@ __global__ void reset_counters_kernel(int32_t* c, int n) {
 
 __global__ void zero_words_kernel(uint32_t* p, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0u;
-}
-
-// out[i][k] = sum over the tiles of instance i (in tile order) of partial[tile][off + k]
-__global__ void reduce_kernel(ReduceArgs a) {
-    const int i = blockIdx.x;
-    if (i >= a.n_inst) return;
-    const int tb = a.inst[i].tile_begin, tc = a.inst[i].tile_count;
-    for (int s = 0; s < a.nseg; ++s) {
-        const ReduceSeg sg = a.seg[s];
-        for (int k = threadIdx.x; k < sg.count; k += blockDim.x) {
-            double sum = 0.0;
-            for (int t = tb; t < tb + tc; ++t) sum += a.partial[(int64_t)t * a.stride + sg.off + k];
-            sg.dst[(int64_t)i * sg.count + k] = sum;
-        }
-    }
-    if (a.n_active && threadIdx.x == 0) {
-        int c = 0;
-        for (int t = tb; t < tb + tc; ++t) c += a.tile_active[t];
-        a.n_active[i] = c;
-    }
 }
 
 // ------------------------------------------------------------------ plain per-instance path
@@ -520,12 +572,6 @@ hipError_t launch_zero(void* p, int64_t n_words, hipStream_t st) {
     if (n_words <= 0) return hipSuccess;
     const int64_t blocks = std::min<int64_t>((n_words + 255) / 256, 4096);
     hipLaunchKernelGGL(zero_words_kernel, dim3((unsigned)blocks), dim3(256), 0, st, (uint32_t*)p, n_words);
-    return hipGetLastError();
-}
-
-hipError_t launch_reduce(const ReduceArgs& a, hipStream_t st) {
-    if (a.n_inst <= 0) return hipSuccess;
-    hipLaunchKernelGGL(reduce_kernel, dim3(a.n_inst), dim3(64), 0, st, a);
     return hipGetLastError();
 }
 

@@ -55,7 +55,8 @@ typedef struct {
     int32_t tile_points;   /* points per tile (0: 4096); at most 65536 */
     int32_t detect_block;  /* threads per detection workgroup: 128, 256, 512, 1024 (0: 256) */
     int32_t blocks_per_cu; /* persistent workgroups per CU (0: occupancy limit) */
-    int32_t order;         /* tile processing order: 0 instance order, 1 grouped by grid */
+    int32_t order;         /* work queues: 0 one per XCD, items grouped by grid (default);
+                              1 one queue in instance order; 2 one queue grouped by grid */
 } atac_tuning;
 
 /* flags of atac_batch_eval / atac_eval_plain */

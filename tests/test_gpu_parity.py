@@ -106,7 +106,8 @@ def test_tuning_invariance(peg, peg_batch):
     base = _np(peg_batch.eval(p6=True))
     try:
         for tu in ({"tile_points": 256, "detect_block": 128}, {"tile_points": 1024, "detect_block": 1024},
-                   {"tile_points": 16384, "detect_block": 512, "order": 1}, {"tile_points": 4096, "blocks_per_cu": 1}):
+                   {"tile_points": 16384, "detect_block": 512, "order": 1}, {"tile_points": 4096, "blocks_per_cu": 1},
+                   {"order": 2}):
             peg_batch.retune(tu)
             o = _np(peg_batch.eval(p6=True))
             assert np.array_equal(o["n_active"], base["n_active"])
