@@ -18,6 +18,7 @@ EVAL_DERIV = 1
 EVAL_P6 = 2
 EVAL_CHECKPOINT = 4
 EVAL_PRECISE = 8
+EVAL_ALL_ACTIVE = 16
 
 EXPORTED = [
     "atac_last_error", "atac_version", "atac_device_init",
@@ -27,6 +28,7 @@ EXPORTED = [
     "atac_batch_eval", "atac_batch_vjp", "atac_batch_graph_capture", "atac_batch_graph_launch",
     "atac_batch_active_ids", "atac_batch_touched_values", "atac_eval_plain", "atac_batch_timing",
     "atac_sdf_distance", "atac_sdf_collision",
+    "atac_batch_eval_host", "atac_batch_vjp_host", "atac_sdf_distance_host", "atac_sdf_collision_host",
 ]
 
 
@@ -100,6 +102,10 @@ def lib():
     L.atac_batch_timing.argtypes = [vp, i32, vp, vp]
     L.atac_sdf_distance.argtypes = [vp, dp, C.c_double, vp, i64, vp, vp]
     L.atac_sdf_collision.argtypes = [vp, dp, dp, C.c_double, u32, vp, vp, i64, vp, vp, vp]
+    L.atac_batch_eval_host.argtypes = [vp, u32, vp, vp, vp, vp, vp]
+    L.atac_batch_vjp_host.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
+    L.atac_sdf_distance_host.argtypes = [vp, dp, C.c_double, vp, i64, vp]
+    L.atac_sdf_collision_host.argtypes = [vp, dp, dp, C.c_double, u32, vp, vp, i64, vp, vp]
     for name in EXPORTED:
         if name not in ("atac_last_error",):
             getattr(L, name).restype = C.c_int

@@ -214,14 +214,15 @@ class ContactBatch:
         return o
 
     @staticmethod
-    def flags(deriv=False, p6=False, checkpoint=False, precise=False):
+    def flags(deriv=False, p6=False, checkpoint=False, precise=False, all_active=False):
         return ((L.EVAL_DERIV if deriv else 0) | (L.EVAL_P6 if p6 else 0) | (L.EVAL_CHECKPOINT if checkpoint else 0)
-                | (L.EVAL_PRECISE if precise else 0))
+                | (L.EVAL_PRECISE if precise else 0) | (L.EVAL_ALL_ACTIVE if all_active else 0))
 
-    def eval(self, deriv=False, p6=False, checkpoint=False, precise=False, out=None):
+    def eval(self, deriv=False, p6=False, checkpoint=False, precise=False, all_active=False, out=None):
         out = out if out is not None else self.alloc_outputs(deriv, p6)
         o = self._outputs_struct(out)
-        check(lib().atac_batch_eval(self.handle, self.flags(deriv, p6, checkpoint, precise), C.byref(o), _stream()))
+        check(lib().atac_batch_eval(self.handle, self.flags(deriv, p6, checkpoint, precise, all_active), C.byref(o),
+                                    _stream()))
         return out
 
     def eval_plain(self, deriv=False, p6=False, precise=False, out=None):

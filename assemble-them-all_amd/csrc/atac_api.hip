@@ -95,6 +95,11 @@ struct atac_batch_s {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     hipStream_t cap_stream = nullptr;
+    // device staging for the synchronous host-buffer entry points
+    double* h_out = nullptr;      // fm 12 | p6 12 | Km 144 | Dm 144 per instance
+    int32_t* h_nact = nullptr;
+    double* h_gfm = nullptr;      // 12 per instance
+    double* h_grad = nullptr;     // 26 per instance
     // optional per-stage timing: event pairs recorded around each kernel launch
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -456,6 +461,7 @@ int atac_batch_destroy(atac_batch b) {
     for (auto& u : b->ev_used) { (void)hipEventDestroy(u.second.first); (void)hipEventDestroy(u.second.second); }
     for (auto e : b->ev_pool) (void)hipEventDestroy(e);
     dfree(b->d_inst); dfree(b->d_grids);
+    dfree(b->h_out); dfree(b->h_nact); dfree(b->h_gfm); dfree(b->h_grad);
     if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
     delete b;
     return ATAC_OK;
@@ -508,7 +514,7 @@ static QueueDev queues(atac_batch b, bool tiles, int which) {
 }
 
 static int enqueue_eval(atac_batch b, uint32_t flags, const atac_outputs* out, hipStream_t st) {
-    if (flags & ~(ATAC_EVAL_DERIV | ATAC_EVAL_P6 | ATAC_EVAL_CHECKPOINT | ATAC_EVAL_PRECISE))
+    if (flags & ~(ATAC_EVAL_DERIV | ATAC_EVAL_P6 | ATAC_EVAL_CHECKPOINT | ATAC_EVAL_PRECISE | ATAC_EVAL_ALL_ACTIVE))
         return fail(ATAC_ERR_INVALID, "atac_batch_eval: unknown flag");
     const int mode = mode_of(flags);
     const bool precise = (flags & ATAC_EVAL_PRECISE) != 0;
@@ -525,6 +531,7 @@ static int enqueue_eval(atac_batch b, uint32_t flags, const atac_outputs* out, h
         da.q = queues(b, true, 0);
         da.tiles = b->d_tiles; da.wave_seg = b->wave_seg; da.inst = b->d_inst; da.grids = b->d_grids;
         da.act_off = b->d_act; da.tile_active = b->d_tile_active;
+        da.all_active = (flags & ATAC_EVAL_ALL_ACTIVE) ? 1 : 0;
         hipEvent_t t0 = stage_begin(b, st);
         ATAC_HIP(launch_detect(da, b->tune.detect_block, b->grid_detect, b->lds_detect, st));
         stage_end(b, ST_DETECT, t0, st);
@@ -744,6 +751,102 @@ int atac_batch_timing(atac_batch b, int32_t enable, double* ms_sum, int32_t* lau
     for (auto& u : b->ev_used) { b->ev_pool.push_back(u.second.first); b->ev_pool.push_back(u.second.second); }
     b->ev_used.clear();
     b->timing = enable != 0;
+    return ATAC_OK;
+}
+
+// ---------------------------------------------------------------- host-buffer variants
+static int ensure_staging(atac_batch b) {
+    const size_t n = b->inst.size();
+    int rc = ATAC_OK;
+    if (!b->h_out) rc = dalloc(&b->h_out, n * (12 + 12 + 144 + 144));
+    if (!rc && !b->h_nact) rc = dalloc(&b->h_nact, n);
+    if (!rc && !b->h_gfm) rc = dalloc(&b->h_gfm, n * 12);
+    if (!rc && !b->h_grad) rc = dalloc(&b->h_grad, n * kVjpOut);
+    return rc;
+}
+
+int atac_batch_eval_host(atac_batch b, uint32_t flags, double* fm, double* Km, double* Dm, double* dfm_dp6,
+                         int32_t* n_active) {
+    if (!b) return fail(ATAC_ERR_INVALID, "atac_batch_eval_host: null batch");
+    if (b->inst.empty()) return ATAC_OK;
+    int rc = ensure_staging(b);
+    if (rc) return rc;
+    const size_t n = b->inst.size();
+    double* base = b->h_out;
+    atac_outputs o{};
+    o.fm = base;
+    o.dfm_dp6 = dfm_dp6 ? base + 12 * n : nullptr;
+    o.Km = Km ? base + 24 * n : nullptr;
+    o.Dm = Dm ? base + 168 * n : nullptr;
+    o.n_active = b->h_nact;
+    hipStream_t st = b->cap_stream;
+    rc = enqueue_eval(b, flags, &o, st);
+    if (rc) return rc;
+    if (fm) ATAC_HIP(hipMemcpyAsync(fm, o.fm, sizeof(double) * 12 * n, hipMemcpyDeviceToHost, st));
+    if (dfm_dp6) ATAC_HIP(hipMemcpyAsync(dfm_dp6, o.dfm_dp6, sizeof(double) * 12 * n, hipMemcpyDeviceToHost, st));
+    if (Km) ATAC_HIP(hipMemcpyAsync(Km, o.Km, sizeof(double) * 144 * n, hipMemcpyDeviceToHost, st));
+    if (Dm) ATAC_HIP(hipMemcpyAsync(Dm, o.Dm, sizeof(double) * 144 * n, hipMemcpyDeviceToHost, st));
+    if (n_active) ATAC_HIP(hipMemcpyAsync(n_active, o.n_active, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
+    ATAC_HIP(hipStreamSynchronize(st));
+    return ATAC_OK;
+}
+
+int atac_batch_vjp_host(atac_batch b, const double* g_fm, double* g_q1, double* g_q2, double* g_phi1, double* g_phi2,
+                        double* g_kn, double* g_kt) {
+    if (!b || !g_fm) return fail(ATAC_ERR_INVALID, "atac_batch_vjp_host: null argument");
+    if (b->inst.empty()) return ATAC_OK;
+    int rc = ensure_staging(b);
+    if (rc) return rc;
+    const size_t n = b->inst.size();
+    hipStream_t st = b->cap_stream;
+    ATAC_HIP(hipMemcpyAsync(b->h_gfm, g_fm, sizeof(double) * 12 * n, hipMemcpyHostToDevice, st));
+    double* G = b->h_grad;
+    atac_gradients g{G, G + 6 * n, G + 12 * n, G + 18 * n, G + 24 * n, G + 25 * n};
+    rc = enqueue_vjp(b, b->h_gfm, &g, st);
+    if (rc) return rc;
+    double* outs[6] = {g_q1, g_q2, g_phi1, g_phi2, g_kn, g_kt};
+    double* srcs[6] = {g.g_q1, g.g_q2, g.g_phi1, g.g_phi2, g.g_kn, g.g_kt};
+    const size_t w[6] = {6, 6, 6, 6, 1, 1};
+    for (int k = 0; k < 6; ++k)
+        if (outs[k]) ATAC_HIP(hipMemcpyAsync(outs[k], srcs[k], sizeof(double) * w[k] * n, hipMemcpyDeviceToHost, st));
+    ATAC_HIP(hipStreamSynchronize(st));
+    return ATAC_OK;
+}
+
+int atac_sdf_distance_host(atac_grid g, const double E2[16], double col_th, const double* xw, int64_t n, double* d) {
+    if (!g || !E2 || n < 0 || (n > 0 && (!xw || !d))) return fail(ATAC_ERR_INVALID, "atac_sdf_distance_host: bad arguments");
+    if (n == 0) return ATAC_OK;
+    double *dx = nullptr, *dd = nullptr;
+    int rc = dalloc(&dx, (size_t)n * 3);
+    if (!rc) rc = dalloc(&dd, (size_t)n);
+    hipError_t e = hipSuccess;
+    if (!rc) e = hipMemcpy(dx, xw, sizeof(double) * 3 * n, hipMemcpyHostToDevice);
+    if (!rc && e == hipSuccess) rc = atac_sdf_distance(g, E2, col_th, dx, n, dd, nullptr);
+    if (!rc && e == hipSuccess) e = hipMemcpy(d, dd, sizeof(double) * n, hipMemcpyDeviceToHost);
+    dfree(dx); dfree(dd);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(ATAC_ERR_HIP, hipGetErrorString(e));
+    return ATAC_OK;
+}
+
+int atac_sdf_collision_host(atac_grid g, const double E2[16], const double phi2[6], double col_th, uint32_t flags,
+                            const double* xw, const double* xw_dot, int64_t n, double* values, double* derivs) {
+    if (!g || n < 0 || (n > 0 && (!xw || !xw_dot || !values))) return fail(ATAC_ERR_INVALID, "atac_sdf_collision_host: bad arguments");
+    if (n == 0) return ATAC_OK;
+    double *dx = nullptr, *dv = nullptr, *dval = nullptr, *dder = nullptr;
+    int rc = dalloc(&dx, (size_t)n * 3);
+    if (!rc) rc = dalloc(&dv, (size_t)n * 3);
+    if (!rc) rc = dalloc(&dval, (size_t)n * 11);
+    if (!rc && derivs) rc = dalloc(&dder, (size_t)n * 135);
+    hipError_t e = hipSuccess;
+    if (!rc) e = hipMemcpy(dx, xw, sizeof(double) * 3 * n, hipMemcpyHostToDevice);
+    if (!rc && e == hipSuccess) e = hipMemcpy(dv, xw_dot, sizeof(double) * 3 * n, hipMemcpyHostToDevice);
+    if (!rc && e == hipSuccess) rc = atac_sdf_collision(g, E2, phi2, col_th, flags, dx, dv, n, dval, dder, nullptr);
+    if (!rc && e == hipSuccess) e = hipMemcpy(values, dval, sizeof(double) * 11 * n, hipMemcpyDeviceToHost);
+    if (!rc && e == hipSuccess && derivs) e = hipMemcpy(derivs, dder, sizeof(double) * 135 * n, hipMemcpyDeviceToHost);
+    dfree(dx); dfree(dv); dfree(dval); dfree(dder);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(ATAC_ERR_HIP, hipGetErrorString(e));
     return ATAC_OK;
 }
 

@@ -18,7 +18,7 @@ struct DetectArgs {
     QueueDev q;                 // tiles
     const TileDev* tiles;
     int32_t wave_seg;           // LDS queue entries per wavefront
-    int32_t pad;
+    int32_t all_active;         // every point is a contact (fixed contact lists)
     const InstDev* inst;
     const GridDev* grids;
     uint32_t* act_off;          // active offsets, tile slice starts at tile.slot

@@ -121,7 +121,8 @@ __global__ __launch_bounds__(BLOCK) void detect_kernel(DetectArgs a) {
             bool act = false;
             if (o < t.count) {
                 const int64_t p = t.begin + o;
-                act = detect_value(G, in, stream_load(in.px + p), stream_load(in.py + p), stream_load(in.pz + p)) < 0.0f;
+                act = a.all_active ||
+                      detect_value(G, in, stream_load(in.px + p), stream_load(in.py + p), stream_load(in.pz + p)) < 0.0f;
             }
             const uint64_t m = __ballot(act);
             if (act) q[wcount + __popcll(m & ((1ull << lane) - 1ull))] = (uint16_t)o;

@@ -243,6 +243,25 @@ ATAC_HD float detect_value(const GridDev& G, const InstDev& in, float px, float 
     return lerpf(lerpf(a, b, fy), lerpf(c, d, fy), fz) + (float)in.col_th;
 }
 
+// FP64 value, closed-form gradient and mixed partials of the trilinear cell, from
+// the same FP32 corners (the FP64 contact path).
+ATAC_HD void sdf_cell64(const GridDev& G, const Corners& c, const double f[3], double& val, double g[3], double H[6]) {
+    const double fx = f[0], fy = f[1], fz = f[2];
+    const double v000 = c.v000, v100 = c.v100, v010 = c.v010, v110 = c.v110;
+    const double v001 = c.v001, v101 = c.v101, v011 = c.v011, v111 = c.v111;
+    val = cell_value64(c, f);
+    const double x00 = v100 - v000, x10 = v110 - v010, x01 = v101 - v001, x11 = v111 - v011;
+    const double y00 = v010 - v000, y10 = v110 - v100, y01 = v011 - v001, y11 = v111 - v101;
+    const double z00 = v001 - v000, z10 = v101 - v100, z01 = v011 - v010, z11 = v111 - v110;
+    g[0] = lerpd(lerpd(x00, x10, fy), lerpd(x01, x11, fy), fz) * G.inv_d[0];
+    g[1] = lerpd(lerpd(y00, y10, fx), lerpd(y01, y11, fx), fz) * G.inv_d[1];
+    g[2] = lerpd(lerpd(z00, z10, fx), lerpd(z01, z11, fx), fy) * G.inv_d[2];
+    H[0] = H[1] = H[2] = 0.0;
+    H[3] = lerpd(x10 - x00, x11 - x01, fz) * (G.inv_d[0] * G.inv_d[1]);
+    H[4] = lerpd(x01 - x00, x11 - x10, fy) * (G.inv_d[0] * G.inv_d[2]);
+    H[5] = lerpd(y01 - y00, y11 - y10, fx) * (G.inv_d[1] * G.inv_d[2]);
+}
+
 // ------------------------------------------------------------------ contact math
 template <typename R>
 struct InstR {
@@ -294,7 +313,18 @@ ATAC_HD void contact_sample(const GridDev& G, const InstDev& in, double px, doub
     int idx[3];
     float f[3];
     double fd[3];
-    if (fast_cell(G, x, idx, f, fd)) {
+    if (sizeof(R) == sizeof(double) && fast_cell(G, x, idx, f, fd)) {
+        // FP64 contact path: value and partials in FP64 from the same corners (the
+        // FP32 checkpoint is not used; recomputing gives the forward values exactly)
+        Corners c = load_corners(G, idx[0], idx[1], idx[2]);
+        double val, g[3], H[6];
+        sdf_cell64(G, c, fd, val, g, H);
+        val += in.col_th;
+        s.d = (R)val;
+        s.g = mk3<R>((R)g[0], (R)g[1], (R)g[2]);
+        for (int k = 0; k < 6; ++k) s.H[k] = (R)H[k];
+        s.slow = false;
+    } else if (fast_cell(G, x, idx, f, fd)) {
         Corners c = load_corners(G, idx[0], idx[1], idx[2]);
         float val, g[3], H[6];
         sdf_cell32(G, c, f, val, g, H);

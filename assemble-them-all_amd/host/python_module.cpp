@@ -1,0 +1,206 @@
+// pybind11 module atac._redmax: the C++ SDF-contact classes (Body, BodySDFObj,
+// ForceGeneralSDFContact) for Python, as the reference's redmax_py exposes its
+// simulator (simulation/redmax/python_interface.cpp:34-303). Arrays are numpy FP64.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include "Body/Body.h"
+#include "Body/BodySDFObj.h"
+#include "Force/ForceGeneralSDFContact.h"
+
+namespace py = pybind11;
+using namespace redmax;
+
+namespace {
+
+using arr = py::array_t<double, py::array::c_style | py::array::forcecast>;
+
+std::vector<Vector3> points_of(const arr& a) {
+    if (a.ndim() != 2 || a.shape(1) != 3) throw std::invalid_argument("contact points must have shape (n, 3)");
+    std::vector<Vector3> v((size_t)a.shape(0));
+    auto r = a.unchecked<2>();
+    for (py::ssize_t i = 0; i < a.shape(0); ++i) v[(size_t)i] = make_vector3(r(i, 0), r(i, 1), r(i, 2));
+    return v;
+}
+
+Vector3 vec3(const arr& a) {
+    if (a.size() != 3) throw std::invalid_argument("expected 3 values");
+    const double* p = a.data();
+    return make_vector3(p[0], p[1], p[2]);
+}
+
+py::array_t<double> mat_out(const MatrixX& m) {
+    py::array_t<double> out({m.rows(), m.cols()});
+    auto w = out.mutable_unchecked<2>();
+    for (int r = 0; r < m.rows(); ++r)
+        for (int c = 0; c < m.cols(); ++c) w(r, c) = m(r, c);
+    return out;
+}
+
+py::array_t<double> vec_out(const VectorX& v) {
+    py::array_t<double> out(v.size());
+    auto w = out.mutable_unchecked<1>();
+    for (int i = 0; i < v.size(); ++i) w(i) = v(i);
+    return out;
+}
+
+template <int R, int C>
+py::array_t<double> fixed_out(const FixedMatrix<R, C>& m) {
+    py::array_t<double> out({R, C});
+    auto w = out.mutable_unchecked<2>();
+    for (int r = 0; r < R; ++r)
+        for (int c = 0; c < C; ++c) w(r, c) = m(r, c);
+    return out;
+}
+
+Matrix4 m4(const arr& a) {
+    if (a.size() != 16) throw std::invalid_argument("expected a 4x4 matrix");
+    Matrix4 m;
+    const double* p = a.data();
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) m(r, c) = p[4 * r + c];
+    return m;
+}
+
+Vector6 v6(const arr& a) {
+    if (a.size() != 6) throw std::invalid_argument("expected 6 values");
+    Vector6 v;
+    for (int k = 0; k < 6; ++k) v(k) = a.data()[k];
+    return v;
+}
+
+sdfgen::Array3<dtype> grid_of(const arr& values) {
+    if (values.ndim() != 3) throw std::invalid_argument("grid values must have shape (nk, nj, ni)");
+    sdfgen::Array3<dtype> g;
+    g.resize((int)values.shape(2), (int)values.shape(1), (int)values.shape(0));
+    std::copy(values.data(), values.data() + values.size(), g.a.begin());
+    return g;
+}
+
+}  // namespace
+
+PYBIND11_MODULE(_redmax, m) {
+    m.doc() = "GPU-backed SDF contact force with the reference simulator's C++ interface";
+
+    py::class_<Body>(m, "Body")
+        .def(py::init([](std::string name, const arr& pts, int index0) {
+                 return new Body(nullptr, name, points_of(pts), index0);
+             }),
+             py::arg("name"), py::arg("contact_points"), py::arg("index0") = 0)
+        .def_readwrite("name", &Body::_name)
+        .def_readwrite("contact_scale", &Body::_contact_scale)
+        .def_readwrite("p6_column", &Body::_p6_column)
+        .def_readonly("index", &Body::_index)
+        .def_property("E_0i", [](const Body& b) { return fixed_out(b._E_0i); },
+                      [](Body& b, const arr& a) { b._E_0i = m4(a); })
+        .def_property("phi", [](const Body& b) { return fixed_out(b._phi); },
+                      [](Body& b, const arr& a) { b._phi = v6(a); })
+        .def("get_contact_points", [](const Body& b) {
+            py::array_t<double> out({(py::ssize_t)b._contact_points.size(), (py::ssize_t)3});
+            auto w = out.mutable_unchecked<2>();
+            for (size_t i = 0; i < b._contact_points.size(); ++i)
+                for (int k = 0; k < 3; ++k) w(i, k) = b._contact_points[i](k);
+            return out;
+        })
+        .def("get_contact_bodies", &Body::get_contact_bodies)
+        .def("clear_contact_bodies", &Body::clear_contact_bodies);
+
+    py::class_<BodySDFObj, Body>(m, "BodySDFObj")
+        .def(py::init([](std::string name, const arr& values, const arr& min_box, const arr& max_box, const arr& dxyz,
+                         double col_th, const arr& pts, int index0) {
+                 const double* d = dxyz.data();
+                 return new BodySDFObj(nullptr, name, grid_of(values), vec3(min_box), vec3(max_box), d[0], d[1], d[2],
+                                       col_th, points_of(pts), index0);
+             }),
+             py::arg("name"), py::arg("values"), py::arg("min_box"), py::arg("max_box"), py::arg("dxyz"),
+             py::arg("col_th"), py::arg("contact_points"), py::arg("index0") = 0)
+        .def_static("from_sdf_file", [](std::string name, std::string path, double col_th, const arr& pts, int index0) {
+                        sdfgen::Array3<dtype> g;
+                        Vector3 mn, mx;
+                        dtype dx, dy, dz;
+                        if (!BodySDFObj::load_SDF_file(path, g, mn, mx, dx, dy, dz))
+                            throw std::runtime_error("cannot read " + path);
+                        return new BodySDFObj(nullptr, name, g, mn, mx, dx, dy, dz, col_th, points_of(pts), index0);
+                    },
+                    py::arg("name"), py::arg("path"), py::arg("col_th"), py::arg("contact_points"), py::arg("index0") = 0)
+        .def("save_sdf", &BodySDFObj::save_SDF_file)
+        .def_readwrite("col_th", &BodySDFObj::_col_th)
+        .def_property_readonly("min_box", [](const BodySDFObj& b) { return fixed_out(b._min_box); })
+        .def_property_readonly("max_box", [](const BodySDFObj& b) { return fixed_out(b._max_box); })
+        .def_property_readonly("dxyz", [](const BodySDFObj& b) { return std::vector<double>{b._dx, b._dy, b._dz}; })
+        .def_property_readonly("shape", [](const BodySDFObj& b) { return std::vector<int>{b._SDF.ni, b._SDF.nj, b._SDF.nk}; })
+        .def("values", [](const BodySDFObj& b) {
+            py::array_t<double> out({(py::ssize_t)b._SDF.nk, (py::ssize_t)b._SDF.nj, (py::ssize_t)b._SDF.ni});
+            std::copy(b._SDF.a.begin(), b._SDF.a.end(), out.mutable_data());
+            return out;
+        })
+        .def("distance", [](BodySDFObj& b, const arr& xw) { return b.distance(vec3(xw)); })
+        .def("collision", [](BodySDFObj& b, const arr& xw, const arr& xw_dot, bool derivs) {
+            dtype d, ddot;
+            Vector3 n, tdot, xi2;
+            py::dict out;
+            if (!derivs) {
+                b.collision(vec3(xw), vec3(xw_dot), d, n, ddot, tdot, xi2);
+            } else {
+                RowVector3 dd_dxw, dddot_dxw, dddot_dxwdot;
+                RowVector6 dd_dq2, dddot_dq2, dddot_dphi2;
+                Matrix3 dn_dxw, dtdot_dxw, dtdot_dxwdot, dxi2_dxw;
+                Matrix36 dn_dq2, dtdot_dq2, dtdot_dphi2, dxi2_dq2;
+                b.collision(vec3(xw), vec3(xw_dot), d, n, ddot, tdot, xi2, dd_dxw, dd_dq2, dn_dxw, dn_dq2, dddot_dxw,
+                            dddot_dxwdot, dddot_dq2, dddot_dphi2, dtdot_dxw, dtdot_dxwdot, dtdot_dq2, dtdot_dphi2,
+                            dxi2_dxw, dxi2_dq2);
+                out["dd_dxw"] = fixed_out(dd_dxw); out["dd_dq2"] = fixed_out(dd_dq2);
+                out["dn_dxw"] = fixed_out(dn_dxw); out["dn_dq2"] = fixed_out(dn_dq2);
+                out["dddot_dxw"] = fixed_out(dddot_dxw); out["dddot_dxwdot"] = fixed_out(dddot_dxwdot);
+                out["dddot_dq2"] = fixed_out(dddot_dq2); out["dddot_dphi2"] = fixed_out(dddot_dphi2);
+                out["dtdot_dxw"] = fixed_out(dtdot_dxw); out["dtdot_dxwdot"] = fixed_out(dtdot_dxwdot);
+                out["dtdot_dq2"] = fixed_out(dtdot_dq2); out["dtdot_dphi2"] = fixed_out(dtdot_dphi2);
+                out["dxi2_dxw"] = fixed_out(dxi2_dxw); out["dxi2_dq2"] = fixed_out(dxi2_dq2);
+            }
+            out["d"] = d; out["ddot"] = ddot;
+            out["n"] = fixed_out(n); out["tdot"] = fixed_out(tdot); out["xi2"] = fixed_out(xi2);
+            return out;
+        }, py::arg("xw"), py::arg("xw_dot"), py::arg("derivs") = false);
+
+    py::class_<ForceGeneralSDFContact>(m, "ForceGeneralSDFContact")
+        .def(py::init([](Body* a, Body* b, double kn, double kt, double mu, double damping) {
+                 return new ForceGeneralSDFContact(nullptr, a, b, kn, kt, mu, damping);
+             }),
+             py::arg("contact_body"), py::arg("SDF_body"), py::arg("kn") = 1.0, py::arg("kt") = 0.0,
+             py::arg("mu") = 0.0, py::arg("damping") = 0.0, py::keep_alive<1, 2>(), py::keep_alive<1, 3>())
+        .def("set_stiffness", &ForceGeneralSDFContact::set_stiffness)
+        .def("set_friction", &ForceGeneralSDFContact::set_friction)
+        .def("set_damping", &ForceGeneralSDFContact::set_damping)
+        .def("set_scale", &ForceGeneralSDFContact::set_scale)
+        .def_readonly("kn", &ForceGeneralSDFContact::_kn)
+        .def_readonly("kt", &ForceGeneralSDFContact::_kt)
+        .def_readonly("mu", &ForceGeneralSDFContact::_mu)
+        .def_readonly("damping", &ForceGeneralSDFContact::_damping)
+        .def_readonly("scale", &ForceGeneralSDFContact::_scale)
+        .def("computeForce", [](ForceGeneralSDFContact& f, int ndof_m) {
+            VectorX fm = VectorX::Zero(ndof_m), fr;
+            f.computeForce(fm, fr);
+            return vec_out(fm);
+        }, py::arg("ndof_m"))
+        .def("computeForceWithDerivative", [](ForceGeneralSDFContact& f, int ndof_m) {
+            VectorX fm = VectorX::Zero(ndof_m), fr;
+            MatrixX Km = MatrixX::Zero(ndof_m, ndof_m), Dm = MatrixX::Zero(ndof_m, ndof_m), Kr, Dr;
+            f.computeForceWithDerivative(fm, fr, Km, Dm, Kr, Dr);
+            return py::make_tuple(vec_out(fm), mat_out(Km), mat_out(Dm));
+        }, py::arg("ndof_m"))
+        .def("computeForceWithDesignDerivative", [](ForceGeneralSDFContact& f, int ndof_m, int ndof_p) {
+            VectorX fm = VectorX::Zero(ndof_m), fr;
+            MatrixX Km = MatrixX::Zero(ndof_m, ndof_m), Dm = MatrixX::Zero(ndof_m, ndof_m), Kr, Dr;
+            MatrixX dfm_dp = MatrixX::Zero(ndof_m, ndof_p), dfr_dp;
+            f.computeForceWithDerivative(fm, fr, Km, Dm, Kr, Dr, dfm_dp, dfr_dp);
+            return py::make_tuple(vec_out(fm), mat_out(Km), mat_out(Dm), mat_out(dfm_dp));
+        }, py::arg("ndof_m"), py::arg("ndof_p"))
+        .def("test_derivatives", &ForceGeneralSDFContact::test_derivatives)
+        .def("test_derivatives_runtime", &ForceGeneralSDFContact::test_derivatives_runtime_error)
+        .def("test_design_derivatives_runtime", &ForceGeneralSDFContact::test_design_derivatives_runtime)
+        .def("last_active_count", &ForceGeneralSDFContact::last_active_count)
+        .def("time_report_us", [](ForceGeneralSDFContact& f) {
+            return py::make_tuple(f._time.get("collision detection"), f._time.get("compute force"));
+        });
+}

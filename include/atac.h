@@ -65,6 +65,8 @@ typedef struct {
 #define ATAC_EVAL_P6 2u          /* + contact-scale column of dfm_dp */
 #define ATAC_EVAL_CHECKPOINT 4u  /* keep (d, grad phi) of active points for atac_batch_vjp */
 #define ATAC_EVAL_PRECISE 8u     /* FP64 contact arithmetic (detection stays FP32) */
+#define ATAC_EVAL_ALL_ACTIVE 16u /* every point is a contact, as for the reference's contact-list
+                                    overloads (ForceGeneralSDFContact.cpp:147, :221, :449) */
 
 /* Outputs of one evaluation, device pointers, any may be NULL. Per instance:
  * fm 12 = [tau1 f1 tau2 f2] (each body's frame, angular first), Km/Dm 144 row-major
@@ -174,6 +176,18 @@ int atac_sdf_distance(atac_grid g, const double E2[16], double col_th, const dou
 int atac_sdf_collision(atac_grid g, const double E2[16], const double phi2[6], double col_th, uint32_t flags,
                        const double* xw, const double* xw_dot, int64_t n, double* values, double* derivs,
                        void* stream);
+
+/* Synchronous host-buffer variants for callers that hold no device memory (the C++
+ * Force/Body classes, ctypes without torch): they run on the batch's own stream (or
+ * the default stream) with library-owned device staging and return when the host
+ * buffers are filled. Same meaning and layout as the device-pointer entry points. */
+int atac_batch_eval_host(atac_batch b, uint32_t flags, double* fm, double* Km, double* Dm, double* dfm_dp6,
+                         int32_t* n_active);
+int atac_batch_vjp_host(atac_batch b, const double* g_fm, double* g_q1, double* g_q2, double* g_phi1, double* g_phi2,
+                        double* g_kn, double* g_kt);
+int atac_sdf_distance_host(atac_grid g, const double E2[16], double col_th, const double* xw, int64_t n, double* d);
+int atac_sdf_collision_host(atac_grid g, const double E2[16], const double phi2[6], double col_th, uint32_t flags,
+                            const double* xw, const double* xw_dot, int64_t n, double* values, double* derivs);
 
 #ifdef __cplusplus
 }
