@@ -51,7 +51,7 @@ class Instance(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("tile_points", C.c_int32), ("detect_block", C.c_int32), ("blocks_per_cu", C.c_int32),
-                ("order", C.c_int32)]
+                ("order", C.c_int32), ("points_per_lane", C.c_int32)]
 
 
 class Outputs(C.Structure):

@@ -160,6 +160,7 @@ class ContactBatch:
         t.detect_block = int(tuning.get("detect_block", 0))
         t.blocks_per_cu = int(tuning.get("blocks_per_cu", 0))
         t.order = int(tuning.get("order", 0))
+        t.points_per_lane = int(tuning.get("points_per_lane", 0))
         return t
 
     def close(self):
@@ -181,7 +182,7 @@ class ContactBatch:
         check(lib().atac_batch_info(self.handle, C.byref(n), C.byref(nt), C.byref(pb), C.byref(tu)))
         return {"n_points": int(n.value), "n_tiles": int(nt.value), "persistent_blocks": int(pb.value),
                 "tile_points": tu.tile_points, "detect_block": tu.detect_block, "blocks_per_cu": tu.blocks_per_cu,
-                "order": tu.order}
+                "order": tu.order, "points_per_lane": tu.points_per_lane}
 
     def retune(self, tuning):
         tu = self._tuning(tuning)
@@ -268,13 +269,13 @@ class ContactBatch:
         check(lib().atac_batch_active_ids(self.handle, i, ids.ctypes.data_as(C.c_void_p), len(ids), C.byref(cnt)))
         return ids
 
-    STAGES = ("detect", "contact", "reduce", "vjp", "vjp_reduce")
+    STAGES = ("detect", "slow", "contact", "reduce", "vjp", "vjp_reduce")
 
     def timing(self, enable=True):
         """Read (and clear) per-stage kernel times recorded with HIP events, then
         switch recording on or off. Returns {stage: (total_ms, launches)}."""
-        ms = np.zeros(5)
-        n = np.zeros(5, dtype=np.int32)
+        ms = np.zeros(len(self.STAGES))
+        n = np.zeros(len(self.STAGES), dtype=np.int32)
         check(lib().atac_batch_timing(self.handle, 1 if enable else 0, ms.ctypes.data_as(C.c_void_p),
                                       n.ctypes.data_as(C.c_void_p)))
         return {s: (float(ms[k]), int(n[k])) for k, s in enumerate(self.STAGES)}

@@ -51,6 +51,8 @@ void dfree(T*& p) {
 struct atac_grid_s {
     GridDev dev;
     float* d_values = nullptr;
+    float4* d_cells = nullptr;
+    float2* d_blk = nullptr;
     int64_t count = 0;
 };
 
@@ -60,6 +62,8 @@ struct atac_points_s {
     float* d_z = nullptr;
     int64_t n = 0;
     std::vector<int64_t> seg;   // n_segments + 1 offsets
+    std::vector<int64_t> dseg;  // device offset of each segment (multiples of 4: 16-byte aligned)
+    int64_t dn = 0;             // device floats per coordinate (segments padded to multiples of 4)
     std::vector<int64_t> perm;  // sorted position -> original index within the segment
 };
 
@@ -78,20 +82,26 @@ struct atac_batch_s {
     TileDev* d_tiles = nullptr;
     int32_t* d_order = nullptr;       // tiles, queue-partitioned
     int32_t* d_tq_begin = nullptr;    // nq + 1
-    int32_t* d_iorder = nullptr;      // instances, queue-partitioned
-    int32_t* d_iq_begin = nullptr;    // nq + 1
     int32_t nq = 1;
     int32_t n_tiles = 0;
     int64_t n_slots = 0;
-    uint32_t* d_act = nullptr;
-    CkptDev* d_ck = nullptr;
+    int64_t chunk_cap = 0;            // work items if every point were active
     int32_t* d_tile_active = nullptr;
-    int32_t* d_counters = nullptr;  // 3 x nq queue heads: detect, contact, vjp
-    int grid_detect = 0, grid_contact = 0, grid_vjp = 0;
+    int32_t* d_tile_chunk0 = nullptr;
+    ChunkDev* d_chunks = nullptr;
+    int32_t* d_counters = nullptr;    // nq detection queue heads, then CNT_WORDS
+    // per-evaluation buffers, allocated on first use
+    Rec32* d_rec32 = nullptr;         // n_slots
+    Rec64* d_rec64 = nullptr;         // n_slots
+    uint32_t* d_slow_slot = nullptr;  // n_slots
+    float* d_slow_diag = nullptr;     // 4 x n_slots
+    double* d_part = nullptr;         // chunk_cap x part_width
+    int64_t part_width = 0;
+    int grid_detect = 0, grid_contact[2][3] = {{0, 0, 0}, {0, 0, 0}}, grid_vjp[2] = {0, 0}, grid_slow = 0;
     size_t lds_detect = 0;
-    int wave_seg = 0;
-    bool ckpt_valid = false;
-    bool ckpt_precise = false;
+    int wave_span = 0;
+    bool rec_valid = false;           // records of the last evaluation are in place
+    bool rec_precise = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     hipStream_t cap_stream = nullptr;
@@ -107,7 +117,7 @@ struct atac_batch_s {
 };
 
 namespace {
-enum { ST_DETECT = 0, ST_CONTACT = 1, ST_REDUCE = 2, ST_VJP = 3, ST_VJP_REDUCE = 4, ST_COUNT = 5 };
+enum { ST_DETECT = 0, ST_SLOW = 1, ST_CONTACT = 2, ST_REDUCE = 3, ST_VJP = 4, ST_VJP_REDUCE = 5, ST_COUNT = 6 };
 
 hipEvent_t take_event(atac_batch b) {
     if (!b->ev_pool.empty()) {
@@ -172,12 +182,29 @@ int atac_grid_create(const float* values, int32_t ni, int32_t nj, int32_t nk, co
     if (e != hipSuccess) { dfree(g->d_values); delete g; return fail(ATAC_ERR_HIP, hipGetErrorString(e)); }
     GridDev& G = g->dev;
     G.v = g->d_values;
-    G.n[0] = ni; G.n[1] = nj; G.n[2] = nk; G.pad = 0;
+    G.n[0] = ni; G.n[1] = nj; G.n[2] = nk;
+    G.fast = 0; G.cells = nullptr; G.blk = nullptr; G.pad = 0;
     for (int a = 0; a < 3; ++a) {
         G.minb[a] = min_box[a];
         G.maxb[a] = max_box[a];
         G.d[a] = dxyz[a];
         G.inv_d[a] = 1.0 / dxyz[a];
+        G.nb[a] = (G.n[a] - 1) / kBlk + 1;
+    }
+    if (ni >= 2 && nj >= 2 && nk >= 2) {
+        const int64_t nc = (int64_t)(ni - 1) * (nj - 1) * (nk - 1);
+        const int64_t nb = (int64_t)G.nb[0] * G.nb[1] * G.nb[2];
+        rc = dalloc(&g->d_cells, (size_t)(2 * nc));
+        if (!rc) rc = dalloc(&g->d_blk, (size_t)nb);
+        if (!rc) {
+            e = launch_grid_setup(G, g->d_cells, g->d_blk, nullptr);
+            if (e == hipSuccess) e = hipDeviceSynchronize();
+            if (e != hipSuccess) rc = fail(ATAC_ERR_HIP, hipGetErrorString(e));
+        }
+        if (rc) { dfree(g->d_values); dfree(g->d_cells); dfree(g->d_blk); delete g; return rc; }
+        G.cells = g->d_cells;
+        G.blk = g->d_blk;
+        G.fast = 1;
     }
     *out = g;
     return ATAC_OK;
@@ -185,7 +212,7 @@ int atac_grid_create(const float* values, int32_t ni, int32_t nj, int32_t nk, co
 
 int atac_grid_destroy(atac_grid g) {
     if (!g) return ATAC_OK;
-    dfree(g->d_values);
+    dfree(g->d_values); dfree(g->d_cells); dfree(g->d_blk);
     delete g;
     return ATAC_OK;
 }
@@ -218,10 +245,18 @@ int atac_points_create(const double* xyz, int64_t n, const int64_t* seg_offsets,
     P->n = n;
     P->seg = seg;
     P->perm.resize(n);
-    std::vector<float> hx(n), hy(n), hz(n);
+    P->dseg.resize(seg.size());
+    int64_t dn = 0;
+    for (size_t s = 0; s + 1 < seg.size(); ++s) {
+        P->dseg[s] = dn;
+        dn += (seg[s + 1] - seg[s] + 3) / 4 * 4;
+    }
+    P->dseg.back() = dn;
+    P->dn = dn;
+    std::vector<float> hx(dn, 0.0f), hy(dn, 0.0f), hz(dn, 0.0f);
     std::vector<std::pair<uint64_t, int64_t>> keys;
     for (size_t s = 0; s + 1 < seg.size(); ++s) {
-        const int64_t b = seg[s], e = seg[s + 1], m = e - b;
+        const int64_t b = seg[s], e = seg[s + 1], m = e - b, db = P->dseg[s];
         if (m == 0) continue;
         double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
         for (int64_t i = b; i < e; ++i)
@@ -244,19 +279,19 @@ int atac_points_create(const double* xyz, int64_t n, const int64_t* seg_offsets,
         std::sort(keys.begin(), keys.end());
         for (int64_t i = 0; i < m; ++i) {
             const int64_t src = b + keys[i].second;
-            hx[b + i] = (float)xyz[3 * src];
-            hy[b + i] = (float)xyz[3 * src + 1];
-            hz[b + i] = (float)xyz[3 * src + 2];
+            hx[db + i] = (float)xyz[3 * src];
+            hy[db + i] = (float)xyz[3 * src + 1];
+            hz[db + i] = (float)xyz[3 * src + 2];
             P->perm[b + i] = keys[i].second;
         }
     }
-    int rc = dalloc(&P->d_x, n);
-    if (!rc) rc = dalloc(&P->d_y, n);
-    if (!rc) rc = dalloc(&P->d_z, n);
-    if (!rc && n > 0) {
-        hipError_t e = hipMemcpy(P->d_x, hx.data(), sizeof(float) * n, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(P->d_y, hy.data(), sizeof(float) * n, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(P->d_z, hz.data(), sizeof(float) * n, hipMemcpyHostToDevice);
+    int rc = dalloc(&P->d_x, dn);
+    if (!rc) rc = dalloc(&P->d_y, dn);
+    if (!rc) rc = dalloc(&P->d_z, dn);
+    if (!rc && dn > 0) {
+        hipError_t e = hipMemcpy(P->d_x, hx.data(), sizeof(float) * dn, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(P->d_y, hy.data(), sizeof(float) * dn, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(P->d_z, hz.data(), sizeof(float) * dn, hipMemcpyHostToDevice);
         if (e != hipSuccess) rc = fail(ATAC_ERR_HIP, hipGetErrorString(e));
     }
     if (rc) {
@@ -287,8 +322,9 @@ int atac_points_permutation(atac_points p, int32_t segment, int64_t* sorted_to_o
 // ---------------------------------------------------------------- batches
 static void compose_for(atac_batch b, const atac_instance& s, InstDev& d) {
     const atac_points_s& P = *b->pts[s.points];
-    const int64_t beg = P.seg[s.segment];
-    compose_instance(s, b->grids[s.grid]->dev, P.d_x + beg, P.d_y + beg, P.d_z + beg, P.seg[s.segment + 1] - beg, d);
+    const int64_t beg = P.dseg[s.segment];
+    compose_instance(s, b->grids[s.grid]->dev, P.d_x + beg, P.d_y + beg, P.d_z + beg,
+                     P.seg[s.segment + 1] - P.seg[s.segment], d);
 }
 
 static int check_instances(atac_batch b, const atac_instance* inst, int32_t n) {
@@ -306,35 +342,78 @@ static int check_instances(atac_batch b, const atac_instance* inst, int32_t n) {
 constexpr int kXcds = 8;   // MI355X: 8 XCDs, each with its own L2
 
 static void release_plan(atac_batch b) {
-    dfree(b->d_tiles); dfree(b->d_order); dfree(b->d_act); dfree(b->d_ck); dfree(b->d_tile_active);
-    dfree(b->d_tq_begin); dfree(b->d_iorder); dfree(b->d_iq_begin); dfree(b->d_counters);
-    b->ckpt_valid = false;
+    dfree(b->d_tiles); dfree(b->d_order); dfree(b->d_tq_begin); dfree(b->d_counters);
+    dfree(b->d_tile_active); dfree(b->d_tile_chunk0); dfree(b->d_chunks);
+    dfree(b->d_rec32); dfree(b->d_rec64); dfree(b->d_slow_slot); dfree(b->d_slow_diag); dfree(b->d_part);
+    b->part_width = 0;
+    b->rec_valid = false;
     if (b->graph_exec) { (void)hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
     if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+}
+
+// Width of a work item's partial row for an evaluation mode (0 wrench, 1 + p6,
+// 2 + Km/Dm) and for the adjoint (mode 3).
+static int64_t part_width_of(int mode) {
+    return mode == 3 ? kVjpOut : (mode == 2 ? 24 + 288 : (mode == 1 ? 24 : 12));
+}
+
+// Allocates what an evaluation of this mode and precision (or the adjoint,
+// mode 3) needs; called before any launch or graph capture.
+static int ensure_buffers(atac_batch b, int mode, bool precise) {
+    int rc = ATAC_OK;
+    if (b->n_slots > 0) {
+        if (precise && !b->d_rec64) rc = dalloc(&b->d_rec64, (size_t)b->n_slots);
+        if (!rc && !precise && !b->d_rec32) rc = dalloc(&b->d_rec32, (size_t)b->n_slots);
+        if (!rc && !b->d_slow_slot) rc = dalloc(&b->d_slow_slot, (size_t)b->n_slots);
+        if (!rc && !precise && !b->d_slow_diag) rc = dalloc(&b->d_slow_diag, (size_t)b->n_slots * 4);
+    }
+    const int64_t w = part_width_of(mode);
+    if (!rc && b->chunk_cap > 0 && w > b->part_width) {
+        // a captured graph holds the old buffer: it is dropped with it
+        if (b->graph_exec) { (void)hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
+        if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+        (void)hipDeviceSynchronize();
+        dfree(b->d_part);
+        rc = dalloc(&b->d_part, (size_t)(b->chunk_cap * w));
+        b->part_width = rc ? 0 : w;
+    }
+    return rc;
+}
+
+// LDS of the detection kernel: one u16 offset segment per wavefront.
+static int wave_span_of(int tile_points, int block, int ppl) {
+    const int nw = block / 64, step = 64 * ppl;
+    return ((tile_points + nw - 1) / nw + step - 1) / step * step;
 }
 
 static int plan(atac_batch b, const atac_tuning* t) {
     atac_tuning tu = t ? *t : atac_tuning{};
     if (tu.tile_points <= 0) tu.tile_points = 4096;
     if (tu.detect_block <= 0) tu.detect_block = 256;
-    if (tu.tile_points > 65536) return fail(ATAC_ERR_INVALID, "tile_points must be <= 65536");
+    if (tu.points_per_lane <= 0) tu.points_per_lane = 4;
+    if (tu.tile_points > 65536 || tu.tile_points % 64 != 0)
+        return fail(ATAC_ERR_INVALID, "tile_points must be a multiple of 64 and <= 65536");
     if (tu.detect_block != 128 && tu.detect_block != 256 && tu.detect_block != 512 && tu.detect_block != 1024)
         return fail(ATAC_ERR_INVALID, "detect_block must be 128, 256, 512 or 1024");
+    if (tu.points_per_lane != 1 && tu.points_per_lane != 2 && tu.points_per_lane != 4)
+        return fail(ATAC_ERR_INVALID, "points_per_lane must be 1, 2 or 4");
     if (tu.blocks_per_cu < 0 || tu.order < 0 || tu.order > 2) return fail(ATAC_ERR_INVALID, "bad tuning");
-    {
-        const int B = tu.detect_block;
-        const size_t lds = (size_t)(((tu.tile_points + B - 1) / B) * 64) * (B / 64) * sizeof(uint16_t);
-        if (lds + 1024 > b->lds_limit)
-            return fail(ATAC_ERR_INVALID, "tile_points " + std::to_string(tu.tile_points) + " with detect_block " +
-                                              std::to_string(B) + " needs " + std::to_string(lds) +
-                                              " B of LDS; the device allows " + std::to_string(b->lds_limit) +
-                                              " B per workgroup");
-    }
+    const int B = tu.detect_block;
+    const int span = wave_span_of(tu.tile_points, B, tu.points_per_lane);
+    const size_t lds = (size_t)span * (B / 64) * sizeof(uint16_t);
+    if (lds + 1024 > b->lds_limit)
+        return fail(ATAC_ERR_INVALID, "tile_points " + std::to_string(tu.tile_points) + " with detect_block " +
+                                          std::to_string(B) + " needs " + std::to_string(lds) +
+                                          " B of LDS; the device allows " + std::to_string(b->lds_limit) +
+                                          " B per workgroup");
+    int64_t total_points = 0;
+    for (const InstDev& d : b->inst_dev) total_points += d.pt_count;
+    if (total_points >= ((int64_t)1 << 31)) return fail(ATAC_ERR_INVALID, "a batch holds fewer than 2^31 points");
     release_plan(b);
     b->tune = tu;
     const int T = tu.tile_points;
     b->tiles.clear();
-    int64_t slot = 0;
+    int64_t slot = 0, chunks = 0;
     for (size_t i = 0; i < b->inst_dev.size(); ++i) {
         InstDev& d = b->inst_dev[i];
         d.tile_begin = (int32_t)b->tiles.size();
@@ -345,51 +424,43 @@ static int plan(atac_batch b, const atac_tuning* t) {
             td.begin = o;
             td.slot = slot + o;
             b->tiles.push_back(td);
+            chunks += (td.count + kChunk - 1) / kChunk;
         }
         d.tile_count = (int32_t)b->tiles.size() - d.tile_begin;
         slot += d.pt_count;
     }
     b->n_slots = slot;
+    b->chunk_cap = chunks;
     b->n_tiles = (int32_t)b->tiles.size();
-    // work queues: order 0 = grouped by grid and cut into one queue per XCD with
+    // tile queues: order 0 = grouped by grid and cut into one queue per XCD with
     // balanced point counts; 1 = one queue in instance order; 2 = one queue grouped by grid
     b->nq = tu.order == 0 ? kXcds : 1;
-    std::vector<int32_t> order(b->n_tiles), iorder(b->inst_dev.size());
+    std::vector<int32_t> order(b->n_tiles);
     std::iota(order.begin(), order.end(), 0);
-    std::iota(iorder.begin(), iorder.end(), 0);
     if (tu.order != 1) {
-        auto grid_of_tile = [&](int32_t t) { return b->inst_dev[b->tiles[t].inst].grid; };
+        auto grid_of_tile = [&](int32_t x) { return b->inst_dev[b->tiles[x].inst].grid; };
         std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return grid_of_tile(x) < grid_of_tile(y); });
-        std::stable_sort(iorder.begin(), iorder.end(), [&](int32_t x, int32_t y) { return b->inst_dev[x].grid < b->inst_dev[y].grid; });
     }
-    auto cut = [&](const std::vector<int32_t>& items, auto weight) {
-        std::vector<int32_t> begin(b->nq + 1, (int32_t)items.size());
-        begin[0] = 0;
-        double total = 0.0;
-        for (int32_t it : items) total += weight(it);
-        double acc = 0.0;
+    std::vector<int32_t> tq(b->nq + 1, b->n_tiles);
+    {
+        tq[0] = 0;
+        double total = 0.0, acc = 0.0;
+        for (int32_t it : order) total += (double)b->tiles[it].count;
         int q = 1;
-        for (size_t k = 0; k < items.size() && q < b->nq; ++k) {
-            acc += weight(items[k]);
-            while (q < b->nq && acc >= total * q / b->nq) begin[q++] = (int32_t)(k + 1);
+        for (size_t k = 0; k < order.size() && q < b->nq; ++k) {
+            acc += (double)b->tiles[order[k]].count;
+            while (q < b->nq && acc >= total * q / b->nq) tq[q++] = (int32_t)(k + 1);
         }
-        return begin;
-    };
-    std::vector<int32_t> tq = cut(order, [&](int32_t t) { return (double)b->tiles[t].count; });
-    std::vector<int32_t> iq = cut(iorder, [&](int32_t i) { return (double)b->inst_dev[i].pt_count + 1.0; });
+    }
     int rc = dalloc(&b->d_tiles, b->n_tiles);
     if (!rc) rc = dalloc(&b->d_order, b->n_tiles);
     if (!rc) rc = dalloc(&b->d_tq_begin, b->nq + 1);
-    if (!rc) rc = dalloc(&b->d_iorder, b->inst_dev.size());
-    if (!rc) rc = dalloc(&b->d_iq_begin, b->nq + 1);
-    if (!rc) rc = dalloc(&b->d_counters, 3 * b->nq);
+    if (!rc) rc = dalloc(&b->d_counters, b->nq + CNT_WORDS);
     if (!rc) rc = dalloc(&b->d_tile_active, b->n_tiles);
-    if (!rc) rc = dalloc(&b->d_act, (size_t)b->n_slots);
+    if (!rc) rc = dalloc(&b->d_tile_chunk0, b->n_tiles);
+    if (!rc) rc = dalloc(&b->d_chunks, (size_t)b->chunk_cap);
     if (rc) return rc;
     ATAC_HIP(hipMemcpy(b->d_tq_begin, tq.data(), sizeof(int32_t) * tq.size(), hipMemcpyHostToDevice));
-    ATAC_HIP(hipMemcpy(b->d_iq_begin, iq.data(), sizeof(int32_t) * iq.size(), hipMemcpyHostToDevice));
-    if (!iorder.empty())
-        ATAC_HIP(hipMemcpy(b->d_iorder, iorder.data(), sizeof(int32_t) * iorder.size(), hipMemcpyHostToDevice));
     if (b->n_tiles > 0) {
         ATAC_HIP(hipMemcpy(b->d_tiles, b->tiles.data(), sizeof(TileDev) * b->n_tiles, hipMemcpyHostToDevice));
         ATAC_HIP(hipMemcpy(b->d_order, order.data(), sizeof(int32_t) * b->n_tiles, hipMemcpyHostToDevice));
@@ -397,22 +468,26 @@ static int plan(atac_batch b, const atac_tuning* t) {
     // inst_dev carries the tile ranges: re-upload
     if (!b->inst_dev.empty())
         ATAC_HIP(hipMemcpy(b->d_inst, b->inst_dev.data(), sizeof(InstDev) * b->inst_dev.size(), hipMemcpyHostToDevice));
-    // persistent launch shapes
-    const int B = tu.detect_block;
-    b->wave_seg = ((T + B - 1) / B) * 64;
-    b->lds_detect = (size_t)b->wave_seg * (B / 64) * sizeof(uint16_t);
+    // persistent launch shapes (the FP32 and FP64 detection kernels share LDS and block size)
+    b->wave_span = span;
+    b->lds_detect = lds;
     int nb = 0;
-    ATAC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, detect_kernel_ptr(B), B, b->lds_detect));
+    ATAC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, detect_kernel_ptr(B, tu.points_per_lane, false), B, lds));
     if (tu.blocks_per_cu > 0) nb = std::min(nb, tu.blocks_per_cu);
     nb = std::max(nb, 1);
     b->grid_detect = std::max(1, std::min(b->n_tiles, nb * b->n_cu));
-    const int n_inst = (int)b->inst_dev.size();
-    int nc = 0;
-    ATAC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nc, contact_kernel_ptr(1, false), 256, 0));
-    b->grid_contact = std::max(1, std::min(std::max(n_inst, 1), std::max(nc, 1) * b->n_cu));
-    int nv = 0;
-    ATAC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nv, vjp_kernel_ptr(false), 256, 0));
-    b->grid_vjp = std::max(1, std::min(std::max(n_inst, 1), std::max(nv, 1) * b->n_cu));
+    const int64_t items = std::max<int64_t>(1, (b->chunk_cap + 3) / 4);
+    for (int pr = 0; pr < 2; ++pr) {
+        for (int m = 0; m < 3; ++m) {
+            int nc = 0;
+            ATAC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nc, contact_kernel_ptr(m, pr == 1), 256, 0));
+            b->grid_contact[pr][m] = (int)std::max<int64_t>(1, std::min<int64_t>(items, (int64_t)std::max(nc, 1) * b->n_cu));
+        }
+        int nv = 0;
+        ATAC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nv, vjp_kernel_ptr(pr == 1), 256, 0));
+        b->grid_vjp[pr] = (int)std::max<int64_t>(1, std::min<int64_t>(items, (int64_t)std::max(nv, 1) * b->n_cu));
+    }
+    b->grid_slow = std::max(1, b->n_cu);
     return ATAC_OK;
 }
 
@@ -504,14 +579,16 @@ int atac_batch_info(atac_batch b, int64_t* n_points, int32_t* n_tiles, int32_t* 
 
 static int mode_of(uint32_t flags) { return (flags & ATAC_EVAL_DERIV) ? 2 : ((flags & ATAC_EVAL_P6) ? 1 : 0); }
 
-static QueueDev queues(atac_batch b, bool tiles, int which) {
+static QueueDev tile_queues(atac_batch b) {
     QueueDev q{};
-    q.order = tiles ? b->d_order : b->d_iorder;
-    q.begin = tiles ? b->d_tq_begin : b->d_iq_begin;
-    q.counter = b->d_counters + which * b->nq;
+    q.order = b->d_order;
+    q.begin = b->d_tq_begin;
+    q.counter = b->d_counters;
     q.nq = b->nq;
     return q;
 }
+
+static int32_t* counters(atac_batch b) { return b->d_counters + b->nq; }
 
 static int enqueue_eval(atac_batch b, uint32_t flags, const atac_outputs* out, hipStream_t st) {
     if (flags & ~(ATAC_EVAL_DERIV | ATAC_EVAL_P6 | ATAC_EVAL_CHECKPOINT | ATAC_EVAL_PRECISE | ATAC_EVAL_ALL_ACTIVE))
@@ -520,54 +597,81 @@ static int enqueue_eval(atac_batch b, uint32_t flags, const atac_outputs* out, h
     const bool precise = (flags & ATAC_EVAL_PRECISE) != 0;
     if (out && out->dfm_dp6 && mode < 1) return fail(ATAC_ERR_INVALID, "dfm_dp6 requested without ATAC_EVAL_P6 or ATAC_EVAL_DERIV");
     if (out && (out->Km || out->Dm) && mode < 2) return fail(ATAC_ERR_INVALID, "Km/Dm requested without ATAC_EVAL_DERIV");
-    if ((flags & ATAC_EVAL_CHECKPOINT) && !b->d_ck && b->n_slots > 0) {
-        int rc = dalloc(&b->d_ck, (size_t)b->n_slots);
-        if (rc) return rc;
-    }
+    int rc = ensure_buffers(b, mode, precise);
+    if (rc) return rc;
     if (b->inst.empty()) return ATAC_OK;
-    ATAC_HIP(launch_reset(b->d_counters, 2 * b->nq, st));
+    void* rec = precise ? (void*)b->d_rec64 : (void*)b->d_rec32;
+    ATAC_HIP(launch_reset(b->d_counters, b->nq + CNT_WORDS, st));
     if (b->n_tiles > 0) {
         DetectArgs da{};
-        da.q = queues(b, true, 0);
-        da.tiles = b->d_tiles; da.wave_seg = b->wave_seg; da.inst = b->d_inst; da.grids = b->d_grids;
-        da.act_off = b->d_act; da.tile_active = b->d_tile_active;
+        da.q = tile_queues(b);
+        da.tiles = b->d_tiles; da.wave_span = b->wave_span; da.inst = b->d_inst; da.grids = b->d_grids;
         da.all_active = (flags & ATAC_EVAL_ALL_ACTIVE) ? 1 : 0;
+        da.rec = rec; da.tile_active = b->d_tile_active; da.tile_chunk0 = b->d_tile_chunk0; da.chunks = b->d_chunks;
+        da.slow_slot = b->d_slow_slot; da.counters = counters(b);
         hipEvent_t t0 = stage_begin(b, st);
-        ATAC_HIP(launch_detect(da, b->tune.detect_block, b->grid_detect, b->lds_detect, st));
+        ATAC_HIP(launch_detect(da, b->tune.detect_block, b->tune.points_per_lane, precise, b->grid_detect, b->lds_detect, st));
         stage_end(b, ST_DETECT, t0, st);
+        SlowArgs sa{};
+        sa.inst = b->d_inst; sa.grids = b->d_grids; sa.rec = rec; sa.slow_diag = b->d_slow_diag;
+        sa.slow_slot = b->d_slow_slot; sa.counters = counters(b);
+        hipEvent_t t1 = stage_begin(b, st);
+        ATAC_HIP(launch_slow(sa, precise, b->grid_slow, st));
+        stage_end(b, ST_SLOW, t1, st);
+        ContactArgs ca{};
+        ca.chunks = b->d_chunks; ca.tiles = b->d_tiles; ca.inst = b->d_inst; ca.rec = rec; ca.slow_diag = b->d_slow_diag;
+        ca.tile_active = b->d_tile_active; ca.counters = counters(b); ca.partial = b->d_part;
+        hipEvent_t t2 = stage_begin(b, st);
+        ATAC_HIP(launch_contact(ca, mode, precise, b->grid_contact[precise ? 1 : 0][mode], st));
+        stage_end(b, ST_CONTACT, t2, st);
     }
-    ContactArgs ca{};
-    ca.q = queues(b, false, 1);
-    ca.tiles = b->d_tiles; ca.inst = b->d_inst; ca.grids = b->d_grids;
-    ca.act_off = b->d_act; ca.tile_active = b->d_tile_active;
-    ca.ck_dg = (flags & ATAC_EVAL_CHECKPOINT) ? b->d_ck : nullptr;
-    if (out) {
-        ca.fm = out->fm; ca.p6 = out->dfm_dp6; ca.Km = out->Km; ca.Dm = out->Dm; ca.n_active = out->n_active;
-    }
-    hipEvent_t t1 = stage_begin(b, st);
-    ATAC_HIP(launch_contact(ca, mode, precise, b->grid_contact, st));
-    stage_end(b, ST_CONTACT, t1, st);
-    b->ckpt_valid = (flags & ATAC_EVAL_CHECKPOINT) != 0;
-    b->ckpt_precise = precise;
+    ReduceArgs ra{};
+    ra.inst = b->d_inst; ra.n_inst = (int32_t)b->inst.size(); ra.nacc = (int32_t)part_width_of(mode);
+    ra.tile_active = b->d_tile_active; ra.tile_chunk0 = b->d_tile_chunk0; ra.partial = b->d_part;
+    ra.nseg = 4;
+    ra.dst[0] = out ? out->fm : nullptr;      ra.begin[0] = 0;   ra.width[0] = 12;
+    ra.dst[1] = out ? out->dfm_dp6 : nullptr; ra.begin[1] = 12;  ra.width[1] = 12;
+    ra.dst[2] = out ? out->Km : nullptr;      ra.begin[2] = 24;  ra.width[2] = 144;
+    ra.dst[3] = out ? out->Dm : nullptr;      ra.begin[3] = 168; ra.width[3] = 144;
+    ra.n_active = out ? out->n_active : nullptr;
+    hipEvent_t t3 = stage_begin(b, st);
+    ATAC_HIP(launch_reduce(ra, st));
+    stage_end(b, ST_REDUCE, t3, st);
+    b->rec_valid = true;
+    b->rec_precise = precise;
     return ATAC_OK;
 }
 
 static int enqueue_vjp(atac_batch b, const double* g_fm, const atac_gradients* g, hipStream_t st) {
-    if (!b->ckpt_valid) return fail(ATAC_ERR_STATE, "atac_batch_vjp: the last evaluation kept no checkpoint (ATAC_EVAL_CHECKPOINT)");
+    if (!b->rec_valid) return fail(ATAC_ERR_STATE, "atac_batch_vjp: no evaluation of this batch to differentiate");
     if (!g_fm) return fail(ATAC_ERR_INVALID, "atac_batch_vjp: g_fm is null");
+    int rc = ensure_buffers(b, 3, b->rec_precise);
+    if (rc) return rc;
     if (b->inst.empty()) return ATAC_OK;
-    ATAC_HIP(launch_reset(b->d_counters + 2 * b->nq, b->nq, st));
-    VjpArgs va{};
-    va.q = queues(b, false, 2);
-    va.tiles = b->d_tiles; va.inst = b->d_inst; va.grids = b->d_grids; va.tile_active = b->d_tile_active;
-    va.act_off = b->d_act; va.ck_dg = b->d_ck; va.g_fm = g_fm;
-    if (g) {
-        va.g_q1 = g->g_q1; va.g_q2 = g->g_q2; va.g_phi1 = g->g_phi1; va.g_phi2 = g->g_phi2; va.g_kn = g->g_kn;
-        va.g_kt = g->g_kt;
+    const bool precise = b->rec_precise;
+    ATAC_HIP(launch_reset(counters(b) + CNT_VJP_HEAD, 1, st));
+    if (b->n_tiles > 0) {
+        VjpArgs va{};
+        va.chunks = b->d_chunks; va.tiles = b->d_tiles; va.inst = b->d_inst;
+        va.rec = precise ? (const void*)b->d_rec64 : (const void*)b->d_rec32; va.slow_diag = b->d_slow_diag;
+        va.tile_active = b->d_tile_active; va.counters = counters(b); va.g_fm = g_fm; va.partial = b->d_part;
+        hipEvent_t t0 = stage_begin(b, st);
+        ATAC_HIP(launch_vjp(va, precise, b->grid_vjp[precise ? 1 : 0], st));
+        stage_end(b, ST_VJP, t0, st);
     }
-    hipEvent_t t3 = stage_begin(b, st);
-    ATAC_HIP(launch_vjp(va, b->ckpt_precise, b->grid_vjp, st));
-    stage_end(b, ST_VJP, t3, st);
+    ReduceArgs ra{};
+    ra.inst = b->d_inst; ra.n_inst = (int32_t)b->inst.size(); ra.nacc = kVjpOut;
+    ra.tile_active = b->d_tile_active; ra.tile_chunk0 = b->d_tile_chunk0; ra.partial = b->d_part;
+    ra.nseg = 6;
+    ra.dst[0] = g ? g->g_q1 : nullptr;   ra.begin[0] = 0;  ra.width[0] = 6;
+    ra.dst[1] = g ? g->g_q2 : nullptr;   ra.begin[1] = 6;  ra.width[1] = 6;
+    ra.dst[2] = g ? g->g_phi1 : nullptr; ra.begin[2] = 12; ra.width[2] = 6;
+    ra.dst[3] = g ? g->g_phi2 : nullptr; ra.begin[3] = 18; ra.width[3] = 6;
+    ra.dst[4] = g ? g->g_kn : nullptr;   ra.begin[4] = 24; ra.width[4] = 1;
+    ra.dst[5] = g ? g->g_kt : nullptr;   ra.begin[5] = 25; ra.width[5] = 1;
+    hipEvent_t t1 = stage_begin(b, st);
+    ATAC_HIP(launch_reduce(ra, st));
+    stage_end(b, ST_VJP_REDUCE, t1, st);
     return ATAC_OK;
 }
 
@@ -584,18 +688,17 @@ int atac_batch_vjp(atac_batch b, const double* g_fm, const atac_gradients* out, 
 int atac_batch_graph_capture(atac_batch b, uint32_t flags, const atac_outputs* out, const double* g_fm,
                              const atac_gradients* grads) {
     if (!b) return fail(ATAC_ERR_INVALID, "atac_batch_graph_capture: null batch");
-    if (g_fm && !(flags & ATAC_EVAL_CHECKPOINT)) return fail(ATAC_ERR_INVALID, "graph with adjoint needs ATAC_EVAL_CHECKPOINT");
-    if ((flags & ATAC_EVAL_CHECKPOINT) && !b->d_ck && b->n_slots > 0) {
-        int rc = dalloc(&b->d_ck, (size_t)b->n_slots);
-        if (rc) return rc;
-    }
+    // every buffer the captured launches use exists before capture begins
+    int rc = ensure_buffers(b, mode_of(flags), (flags & ATAC_EVAL_PRECISE) != 0);
+    if (!rc && g_fm) rc = ensure_buffers(b, 3, (flags & ATAC_EVAL_PRECISE) != 0);
+    if (rc) return rc;
     if (b->graph_exec) { (void)hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
     if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
     ATAC_HIP(hipStreamSynchronize(b->cap_stream));
     const bool timing = b->timing;
     b->timing = false;
     ATAC_HIP(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_eval(b, flags, out, b->cap_stream);
+    rc = enqueue_eval(b, flags, out, b->cap_stream);
     if (!rc && g_fm) rc = enqueue_vjp(b, g_fm, grads, b->cap_stream);
     b->timing = timing;
     hipGraph_t g = nullptr;
@@ -616,6 +719,7 @@ int atac_batch_graph_launch(atac_batch b, void* stream) {
 int atac_batch_active_ids(atac_batch b, int32_t inst, int64_t* ids, int64_t capacity, int64_t* count) {
     if (!b || !count) return fail(ATAC_ERR_INVALID, "atac_batch_active_ids: null argument");
     if (inst < 0 || inst >= (int32_t)b->inst.size()) return fail(ATAC_ERR_INVALID, "atac_batch_active_ids: bad instance");
+    if (!b->rec_valid) return fail(ATAC_ERR_STATE, "atac_batch_active_ids: the batch has not been evaluated");
     ATAC_HIP(hipDeviceSynchronize());
     const InstDev& d = b->inst_dev[inst];
     std::vector<int32_t> ta(d.tile_count);
@@ -623,12 +727,21 @@ int atac_batch_active_ids(atac_batch b, int32_t inst, int64_t* ids, int64_t capa
         ATAC_HIP(hipMemcpy(ta.data(), b->d_tile_active + d.tile_begin, sizeof(int32_t) * d.tile_count, hipMemcpyDeviceToHost));
     const atac_points_s& P = *b->pts[b->inst[inst].points];
     const int64_t seg_begin = P.seg[b->inst[inst].segment];
+    // every record starts with (xi1, bits(point offset within the instance))
+    const size_t stride = b->rec_precise ? sizeof(Rec64) : sizeof(Rec32);
+    const char* base = b->rec_precise ? (const char*)b->d_rec64 : (const char*)b->d_rec32;
     std::vector<int64_t> out;
     for (int k = 0; k < d.tile_count; ++k) {
         const TileDev& t = b->tiles[d.tile_begin + k];
-        std::vector<uint32_t> off(ta[k]);
-        if (ta[k]) ATAC_HIP(hipMemcpy(off.data(), b->d_act + t.slot, sizeof(uint32_t) * ta[k], hipMemcpyDeviceToHost));
-        for (uint32_t o : off) out.push_back(P.perm[seg_begin + t.begin + o]);
+        if (!ta[k]) continue;
+        std::vector<float> a4((size_t)ta[k] * 4);
+        ATAC_HIP(hipMemcpy2D(a4.data(), 4 * sizeof(float), base + (size_t)t.slot * stride, stride, 4 * sizeof(float),
+                             (size_t)ta[k], hipMemcpyDeviceToHost));
+        for (int j = 0; j < ta[k]; ++j) {
+            int32_t off;
+            std::memcpy(&off, &a4[4 * (size_t)j + 3], sizeof off);
+            out.push_back(P.perm[seg_begin + off]);
+        }
     }
     std::sort(out.begin(), out.end());
     *count = (int64_t)out.size();

@@ -49,6 +49,14 @@ inline void compose_instance(const atac_instance& s, const GridDev& G, const flo
     d.pt_count = n_points;
     d.grid = s.grid;
     d.pad = 0;
+    for (int k = 0; k < 9; ++k) { d.fR1[k] = (float)d.R1[k]; d.fR2[k] = (float)d.R2[k]; }
+    for (int k = 0; k < 3; ++k) {
+        d.fw1[k] = (float)d.w1[k]; d.fv1[k] = (float)d.v1[k];
+        d.fw2[k] = (float)d.w2[k]; d.fv2[k] = (float)d.v2[k];
+    }
+    d.fkn = (float)d.kn; d.fkt = (float)d.kt; d.fmu = (float)d.mu;
+    d.fdamping = (float)d.damping; d.fcontact_scale = (float)d.contact_scale;
+    for (float& f : d.fpad) f = 0.0f;
 }
 
 }  // namespace atac

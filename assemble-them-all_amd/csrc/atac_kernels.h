@@ -17,44 +17,66 @@ struct QueueDev {
 struct DetectArgs {
     QueueDev q;                 // tiles
     const TileDev* tiles;
-    int32_t wave_seg;           // LDS queue entries per wavefront
+    int32_t wave_span;          // points of a tile per wavefront (multiple of 64 x points per lane)
     int32_t all_active;         // every point is a contact (fixed contact lists)
     const InstDev* inst;
     const GridDev* grids;
-    uint32_t* act_off;          // active offsets, tile slice starts at tile.slot
+    void* rec;                  // Rec32 or Rec64 per slot
     int32_t* tile_active;       // n_tiles
+    int32_t* tile_chunk0;       // n_tiles: first work item of the tile
+    ChunkDev* chunks;           // work items, pushed per tile
+    uint32_t* slow_slot;        // slow list: record slots
+    int32_t* counters;          // CNT_*
+};
+
+struct SlowArgs {
+    const InstDev* inst;
+    const GridDev* grids;
+    void* rec;
+    float* slow_diag;           // 4 per slow contact (FP32 records)
+    const uint32_t* slow_slot;
+    const int32_t* counters;
 };
 
 struct ContactArgs {
-    QueueDev q;                 // instances
+    const ChunkDev* chunks;
     const TileDev* tiles;
     const InstDev* inst;
-    const GridDev* grids;
-    const uint32_t* act_off;
+    const void* rec;
+    const float* slow_diag;
     const int32_t* tile_active;
-    CkptDev* ck_dg;             // per active point (d, grad phi); nullptr: not kept
-    double* fm;                 // outputs per instance (any may be nullptr)
-    double* p6;
-    double* Km;
-    double* Dm;
-    int32_t* n_active;
+    int32_t* counters;
+    double* partial;            // per work item: fm 12 | p6 12 | Km 144 | Dm 144 (as the mode has them)
 };
 
 struct VjpArgs {
-    QueueDev q;                 // instances
+    const ChunkDev* chunks;
     const TileDev* tiles;
     const InstDev* inst;
-    const GridDev* grids;
+    const void* rec;
+    const float* slow_diag;
     const int32_t* tile_active;
-    const uint32_t* act_off;
-    const CkptDev* ck_dg;
+    int32_t* counters;
     const double* g_fm;         // n_inst x 12
-    double* g_q1;
-    double* g_q2;
-    double* g_phi1;
-    double* g_phi2;
-    double* g_kn;
-    double* g_kt;
+    double* partial;            // per work item: kVjpOut
+};
+
+// Per-instance sum of the work items' partials, in tile order then item order.
+// Component c of the partial row goes to dst[s][i * width[s] + c - begin[s]]
+// for the segment s holding it (dst[s] may be nullptr).
+struct ReduceArgs {
+    const InstDev* inst;
+    int32_t n_inst;
+    int32_t nacc;
+    const int32_t* tile_active;
+    const int32_t* tile_chunk0;
+    const double* partial;
+    int32_t nseg;
+    int32_t pad;
+    double* dst[6];
+    int32_t begin[6];
+    int32_t width[6];
+    int32_t* n_active;          // optional: active contacts per instance
 };
 
 struct PlainArgs {
@@ -86,12 +108,15 @@ struct TouchedArgs {
     const int64_t* bit_offset;  // per grid, in 32-bit words
 };
 
-hipError_t launch_detect(const DetectArgs& a, int block, int grid, size_t lds, hipStream_t st);
-const void* detect_kernel_ptr(int block);
+hipError_t launch_detect(const DetectArgs& a, int block, int ppl, bool precise, int grid, size_t lds, hipStream_t st);
+const void* detect_kernel_ptr(int block, int ppl, bool precise);
+hipError_t launch_slow(const SlowArgs& a, bool precise, int grid, hipStream_t st);
 hipError_t launch_contact(const ContactArgs& a, int mode, bool precise, int grid, hipStream_t st);
 const void* contact_kernel_ptr(int mode, bool precise);
 hipError_t launch_vjp(const VjpArgs& a, bool precise, int grid, hipStream_t st);
 const void* vjp_kernel_ptr(bool precise);
+hipError_t launch_reduce(const ReduceArgs& a, hipStream_t st);
+hipError_t launch_grid_setup(const GridDev& G, float4* cells, float2* blk, hipStream_t st);
 hipError_t launch_reset(int32_t* c, int n, hipStream_t st);
 hipError_t launch_zero(void* p, int64_t n_words, hipStream_t st);
 hipError_t launch_plain(const PlainArgs& a, int64_t n_points, bool precise, hipStream_t st);

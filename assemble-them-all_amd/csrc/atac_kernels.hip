@@ -1,27 +1,31 @@
 // gfx950 kernels of the point-vs-SDF contact path.
 //
-// Batched path (one persistent launch over all instances):
-//   detect_kernel  tiles of Morton-sorted FP32 contact points, one tile per
-//                  workgroup iteration, tiles pulled from an atomic queue. It
-//                  streams the tile's points (nontemporal loads), maps each into
-//                  grid index space with an FP64 affine map, gathers the 8 FP32
-//                  corners and compacts the points with d < 0 (per-wave LDS
-//                  segments filled by ballot + popcount, deterministic order)
-//                  into the tile's slice of a global active list.
-//   contact_kernel one force object per work item: the full contact (and
-//                  optionally the Km/Dm tangent columns) of its active points
-//                  with every lane busy, compensated FP32 sums, DPP wave
-//                  reductions, the FP64 result written in a fixed order.
-//   vjp_kernel     the adjoint over the checkpointed active contacts only.
-// Work items come from per-XCD queues (next_item).
+// Batched path:
+//   detect_kernel  persistent; tiles of Morton-sorted FP32 contact points pulled
+//                  from per-XCD queues. Streams each tile (several points per
+//                  lane, nontemporal vector loads), maps every point into grid
+//                  index space with an FP64 affine map, gathers the 8 FP32
+//                  corners, compacts the points with d < 0 in LDS and writes one
+//                  record per active point (sample of phi, grad phi, Hessian).
+//   slow_kernel    the FP64 FD-stencil samples of the few records whose stencil
+//                  leaves their cell.
+//   contact_kernel work items of kChunk records, one wavefront each: the wrench
+//                  (optionally the p6 column and Km/Dm tangent columns) summed
+//                  into the item's partial row.
+//   vjp_kernel     the adjoint of the same records.
+//   reduce_kernel  per-instance sums of the partial rows in a fixed order.
 // Plain path (the one-launch-per-instance baseline): plain_kernel, one lane per
 // point, inline contact math, FP64 atomics.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <type_traits>
 #include "atac_kernels.h"
 #include "contact_math.h"
 
 namespace atac {
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef float f2v __attribute__((ext_vector_type(2)));
 
 // ------------------------------------------------------------------ wave primitives
 template <int CTRL>
@@ -57,24 +61,24 @@ __device__ __forceinline__ T wave_sum(T v) {
     return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
 }
 
+// Per-lane wrench accumulator: Neumaier-compensated FP32 for the FP32 contact
+// arithmetic, plain FP64 for the FP64 one.
 template <typename R>
-struct Acc;
+struct LaneAcc;
 template <>
-struct Acc<float> {
+struct LaneAcc<float> {
     Comp32 c;
     __device__ void zero() { c.zero(); }
     __device__ void add(float v) { c.add(v); }
     __device__ double value() const { return c.value(); }
 };
 template <>
-struct Acc<double> {
+struct LaneAcc<double> {
     double s;
     __device__ void zero() { s = 0.0; }
     __device__ void add(double v) { s += v; }
     __device__ double value() const { return s; }
 };
-
-__device__ __forceinline__ float stream_load(const float* p) { return __builtin_nontemporal_load(p); }
 
 // ------------------------------------------------------------------ work queues
 // Items (tiles or instances) are split into nq contiguous queues; workgroup b
@@ -93,18 +97,59 @@ __device__ __forceinline__ int next_item(const QueueDev& Q, int& qid, int& tried
     return -1;
 }
 
-// ------------------------------------------------------------------ batched forward
-// Phase 1, streaming: every point of the tile is tested; active offsets are
-// compacted into per-wave LDS segments (ballot + popcount, deterministic order)
-// and copied to the tile's slice of the global active list.
-template <int BLOCK>
+// ------------------------------------------------------------------ records
+template <typename R> struct RecOf;
+template <> struct RecOf<float> { using type = Rec32; };
+template <> struct RecOf<double> { using type = Rec64; };
+
+__device__ __forceinline__ void store_rec(Rec32* p, const Rec32& r) {
+    float4* q = reinterpret_cast<float4*>(p);
+    q[0] = make_float4(r.a[0], r.a[1], r.a[2], r.a[3]);
+    q[1] = make_float4(r.b[0], r.b[1], r.b[2], r.b[3]);
+    q[2] = make_float4(r.c[0], r.c[1], r.c[2], r.c[3]);
+}
+__device__ __forceinline__ Rec32 load_rec(const Rec32* p) {
+    const float4* q = reinterpret_cast<const float4*>(p);
+    const float4 a = q[0], b = q[1], c = q[2];
+    Rec32 r;
+    r.a[0] = a.x; r.a[1] = a.y; r.a[2] = a.z; r.a[3] = a.w;
+    r.b[0] = b.x; r.b[1] = b.y; r.b[2] = b.z; r.b[3] = b.w;
+    r.c[0] = c.x; r.c[1] = c.y; r.c[2] = c.z; r.c[3] = c.w;
+    return r;
+}
+__device__ __forceinline__ void store_rec(Rec64* p, const Rec64& r) { *p = r; }
+__device__ __forceinline__ Rec64 load_rec(const Rec64* p) { return *p; }
+
+// Marks a record as waiting for the slow path: its instance is parked in the
+// depth field until the slow pass fills the sample in.
+__device__ __forceinline__ void mark_slow(Rec32& r, int inst, int slow) {
+    r.b[0] = __int_as_float(inst);
+    r.c[3] = __int_as_float(slow);
+}
+__device__ __forceinline__ void mark_slow(Rec64& r, int inst, int) { r.d = (double)inst; }
+__device__ __forceinline__ int parked_inst(const Rec32& r) { return __float_as_int(r.b[0]); }
+__device__ __forceinline__ int parked_inst(const Rec64& r) { return (int)r.d; }
+
+// ------------------------------------------------------------------ batched detection
+// Points of one tile per work item. Phase 1 streams the tile (PPL consecutive
+// points per lane, vector loads, each wavefront a contiguous span of the tile)
+// and compacts the offsets of the points with d < 0 into per-wave LDS segments
+// in ascending order (a wave-wide exclusive prefix of the per-lane counts built
+// from ballots of their bits). Phase 2 walks the compacted list with every lane
+// busy and writes each active point's record: the fast-path sample from its
+// 8 corners, or the point queued for the slow pass. The tile then pushes its
+// work items for the contact and adjoint passes.
+template <int BLOCK, int PPL, bool PREC>
 __global__ __launch_bounds__(BLOCK) void detect_kernel(DetectArgs a) {
+    using Rec = typename RecOf<typename std::conditional<PREC, double, float>::type>::type;
     constexpr int NW = BLOCK / 64;
+    constexpr int NB = PPL == 1 ? 1 : (PPL == 2 ? 2 : 3);   // bits of a per-lane count <= PPL
     extern __shared__ uint16_t s_queue[];   // tile offsets (tiles hold at most 65536 points)
     __shared__ int s_item;
     __shared__ int s_wcount[NW];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int seg = a.wave_seg;
+    const int span = a.wave_span;
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
     int qid = blockIdx.x % a.q.nq, tried = 0;
     for (;;) {
         if (tid == 0) s_item = next_item(a.q, qid, tried);
@@ -114,243 +159,291 @@ __global__ __launch_bounds__(BLOCK) void detect_kernel(DetectArgs a) {
         const TileDev t = a.tiles[tile];
         const InstDev& in = a.inst[t.inst];
         const GridDev G = a.grids[in.grid];
-        uint16_t* q = s_queue + w * seg;
+        const float* px = in.px + t.begin;
+        const float* py = in.py + t.begin;
+        const float* pz = in.pz + t.begin;
+        uint16_t* q = s_queue + w * span;
+        const int wb = w * span, we = min(t.count, wb + span);
         int wcount = 0;
-        for (int o0 = w * 64; o0 < t.count; o0 += BLOCK) {
-            const int o = o0 + lane;
-            bool act = false;
-            if (o < t.count) {
-                const int64_t p = t.begin + o;
-                act = a.all_active ||
-                      detect_value(G, in, stream_load(in.px + p), stream_load(in.py + p), stream_load(in.pz + p)) < 0.0f;
+        for (int o0 = wb; o0 < we; o0 += 64 * PPL) {
+            const int o = o0 + lane * PPL;
+            unsigned m = 0;
+            if (o < we) {
+                float x[PPL], y[PPL], z[PPL];
+                if constexpr (PPL == 4) {
+                    const f4v vx = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(px + o));
+                    const f4v vy = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(py + o));
+                    const f4v vz = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(pz + o));
+#pragma unroll
+                    for (int k = 0; k < PPL; ++k) { x[k] = vx[k]; y[k] = vy[k]; z[k] = vz[k]; }
+                } else if constexpr (PPL == 2) {
+                    const f2v vx = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(px + o));
+                    const f2v vy = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(py + o));
+                    const f2v vz = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(pz + o));
+#pragma unroll
+                    for (int k = 0; k < PPL; ++k) { x[k] = vx[k]; y[k] = vy[k]; z[k] = vz[k]; }
+                } else {
+                    x[0] = __builtin_nontemporal_load(px + o);
+                    y[0] = __builtin_nontemporal_load(py + o);
+                    z[0] = __builtin_nontemporal_load(pz + o);
+                }
+#pragma unroll
+                for (int k = 0; k < PPL; ++k)
+                    if (o + k < we && (a.all_active || detect_active(G, in, x[k], y[k], z[k]))) m |= 1u << k;
             }
-            const uint64_t m = __ballot(act);
-            if (act) q[wcount + __popcll(m & ((1ull << lane) - 1ull))] = (uint16_t)o;
-            wcount += __popcll(m);
+            const int c = __popc(m);
+            int pre = 0, tot = 0;
+#pragma unroll
+            for (int bit = 0; bit < NB; ++bit) {
+                const uint64_t bb = __ballot((c >> bit) & 1);
+                pre += __popcll(bb & lt_mask) << bit;
+                tot += __popcll(bb) << bit;
+            }
+            int pos = wcount + pre;
+#pragma unroll
+            for (int k = 0; k < PPL; ++k)
+                if ((m >> k) & 1u) q[pos++] = (uint16_t)(o + k);
+            wcount += tot;
         }
         if (lane == 0) s_wcount[w] = wcount;
         __syncthreads();
-        int pre = 0, cnt = 0;
+        int cnt = 0;
 #pragma unroll
-        for (int k = 0; k < NW; ++k) {
-            pre += (k < w) ? s_wcount[k] : 0;
-            cnt += s_wcount[k];
+        for (int k = 0; k < NW; ++k) cnt += s_wcount[k];
+        Rec* rec = static_cast<Rec*>(a.rec) + t.slot;
+        for (int qi = tid; qi < cnt; qi += BLOCK) {
+            int rem = qi, ww = 0;
+            while (rem >= s_wcount[ww]) { rem -= s_wcount[ww]; ++ww; }
+            const int off = s_queue[ww * span + rem];
+            const float x = px[off], y = py[off], z = pz[off];
+            double cc[3], fd[3];
+            int idx[3];
+            grid_coords(in, x, y, z, cc);
+            Rec r;
+            r.a[0] = x; r.a[1] = y; r.a[2] = z; r.a[3] = __int_as_float(off + (int)(t.begin));
+            if (fast_cell_c(G, in, cc, idx, fd)) {
+                const Corners cr = G.fast ? load_cell(G, idx[0], idx[1], idx[2]) : load_corners(G, idx[0], idx[1], idx[2]);
+                record_fast(G, in, cr, fd, r);
+            } else {
+                const int si = atomicAdd(a.counters + CNT_SLOW, 1);
+                a.slow_slot[si] = (uint32_t)(t.slot + qi);
+                mark_slow(r, t.inst, si);
+            }
+            store_rec(rec + qi, r);
         }
-        uint32_t* dst = a.act_off + t.slot + pre;
-        for (int k = lane; k < wcount; k += 64) dst[k] = q[k];
-        if (tid == 0) a.tile_active[tile] = cnt;
+        if (tid == 0) {
+            a.tile_active[tile] = cnt;
+            if (cnt > 0) {
+                const int nch = (cnt + kChunk - 1) / kChunk;
+                const int c0 = atomicAdd(a.counters + CNT_CHUNKS, nch);
+                a.tile_chunk0[tile] = c0;
+                for (int k = 0; k < nch; ++k) a.chunks[c0 + k] = ChunkDev{tile, k};
+            }
+        }
         __syncthreads();
     }
 }
 
-// The active contacts of one instance, flattened over its tiles: an exclusive
-// prefix of the tiles' active counts in LDS maps a flat index to (tile, entry).
-// Instances with more than kMaxTilesPerPass tiles are walked in passes.
-constexpr int kMaxTilesPerPass = 256;
-
-template <int BLOCK>
-struct ActiveList {
-    int32_t* pre;     // LDS, kMaxTilesPerPass + 1 entries
-    int t0, nt;       // tiles [t0, t0 + nt) of this pass
-    int total;
-    // fills pre[] for tiles t0 .. t0+nt-1 of the instance (all threads call)
-    __device__ void load(const int32_t* tile_active, int tile_first, int pass_t0, int pass_nt) {
-        t0 = pass_t0;
-        nt = pass_nt;
-        const int tid = threadIdx.x;
-        for (int k = tid; k < kMaxTilesPerPass + 1; k += BLOCK) pre[k] = 0;
-        __syncthreads();
-        for (int k = tid; k < nt; k += BLOCK) pre[k + 1] = tile_active[tile_first + t0 + k];
-        __syncthreads();
-        if (tid == 0)
-            for (int k = 0; k < nt; ++k) pre[k + 1] += pre[k];
-        __syncthreads();
-        total = pre[nt];
+// Slow-path samples of the queued records (a few per thousand contacts).
+template <bool PREC>
+__global__ __launch_bounds__(64) void slow_kernel(SlowArgs a) {
+    using Rec = typename RecOf<typename std::conditional<PREC, double, float>::type>::type;
+    const int n = a.counters[CNT_SLOW];
+    for (int i = blockIdx.x * 64 + threadIdx.x; i < n; i += gridDim.x * 64) {
+        Rec* p = static_cast<Rec*>(a.rec) + a.slow_slot[i];
+        Rec r = load_rec(p);
+        const InstDev& in = a.inst[parked_inst(r)];
+        record_slow(a.grids[in.grid], in, r, a.slow_diag + 4 * (int64_t)i);
+        store_rec(p, r);
     }
-    // tile index (relative to t0) holding flat entry q
-    __device__ int tile_of(int q) const {
-        int lo = 0, hi = nt - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (pre[mid] <= q) lo = mid; else hi = mid - 1;
-        }
-        return lo;
-    }
-};
+}
 
-// Phase 2: full contact (wrench, p6 column, optionally Km/Dm tangent columns)
-// of the active points of one instance per work item; final FP64 results are
-// written directly (fixed summation order: tile order, then queue order).
-template <int BLOCK, int MODE, typename R>
-__global__ __launch_bounds__(BLOCK) void contact_kernel(ContactArgs a) {
-    constexpr int NW = BLOCK / 64;
+// Next work item of a wavefront-granular pass, called with every lane active.
+// Every lane issues the atomic (lane 0 adds 1, the others 0; the compiler
+// folds them into one atomic per wavefront) and the index comes back
+// wave-uniform through readfirstlane: no lane-0-only branch feeds the loop
+// condition, so the pass loop is uniform and the lanes of a wavefront cannot be
+// split across its iterations (a lane-0-only fetch let the compiler retire lane
+// 0 from the loop while the other lanes kept iterating on a stale index).
+__device__ __forceinline__ int next_chunk(int32_t* head) {
+    const int lane = threadIdx.x & 63;
+    const int c = atomicAdd(head, lane == 0 ? 1 : 0);
+    return __builtin_amdgcn_readfirstlane(c);
+}
+
+// Writes n (<= 64) wave-uniform values to dst[0..n) with one store per lane.
+template <int N>
+__device__ __forceinline__ void store_row(double* dst, const double (&v)[N]) {
+    static_assert(N <= 64, "one value per lane");
+    const int lane = threadIdx.x & 63;
+    double mine = 0.0;
+#pragma unroll
+    for (int k = 0; k < N; ++k) mine = (lane == k) ? v[k] : mine;
+    if (lane < N) dst[lane] = mine;
+}
+
+// ------------------------------------------------------------------ batched contact pass
+// Work items of kChunk records, one wavefront each: the wrench (and p6 column,
+// and the Km/Dm tangent columns) of every record, summed per lane, then over
+// the wave in a fixed order into the item's partial row.
+template <int MODE, typename R>
+__global__ __launch_bounds__(256) void contact_kernel(ContactArgs a) {
+    using Rec = typename RecOf<R>::type;
     constexpr bool P6 = MODE >= 1;
     constexpr bool DERIV = MODE >= 2;
-    constexpr int NACC = P6 ? 24 : 12;
-    __shared__ int s_item;
-    __shared__ int32_t s_pre[kMaxTilesPerPass + 1];
-    __shared__ double s_red[NW][NACC];
-    __shared__ double s_km[DERIV ? NW : 1][DERIV ? 288 : 1];
-    __shared__ int s_cnt;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    int qid = blockIdx.x % a.q.nq, tried = 0;
-    ActiveList<BLOCK> L;
-    L.pre = s_pre;
+    constexpr int NF = P6 ? 24 : 12;
+    constexpr int NACC = NF + (DERIV ? 288 : 0);
+    __shared__ double s_km[DERIV ? 4 : 1][DERIV ? 288 : 1];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int total = a.counters[CNT_CHUNKS];
     for (;;) {
-        if (tid == 0) s_item = next_item(a.q, qid, tried);
-        __syncthreads();
-        const int i = s_item;
-        if (i < 0) break;
-        const InstDev& in = a.inst[i];
-        const GridDev G = a.grids[in.grid];
+        const int c = next_chunk(a.counters + CNT_CONTACT_HEAD);
+        if (c >= total) break;
+        const ChunkDev ch = a.chunks[c];
+        const TileDev t = a.tiles[ch.tile];
+        const InstDev& in = a.inst[t.inst];
+        const int n = min(kChunk, a.tile_active[ch.tile] - ch.k * kChunk);
+        const Rec* rec = static_cast<const Rec*>(a.rec) + t.slot + (int64_t)ch.k * kChunk;
+        const InstR<R> I = make_inst<R>(in);
+        LaneAcc<R> acc[NF];
+#pragma unroll
+        for (int k = 0; k < NF; ++k) acc[k].zero();
         if (DERIV)
             for (int k = lane; k < 288; k += 64) s_km[DERIV ? w : 0][DERIV ? k : 0] = 0.0;
-        Acc<R> acc[NACC];
+        for (int j0 = 0; j0 < n; j0 += 64) {
+            const int j = j0 + lane;
+            const bool have = j < n;
+            ContactFwd<R> s;
+            if (have) {
+                record_decode(load_rec(rec + j), a.slow_diag, in, s);
+                R p6[12];
+                contact_forward<R>(I, s, P6 ? p6 : nullptr);
 #pragma unroll
-        for (int k = 0; k < NACC; ++k) acc[k].zero();
-        const InstR<R> I = make_inst<R>(in);
-        int n_active = 0;
-        for (int pt0 = 0; pt0 < in.tile_count; pt0 += kMaxTilesPerPass) {
-            L.load(a.tile_active, in.tile_begin, pt0, min(kMaxTilesPerPass, in.tile_count - pt0));
-            n_active += L.total;
-            for (int base = 0; base < L.total; base += BLOCK) {
-                const int qi = base + tid;
-                const bool have = qi < L.total;
-                ContactFwd<R> s;
-                if (have) {
-                    const int tt = L.tile_of(qi);
-                    const TileDev t = a.tiles[in.tile_begin + L.t0 + tt];
-                    const int64_t slot = t.slot + (qi - L.pre[tt]);
-                    const int64_t p = t.begin + a.act_off[slot];
-                    contact_sample<R>(G, in, (double)in.px[p], (double)in.py[p], (double)in.pz[p], s);
-                    R p6[12];
-                    contact_forward<R>(I, s, P6 ? p6 : nullptr);
+                for (int k = 0; k < 12; ++k) acc[k].add(s.f[k]);
+                if (P6) {
 #pragma unroll
-                    for (int k = 0; k < 12; ++k) acc[k].add(s.f[k]);
-                    if (P6) {
-#pragma unroll
-                        for (int k = 0; k < 12; ++k) acc[P6 ? 12 + k : 0].add(p6[k]);
-                    }
-                    if (a.ck_dg) {
-                        const float hi = (float)s.d;
-                        a.ck_dg[slot] = CkptDev{hi, (float)((double)s.d - (double)hi), (float)s.g.x, (float)s.g.y,
-                                                (float)s.g.z};
-                    }
-                }
-                if (DERIV) {
-                    // columns of Km (j < 12) and Dm (j >= 12); every lane takes part in the sums
-                    for (int j = 0; j < 24; ++j) {
-                        R col[12];
-#pragma unroll
-                        for (int r = 0; r < 12; ++r) col[r] = (R)0;
-                        if (have) contact_jvp<R>(I, s, unit_tangent<R>(j), col);
-                        const int obase = (j < 12) ? j : 144 + (j - 12);
-#pragma unroll
-                        for (int r = 0; r < 12; ++r) {
-                            const R v = wave_sum(col[r]);
-                            if (lane == 0) s_km[DERIV ? w : 0][DERIV ? obase + 12 * r : 0] += (double)v;
-                        }
-                    }
+                    for (int k = 0; k < 12; ++k) acc[P6 ? 12 + k : 0].add(p6[k]);
                 }
             }
-            __syncthreads();   // s_pre is reloaded by the next pass
-        }
+            if (DERIV) {
+                // columns of Km (jj < 12) and Dm (jj >= 12); every lane takes part in the sums
+                for (int jj = 0; jj < 24; ++jj) {
+                    R col[12];
 #pragma unroll
-        for (int k = 0; k < NACC; ++k) {
-            const double v = wave_sum(acc[k].value());
-            if (lane == 0) s_red[w][k] = v;
-        }
-        if (tid == 0) s_cnt = n_active;
-        __syncthreads();
-        if (tid < NACC) {
-            double sum = 0.0;
+                    for (int r = 0; r < 12; ++r) col[r] = (R)0;
+                    if (have) contact_jvp<R>(I, s, unit_tangent<R>(jj), col);
+                    const int obase = (jj < 12) ? jj : 144 + (jj - 12);
+                    double mine = 0.0;   // lane r < 12 keeps row r of the column
 #pragma unroll
-            for (int k = 0; k < NW; ++k) sum += s_red[k][tid];
-            double* dst = tid < 12 ? a.fm : a.p6;
-            if (dst) dst[(int64_t)i * 12 + (tid % 12)] = sum;
+                    for (int r = 0; r < 12; ++r) {
+                        const double v = wave_sum((double)col[r]);
+                        mine = (lane == r) ? v : mine;
+                    }
+                    if (lane < 12) s_km[DERIV ? w : 0][DERIV ? obase + 12 * lane : 0] += mine;
+                }
+            }
         }
+        double* dst = a.partial + (int64_t)c * NACC;
+        double row[NF];
+#pragma unroll
+        for (int k = 0; k < NF; ++k) row[k] = wave_sum(acc[k].value());
+        store_row<NF>(dst, row);
         if (DERIV) {
-            for (int k = tid; k < 288; k += BLOCK) {
-                double sum = 0.0;
-#pragma unroll
-                for (int ww = 0; ww < NW; ++ww) sum += s_km[DERIV ? ww : 0][DERIV ? k : 0];
-                double* dst = k < 144 ? a.Km : a.Dm;
-                if (dst) dst[(int64_t)i * 144 + (k % 144)] = sum;
-            }
+            __builtin_amdgcn_wave_barrier();
+            for (int k = lane; k < 288; k += 64) dst[NF + k] = s_km[DERIV ? w : 0][DERIV ? k : 0];
+            __builtin_amdgcn_wave_barrier();
         }
-        if (tid == 0 && a.n_active) a.n_active[i] = s_cnt;
-        __syncthreads();
     }
 }
 
 // ------------------------------------------------------------------ batched adjoint
-// One instance per work item over its checkpointed active contacts.
-template <int BLOCK, typename R>
-__global__ __launch_bounds__(BLOCK) void vjp_kernel(VjpArgs a) {
-    constexpr int NW = BLOCK / 64;
-    __shared__ int s_item;
-    __shared__ int32_t s_pre[kMaxTilesPerPass + 1];
-    __shared__ double s_red[NW][kVjpOut];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    int qid = blockIdx.x % a.q.nq, tried = 0;
-    ActiveList<BLOCK> L;
-    L.pre = s_pre;
+// The adjoint of every record of a work item, with d(loss)/d(wrench) of its
+// instance, summed into the item's partial row (kVjpOut).
+template <typename R>
+__global__ __launch_bounds__(256) void vjp_kernel(VjpArgs a) {
+    using Rec = typename RecOf<R>::type;
+    const int lane = threadIdx.x & 63;
+    const int total = a.counters[CNT_CHUNKS];
     for (;;) {
-        if (tid == 0) s_item = next_item(a.q, qid, tried);
-        __syncthreads();
-        const int i = s_item;
-        if (i < 0) break;
-        const InstDev& in = a.inst[i];
-        const GridDev G = a.grids[in.grid];
+        const int c = next_chunk(a.counters + CNT_VJP_HEAD);
+        if (c >= total) break;
+        const ChunkDev ch = a.chunks[c];
+        const TileDev t = a.tiles[ch.tile];
+        const InstDev& in = a.inst[t.inst];
+        const int n = min(kChunk, a.tile_active[ch.tile] - ch.k * kChunk);
+        const Rec* rec = static_cast<const Rec*>(a.rec) + t.slot + (int64_t)ch.k * kChunk;
         const InstR<R> I = make_inst<R>(in);
         R gb[12];
 #pragma unroll
-        for (int k = 0; k < 12; ++k) gb[k] = (R)a.g_fm[(int64_t)i * 12 + k];
+        for (int k = 0; k < 12; ++k) gb[k] = (R)a.g_fm[(int64_t)t.inst * 12 + k];
         R acc[kVjpOut];
 #pragma unroll
         for (int k = 0; k < kVjpOut; ++k) acc[k] = (R)0;
-        for (int pt0 = 0; pt0 < in.tile_count; pt0 += kMaxTilesPerPass) {
-            L.load(a.tile_active, in.tile_begin, pt0, min(kMaxTilesPerPass, in.tile_count - pt0));
-            for (int qi = tid; qi < L.total; qi += BLOCK) {
-                const int tt = L.tile_of(qi);
-                const TileDev t = a.tiles[in.tile_begin + L.t0 + tt];
-                const int64_t slot = t.slot + (qi - L.pre[tt]);
-                const int64_t p = t.begin + a.act_off[slot];
-                const CkptDev ck = a.ck_dg[slot];
-                ContactFwd<R> s;
-                contact_sample<R>(G, in, (double)in.px[p], (double)in.py[p], (double)in.pz[p], s, &ck);
-                contact_forward<R>(I, s, nullptr);
-                R v[kVjpOut];
-                contact_vjp<R>(I, s, gb, v);
-                v[24] *= (R)in.kn_chain;   // d/d kn = kn_chain * d/d kn_eff (ForceGeneralSDFContact.cpp:155-156)
-                v[25] *= (R)in.kn_chain;
-#pragma unroll
-                for (int k = 0; k < kVjpOut; ++k) acc[k] += v[k];
-            }
-            __syncthreads();
+        for (int j = lane; j < n; j += 64) {
+            ContactFwd<R> s;
+            record_decode(load_rec(rec + j), a.slow_diag, in, s);
+            contact_forward<R>(I, s, nullptr);
+            contact_vjp<R, true>(I, s, gb, acc);
         }
+        double* dst = a.partial + (int64_t)c * kVjpOut;
+        double row[kVjpOut];
 #pragma unroll
-        for (int k = 0; k < kVjpOut; ++k) {
-            const double v = wave_sum((double)acc[k]);
-            if (lane == 0) s_red[w][k] = v;
-        }
-        __syncthreads();
-        if (tid < kVjpOut) {
-            double sum = 0.0;
-#pragma unroll
-            for (int k = 0; k < NW; ++k) sum += s_red[k][tid];
-            double* dst = nullptr;
-            int col = 0, width = 6;
-            if (tid < 6) { dst = a.g_q1; col = tid; }
-            else if (tid < 12) { dst = a.g_q2; col = tid - 6; }
-            else if (tid < 18) { dst = a.g_phi1; col = tid - 12; }
-            else if (tid < 24) { dst = a.g_phi2; col = tid - 18; }
-            else if (tid == 24) { dst = a.g_kn; col = 0; width = 1; }
-            else { dst = a.g_kt; col = 0; width = 1; }
-            if (dst) dst[(int64_t)i * width + col] = sum;
-        }
-        __syncthreads();
+        for (int k = 0; k < kVjpOut; ++k) row[k] = wave_sum((double)acc[k]);
+        row[24] *= in.kn_chain;   // d/d kn = kn_chain * d/d kn_eff (ForceGeneralSDFContact.cpp:155-156)
+        row[25] *= in.kn_chain;
+        store_row<kVjpOut>(dst, row);
     }
+}
+
+// ------------------------------------------------------------------ per-instance reduction
+// One wavefront per instance: the partial rows of its tiles' work items, in
+// tile order then item order; every instance's outputs are written (zeros when
+// it has no active contact).
+__global__ __launch_bounds__(256) void reduce_kernel(ReduceArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= a.n_inst) return;
+    const InstDev& in = a.inst[i];
+    const int tb = in.tile_begin, te = in.tile_begin + in.tile_count;
+    for (int comp = lane; comp < a.nacc; comp += 64) {
+        double sum = 0.0;
+        for (int t = tb; t < te; ++t) {
+            const int cnt = a.tile_active[t];
+            if (cnt <= 0) continue;
+            const int c0 = a.tile_chunk0[t], nch = (cnt + kChunk - 1) / kChunk;
+            for (int k = 0; k < nch; ++k) sum += a.partial[(int64_t)(c0 + k) * a.nacc + comp];
+        }
+        for (int sg = 0; sg < a.nseg; ++sg) {
+            const int rel = comp - a.begin[sg];
+            if (rel >= 0 && rel < a.width[sg] && a.dst[sg]) a.dst[sg][(int64_t)i * a.width[sg] + rel] = sum;
+        }
+    }
+    if (a.n_active && lane == 0) {
+        int n = 0;
+        for (int t = tb; t < te; ++t) n += a.tile_active[t];
+        a.n_active[i] = n;
+    }
+}
+
+// ------------------------------------------------------------------ grid set-up
+__global__ void build_cells_kernel(GridDev G, float4* cells) {
+    const int64_t ci = G.n[0] - 1, cj = G.n[1] - 1, ck = G.n[2] - 1, nc = ci * cj * ck;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x)
+        pack_cell(G, (int)(c % ci), (int)((c / ci) % cj), (int)(c / (ci * cj)), cells + 2 * c);
+}
+
+__global__ void build_blocks_kernel(GridDev G, float2* blk) {
+    const int64_t nb0 = G.nb[0], nb1 = G.nb[1], nb = nb0 * nb1 * G.nb[2];
+    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += (int64_t)gridDim.x * blockDim.x)
+        blk[b] = block_bounds(G, (int)(b % nb0), (int)((b / nb0) % nb1), (int)(b / (nb0 * nb1)));
+}
+
+hipError_t launch_grid_setup(const GridDev& G, float4* cells, float2* blk, hipStream_t st) {
+    hipLaunchKernelGGL(build_cells_kernel, dim3(1024), dim3(256), 0, st, G, cells);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(build_blocks_kernel, dim3(1024), dim3(256), 0, st, G, blk);
+    return hipGetLastError();
 }
 
 __global__ void reset_counters_kernel(int32_t* c, int n) {
@@ -512,29 +605,43 @@ __global__ void popcount_kernel(const uint32_t* bits, int64_t nwords, unsigned l
 }
 
 // ------------------------------------------------------------------ launchers
-hipError_t launch_detect(const DetectArgs& a, int block, int grid, size_t lds, hipStream_t st) {
-    switch (block) {
-        case 128: hipLaunchKernelGGL(detect_kernel<128>, dim3(grid), dim3(128), lds, st, a); break;
-        case 256: hipLaunchKernelGGL(detect_kernel<256>, dim3(grid), dim3(256), lds, st, a); break;
-        case 512: hipLaunchKernelGGL(detect_kernel<512>, dim3(grid), dim3(512), lds, st, a); break;
-        case 1024: hipLaunchKernelGGL(detect_kernel<1024>, dim3(grid), dim3(1024), lds, st, a); break;
-        default: return hipErrorInvalidValue;
+template <int BLOCK>
+static const void* detect_ptr_b(int ppl, bool precise) {
+    if (precise) {
+        return ppl == 1 ? (const void*)&detect_kernel<BLOCK, 1, true>
+             : ppl == 2 ? (const void*)&detect_kernel<BLOCK, 2, true> : (const void*)&detect_kernel<BLOCK, 4, true>;
     }
-    return hipGetLastError();
+    return ppl == 1 ? (const void*)&detect_kernel<BLOCK, 1, false>
+         : ppl == 2 ? (const void*)&detect_kernel<BLOCK, 2, false> : (const void*)&detect_kernel<BLOCK, 4, false>;
 }
 
-const void* detect_kernel_ptr(int block) {
+const void* detect_kernel_ptr(int block, int ppl, bool precise) {
+    if (ppl != 1 && ppl != 2 && ppl != 4) return nullptr;
     switch (block) {
-        case 128: return (const void*)&detect_kernel<128>;
-        case 256: return (const void*)&detect_kernel<256>;
-        case 512: return (const void*)&detect_kernel<512>;
-        case 1024: return (const void*)&detect_kernel<1024>;
+        case 128: return detect_ptr_b<128>(ppl, precise);
+        case 256: return detect_ptr_b<256>(ppl, precise);
+        case 512: return detect_ptr_b<512>(ppl, precise);
+        case 1024: return detect_ptr_b<1024>(ppl, precise);
         default: return nullptr;
     }
 }
 
+hipError_t launch_detect(const DetectArgs& a, int block, int ppl, bool precise, int grid, size_t lds, hipStream_t st) {
+    const void* f = detect_kernel_ptr(block, ppl, precise);
+    if (!f) return hipErrorInvalidValue;
+    DetectArgs args = a;
+    void* params[] = {&args};
+    return hipLaunchKernel(f, dim3(grid), dim3(block), params, lds, st);
+}
+
+hipError_t launch_slow(const SlowArgs& a, bool precise, int grid, hipStream_t st) {
+    if (precise) hipLaunchKernelGGL(slow_kernel<true>, dim3(grid), dim3(64), 0, st, a);
+    else hipLaunchKernelGGL(slow_kernel<false>, dim3(grid), dim3(64), 0, st, a);
+    return hipGetLastError();
+}
+
 template <int MODE, typename R>
-static const void* contact_ptr_t() { return (const void*)&contact_kernel<256, MODE, R>; }
+static const void* contact_ptr_t() { return (const void*)&contact_kernel<MODE, R>; }
 
 const void* contact_kernel_ptr(int mode, bool precise) {
     if (precise) return mode == 0 ? contact_ptr_t<0, double>() : mode == 1 ? contact_ptr_t<1, double>() : contact_ptr_t<2, double>();
@@ -542,26 +649,25 @@ const void* contact_kernel_ptr(int mode, bool precise) {
 }
 
 hipError_t launch_contact(const ContactArgs& a, int mode, bool precise, int grid, hipStream_t st) {
-    if (precise) {
-        if (mode == 0) hipLaunchKernelGGL((contact_kernel<256, 0, double>), dim3(grid), dim3(256), 0, st, a);
-        else if (mode == 1) hipLaunchKernelGGL((contact_kernel<256, 1, double>), dim3(grid), dim3(256), 0, st, a);
-        else hipLaunchKernelGGL((contact_kernel<256, 2, double>), dim3(grid), dim3(256), 0, st, a);
-    } else {
-        if (mode == 0) hipLaunchKernelGGL((contact_kernel<256, 0, float>), dim3(grid), dim3(256), 0, st, a);
-        else if (mode == 1) hipLaunchKernelGGL((contact_kernel<256, 1, float>), dim3(grid), dim3(256), 0, st, a);
-        else hipLaunchKernelGGL((contact_kernel<256, 2, float>), dim3(grid), dim3(256), 0, st, a);
-    }
-    return hipGetLastError();
+    ContactArgs args = a;
+    void* params[] = {&args};
+    return hipLaunchKernel(contact_kernel_ptr(mode, precise), dim3(grid), dim3(256), params, 0, st);
 }
 
 hipError_t launch_vjp(const VjpArgs& a, bool precise, int grid, hipStream_t st) {
-    if (precise) hipLaunchKernelGGL((vjp_kernel<256, double>), dim3(grid), dim3(256), 0, st, a);
-    else hipLaunchKernelGGL((vjp_kernel<256, float>), dim3(grid), dim3(256), 0, st, a);
+    if (precise) hipLaunchKernelGGL(vjp_kernel<double>, dim3(grid), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL(vjp_kernel<float>, dim3(grid), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
 const void* vjp_kernel_ptr(bool precise) {
-    return precise ? (const void*)&vjp_kernel<256, double> : (const void*)&vjp_kernel<256, float>;
+    return precise ? (const void*)&vjp_kernel<double> : (const void*)&vjp_kernel<float>;
+}
+
+hipError_t launch_reduce(const ReduceArgs& a, hipStream_t st) {
+    if (a.n_inst <= 0) return hipSuccess;
+    hipLaunchKernelGGL(reduce_kernel, dim3((a.n_inst + 3) / 4), dim3(256), 0, st, a);
+    return hipGetLastError();
 }
 
 hipError_t launch_reset(int32_t* c, int n, hipStream_t st) {

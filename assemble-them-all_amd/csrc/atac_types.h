@@ -1,6 +1,7 @@
 // Device-side data layout shared by the kernels and the C-ABI implementation.
 #pragma once
 #include <cstdint>
+#include <hip/hip_runtime.h>
 
 namespace atac {
 
@@ -9,12 +10,23 @@ namespace atac {
 struct GridDev {
     const float* v;
     int32_t n[3];
-    int32_t pad;
+    int32_t fast;       // 1: cells and blk below exist (every n >= 2)
     double minb[3];
     double maxb[3];
     double d[3];
     double inv_d[3];
+    // corner-packed copy: cell (i, j, k) < n - 1 holds its 8 corners
+    // (v000 v100 v010 v110 | v001 v101 v011 v111) in 2 x float4, cell-index order
+    const float4* cells;
+    // conservative bounds of the detection value per block of kBlk^3 cells:
+    // (min, max) of every node the block's cells read (with the detection's clamp
+    // at the last node), or (-inf, +inf) if one of them is not finite
+    const float2* blk;
+    int32_t nb[3];      // blocks per axis: (n - 1) / kBlk + 1
+    int32_t pad;
 };
+
+constexpr int kBlk = 2;
 
 // One force object = (contact body points, SDF body grid, poses, parameters).
 // Everything the kernels read per instance, in FP64 as the reference keeps it.
@@ -39,7 +51,13 @@ struct InstDev {
     int32_t pad;
     int32_t tile_begin; // first tile of this instance (tiles are contiguous per instance)
     int32_t tile_count;
+    // the same poses, twists and parameters rounded to FP32 for the FP32 contact
+    // arithmetic (read as wave-uniform scalars)
+    float fR1[9], fR2[9], fw1[3], fv1[3], fw2[3], fv2[3];
+    float fkn, fkt, fmu, fdamping, fcontact_scale;
+    float fpad[13];
 };
+static_assert(sizeof(InstDev) == 704, "InstDev layout");
 
 struct TileDev {
     int32_t inst;
@@ -48,11 +66,47 @@ struct TileDev {
     int64_t slot;       // first slot of the tile in the batch-wide active list
 };
 
-// Active-contact checkpoint, written by the forward pass at slot
-// (tile.slot + queue position): (d, grad phi) of the active point; d is kept as
-// an unevaluated FP32 pair (d = d_hi + d_lo) so the FP64 depth survives.
-struct CkptDev {
-    float d_hi, d_lo, g0, g1, g2;
+// Active-contact record, written by the detection pass for every point with
+// d < 0 at rec[tile.slot + k] (k-th active point of the tile, ascending point
+// order) and read by the contact and adjoint passes (it is also the adjoint's
+// checkpoint: the same d, grad phi and Hessian are used by both).
+//   a = (xi1.x, xi1.y, xi1.z, bits(point offset within the instance))
+//   b = (d, g.x, g.y, g.z)            d = query_SDF(x) + col_th, g = grad phi
+//   c = (Hxy, Hxz, Hyz, bits(slow))   slow = -1, or the index of the contact in
+//                                     the slow list (its FD stencil leaves the
+//                                     cell; Hxx Hyy Hzz then sit in slow_diag)
+// FP32 record (48 B), for the FP32 contact arithmetic.
+struct Rec32 {
+    float a[4];
+    float b[4];
+    float c[4];
+};
+
+// FP64 record (96 B), for the FP64 contact arithmetic: (xi1, point offset) as in
+// Rec32, then d, grad phi and the six Hessian entries (xx yy zz xy xz yz).
+struct Rec64 {
+    float a[4];
+    double d;
+    double g[3];
+    double H[6];
+};
+
+// A work item of the contact and adjoint passes: records
+// [tile.slot + k*kChunk, min(+kChunk, tile_active)) of one tile.
+struct ChunkDev {
+    int32_t tile;
+    int32_t k;
+};
+
+constexpr int kChunk = 256;         // records per work item
+
+// Device counters of one evaluation (reset before detection).
+enum {
+    CNT_CHUNKS = 0,                 // work items pushed by detection
+    CNT_SLOW = 1,                   // contacts in the slow list
+    CNT_CONTACT_HEAD = 2,           // next work item of the contact pass
+    CNT_VJP_HEAD = 3,               // next work item of the adjoint pass
+    CNT_WORDS = 4
 };
 
 constexpr int kFmOut = 12;          // wrench of both bodies

@@ -184,6 +184,41 @@ ATAC_HD Corners load_corners(const GridDev& G, int i, int j, int k) {
     return c;
 }
 
+// The corners of interior cell (i, j, k) from the corner-packed copy.
+ATAC_HD Corners load_cell(const GridDev& G, int i, int j, int k) {
+    const float4* q = G.cells + 2 * ((int64_t)i + (int64_t)(G.n[0] - 1) * ((int64_t)j + (int64_t)(G.n[1] - 1) * k));
+    const float4 lo = q[0], hi = q[1];
+    Corners c;
+    c.v000 = lo.x; c.v100 = lo.y; c.v010 = lo.z; c.v110 = lo.w;
+    c.v001 = hi.x; c.v101 = hi.y; c.v011 = hi.z; c.v111 = hi.w;
+    return c;
+}
+
+// Set-up of the corner-packed copy and the block bounds (one cell / one block).
+ATAC_HD void pack_cell(const GridDev& G, int i, int j, int k, float4 out[2]) {
+    const Corners q = load_corners(G, i, j, k);
+    out[0] = make_float4(q.v000, q.v100, q.v010, q.v110);
+    out[1] = make_float4(q.v001, q.v101, q.v011, q.v111);
+}
+
+ATAC_HD float2 block_bounds(const GridDev& G, int bi, int bj, int bk) {
+    // nodes read by the block's cells: [b*kBlk, min(b*kBlk + kBlk, n - 1)] per axis
+    const int i0 = bi * kBlk, i1 = i0 + kBlk < G.n[0] - 1 ? i0 + kBlk : G.n[0] - 1;
+    const int j0 = bj * kBlk, j1 = j0 + kBlk < G.n[1] - 1 ? j0 + kBlk : G.n[1] - 1;
+    const int k0 = bk * kBlk, k1 = k0 + kBlk < G.n[2] - 1 ? k0 + kBlk : G.n[2] - 1;
+    float lo = INFINITY, hi = -INFINITY;
+    bool finite = true;
+    for (int k = k0; k <= k1; ++k)
+        for (int j = j0; j <= j1; ++j)
+            for (int i = i0; i <= i1; ++i) {
+                const float v = G.v[grid_index(G, i, j, k)];
+                finite = finite && std::isfinite(v);
+                lo = fminf(lo, v);
+                hi = fmaxf(hi, v);
+            }
+    return finite ? make_float2(lo, hi) : make_float2(-INFINITY, INFINITY);
+}
+
 // FP64 trilinear value of a cell from its FP32 corners, in the reference's
 // i -> j -> k order (BodySDFObj.cpp:143-153).
 ATAC_HD double cell_value64(const Corners& c, const double f[3]) {
@@ -221,11 +256,17 @@ ATAC_HD void sdf_cell32(const GridDev& G, const Corners& c, const float f[3], fl
 // c = A xi + b, the cell and fraction are taken in FP64, the 8 corners and the
 // trilinear value in FP32. Returns the value query_SDF(x) + col_th, or +inf when
 // x is outside [min_box, max_box] (the reference returns FLT_MAX + col_th).
-ATAC_HD float detect_value(const GridDev& G, const InstDev& in, float px, float py, float pz) {
+ATAC_HD void grid_coords(const InstDev& in, float px, float py, float pz, double c[3]) {
     const double x = px, y = py, z = pz;
-    const double c0 = in.A[0] * x + in.A[1] * y + in.A[2] * z + in.b[0];
-    const double c1 = in.A[3] * x + in.A[4] * y + in.A[5] * z + in.b[1];
-    const double c2 = in.A[6] * x + in.A[7] * y + in.A[8] * z + in.b[2];
+    c[0] = in.A[0] * x + in.A[1] * y + in.A[2] * z + in.b[0];
+    c[1] = in.A[3] * x + in.A[4] * y + in.A[5] * z + in.b[1];
+    c[2] = in.A[6] * x + in.A[7] * y + in.A[8] * z + in.b[2];
+}
+
+ATAC_HD float detect_value(const GridDev& G, const InstDev& in, float px, float py, float pz) {
+    double cc[3];
+    grid_coords(in, px, py, pz, cc);
+    const double c0 = cc[0], c1 = cc[1], c2 = cc[2];
     const bool inside = c0 >= 0.0 && c0 <= in.cmax[0] && c1 >= 0.0 && c1 <= in.cmax[1] && c2 >= 0.0 && c2 <= in.cmax[2];
     if (!inside) return INFINITY;
     int i = (int)c0, j = (int)c1, k = (int)c2;  // c >= 0: truncation is floor
@@ -241,6 +282,55 @@ ATAC_HD float detect_value(const GridDev& G, const InstDev& in, float px, float 
     const float a = lerpf(v[0], v[di], fx), b = lerpf(v[dj], v[dj + di], fx);
     const float c = lerpf(v[dk], v[dk + di], fx), d = lerpf(v[dk + dj], v[dk + dj + di], fx);
     return lerpf(lerpf(a, b, fy), lerpf(c, d, fy), fz) + (float)in.col_th;
+}
+
+// The detection decision detect_value(...) < 0, bit for bit, with fewer grid
+// reads. The trilinear value is a convex combination of its 8 corners, so it lies
+// in the (min, max) of the block holding the cell, up to the FP32 rounding of the
+// 7 lerps and the col_th add: at most 13 units of 2^-24 of max|corner| + |col_th|
+// (each lerp adds <= 4 units of its largest input; the add 1 unit). A block whose
+// bounds clear the threshold by 4e-6 of that scale (5x the rounding bound) settles
+// the point; otherwise the value is interpolated exactly as detect_value does,
+// from the corner-packed cell (interior cells) or the 8 nodes (clamped cells).
+ATAC_HD bool detect_active(const GridDev& G, const InstDev& in, float px, float py, float pz) {
+    if (!G.fast) return detect_value(G, in, px, py, pz) < 0.0f;
+    double cc[3];
+    grid_coords(in, px, py, pz, cc);
+    const double c0 = cc[0], c1 = cc[1], c2 = cc[2];
+    const bool inside = c0 >= 0.0 && c0 <= in.cmax[0] && c1 >= 0.0 && c1 <= in.cmax[1] && c2 >= 0.0 && c2 <= in.cmax[2];
+    if (!inside) return false;
+    int i = (int)c0, j = (int)c1, k = (int)c2;
+    i = i < G.n[0] - 1 ? i : G.n[0] - 1;
+    j = j < G.n[1] - 1 ? j : G.n[1] - 1;
+    k = k < G.n[2] - 1 ? k : G.n[2] - 1;
+    const float cth = (float)in.col_th;
+    const float2 b = G.blk[i / kBlk + G.nb[0] * (j / kBlk + G.nb[1] * (k / kBlk))];
+    const float tol = 4e-6f * (fmaxf(fabsf(b.x), fabsf(b.y)) + fabsf(cth));
+    if (b.x + cth > tol) return false;
+    if (b.y + cth < -tol) return true;
+    if (i > G.n[0] - 2 || j > G.n[1] - 2 || k > G.n[2] - 2) return detect_value(G, in, px, py, pz) < 0.0f;
+    const float fx = (float)(c0 - (double)i), fy = (float)(c1 - (double)j), fz = (float)(c2 - (double)k);
+    const float4* q = G.cells + 2 * ((int64_t)i + (int64_t)(G.n[0] - 1) * ((int64_t)j + (int64_t)(G.n[1] - 1) * k));
+    const float4 lo = q[0], hi = q[1];
+    const float a = lerpf(lo.x, lo.y, fx), bb = lerpf(lo.z, lo.w, fx);
+    const float c = lerpf(hi.x, hi.y, fx), d = lerpf(hi.z, hi.w, fx);
+    return lerpf(lerpf(a, bb, fy), lerpf(c, d, fy), fz) + cth < 0.0f;
+}
+
+// fast_cell in grid-index coordinates c = (x - min_box) / d (the detection map):
+// the cell is interior, and the FD stencil (+-1e-5 along each axis) stays inside
+// both the cell and [min_box, max_box] with kCellMargin cell units to spare.
+ATAC_HD bool fast_cell_c(const GridDev& G, const InstDev& in, const double c[3], int idx[3], double fd[3]) {
+    for (int a = 0; a < 3; ++a) {
+        const double fl = floor(c[a]);
+        if (!(fl >= 0.0 && fl <= (double)(G.n[a] - 2))) return false;
+        const double fr = c[a] - fl;
+        const double m = kFdEps * G.inv_d[a] + kCellMargin;
+        if (fr < m || fr > 1.0 - m || c[a] + m > in.cmax[a]) return false;
+        idx[a] = (int)fl;
+        fd[a] = fr;
+    }
+    return true;
 }
 
 // FP64 value, closed-form gradient and mixed partials of the trilinear cell, from
@@ -282,6 +372,20 @@ ATAC_HD InstR<R> make_inst(const InstDev& in) {
     return I;
 }
 
+// FP32: the rounded copies kept in the instance record (the same values as the
+// conversions above, loaded as scalars instead of converted per lane)
+template <>
+ATAC_HD InstR<float> make_inst<float>(const InstDev& in) {
+    InstR<float> I;
+    for (int k = 0; k < 9; ++k) { I.R1.a[k] = in.fR1[k]; I.R2.a[k] = in.fR2[k]; }
+    I.w1 = mk3<float>(in.fw1[0], in.fw1[1], in.fw1[2]);
+    I.v1 = mk3<float>(in.fv1[0], in.fv1[1], in.fv1[2]);
+    I.w2 = mk3<float>(in.fw2[0], in.fw2[1], in.fw2[2]);
+    I.v2 = mk3<float>(in.fv2[0], in.fv2[1], in.fv2[2]);
+    I.kn = in.fkn; I.kt = in.fkt; I.mu = in.fmu; I.c = in.fdamping; I.cs = in.fcontact_scale;
+    return I;
+}
+
 // Forward state of one active contact (values of BodySDFObj::collision and of
 // the force loop body, plus what the tangent and adjoint maps reuse).
 template <typename R>
@@ -297,25 +401,29 @@ struct ContactFwd {
     R f[12];            // wrench contribution [tau1 f1 tau2 f2]
 };
 
-// Sample phi at the contact point xi1 (FP32 as stored) of instance `in`.
-// x is formed as the reference does: xw1 = R1 xi1 + p1 (CollisionDetection.cpp:136),
-// x = R2^T (xw1 - p2) (BodySDFObj.cpp:251).
-template <typename R>
-ATAC_HD void contact_sample(const GridDev& G, const InstDev& in, double px, double py, double pz, ContactFwd<R>& s,
-                           const CkptDev* ck = nullptr) {
-    double xi[3] = {px, py, pz};
-    double xw[3], x[3];
+// The contact point in the SDF body frame, formed as the reference does:
+// xw1 = R1 xi1 + p1 (CollisionDetection.cpp:136), x = R2^T (xw1 - p2)
+// (BodySDFObj.cpp:251).
+ATAC_HD void body2_point(const InstDev& in, double px, double py, double pz, double x[3]) {
+    const double xi[3] = {px, py, pz};
+    double xw[3];
     for (int r = 0; r < 3; ++r) xw[r] = in.R1[3 * r] * xi[0] + in.R1[3 * r + 1] * xi[1] + in.R1[3 * r + 2] * xi[2] + in.p1[r];
-    double q0 = xw[0] - in.p2[0], q1 = xw[1] - in.p2[1], q2 = xw[2] - in.p2[2];
+    const double q0 = xw[0] - in.p2[0], q1 = xw[1] - in.p2[1], q2 = xw[2] - in.p2[2];
     for (int r = 0; r < 3; ++r) x[r] = in.R2[r] * q0 + in.R2[3 + r] * q1 + in.R2[6 + r] * q2;
-    s.xi1 = mk3<R>((R)xi[0], (R)xi[1], (R)xi[2]);
+}
+
+// Sample phi at the contact point xi1 (FP32 as stored) of instance `in`.
+template <typename R>
+ATAC_HD void contact_sample(const GridDev& G, const InstDev& in, double px, double py, double pz, ContactFwd<R>& s) {
+    double x[3];
+    body2_point(in, px, py, pz, x);
+    s.xi1 = mk3<R>((R)px, (R)py, (R)pz);
     s.x = mk3<R>((R)x[0], (R)x[1], (R)x[2]);
     int idx[3];
     float f[3];
     double fd[3];
     if (sizeof(R) == sizeof(double) && fast_cell(G, x, idx, f, fd)) {
-        // FP64 contact path: value and partials in FP64 from the same corners (the
-        // FP32 checkpoint is not used; recomputing gives the forward values exactly)
+        // FP64 contact path: value and partials in FP64 from the same corners
         Corners c = load_corners(G, idx[0], idx[1], idx[2]);
         double val, g[3], H[6];
         sdf_cell64(G, c, fd, val, g, H);
@@ -331,11 +439,7 @@ ATAC_HD void contact_sample(const GridDev& G, const InstDev& in, double px, doub
         // the penetration depth of an active contact is taken in FP64 from the
         // same corners: the FP32 interpolant's ~1e-8 absolute error would be
         // ~1e-5 of a shallow contact's depth
-        double dval = cell_value64(c, fd) + in.col_th;
-        if (ck) {  // forward checkpoint: the same values, read back
-            dval = (double)ck->d_hi + (double)ck->d_lo;
-            g[0] = ck->g0; g[1] = ck->g1; g[2] = ck->g2;
-        }
+        const double dval = cell_value64(c, fd) + in.col_th;
         s.d = (R)dval;
         s.g = mk3<R>((R)g[0], (R)g[1], (R)g[2]);
         for (int k = 0; k < 6; ++k) s.H[k] = (R)H[k];
@@ -348,6 +452,82 @@ ATAC_HD void contact_sample(const GridDev& G, const InstDev& in, double px, doub
         for (int k = 0; k < 6; ++k) s.H[k] = (R)H[k];
         s.slow = true;
     }
+}
+
+// ------------------------------------------------------------------ contact records
+// Fast-path record of an active point from its grid coordinates c (detection
+// map) and cell: d in FP64 from the FP32 corners (stored rounded to the record's
+// precision), gradient and mixed partials in the record's precision. The
+// diagonal Hessian of a fast-path sample is exactly 0.
+ATAC_HD void record_fast(const GridDev& G, const InstDev& in, const Corners& cr, const double fd[3], Rec32& r) {
+    const float f[3] = {(float)fd[0], (float)fd[1], (float)fd[2]};
+    float val, g[3], H[6];
+    sdf_cell32(G, cr, f, val, g, H);
+    r.b[0] = (float)(cell_value64(cr, fd) + in.col_th);
+    r.b[1] = g[0]; r.b[2] = g[1]; r.b[3] = g[2];
+    r.c[0] = H[3]; r.c[1] = H[4]; r.c[2] = H[5];
+    r.c[3] = __builtin_bit_cast(float, -1);
+}
+
+ATAC_HD void record_fast(const GridDev& G, const InstDev& in, const Corners& cr, const double fd[3], Rec64& r) {
+    double val, g[3], H[6];
+    sdf_cell64(G, cr, fd, val, g, H);
+    r.d = val + in.col_th;
+    for (int k = 0; k < 3; ++k) r.g[k] = g[k];
+    for (int k = 0; k < 6; ++k) r.H[k] = H[k];
+}
+
+// Slow-path sample (FD stencils operation for operation in FP64) of a record's
+// point; fills the record (and, for FP32 records, the diagonal Hessian entry).
+ATAC_HD void record_slow(const GridDev& G, const InstDev& in, Rec32& r, float* diag) {
+    double x[3], val, g[3], H[6];
+    body2_point(in, (double)r.a[0], (double)r.a[1], (double)r.a[2], x);
+    sdf_stencil64(G, x, val, g, H);
+    r.b[0] = (float)(val + in.col_th);
+    r.b[1] = (float)g[0]; r.b[2] = (float)g[1]; r.b[3] = (float)g[2];
+    r.c[0] = (float)H[3]; r.c[1] = (float)H[4]; r.c[2] = (float)H[5];
+    diag[0] = (float)H[0]; diag[1] = (float)H[1]; diag[2] = (float)H[2]; diag[3] = 0.0f;
+}
+
+ATAC_HD void record_slow(const GridDev& G, const InstDev& in, Rec64& r, float*) {
+    double x[3], val, g[3], H[6];
+    body2_point(in, (double)r.a[0], (double)r.a[1], (double)r.a[2], x);
+    sdf_stencil64(G, x, val, g, H);
+    r.d = val + in.col_th;
+    for (int k = 0; k < 3; ++k) r.g[k] = g[k];
+    for (int k = 0; k < 6; ++k) r.H[k] = H[k];
+}
+
+// Forward state of a record: the contact point, its SDF-frame position (formed
+// from xi1 as the reference does) and the sampled d, grad phi, Hessian.
+ATAC_HD void record_decode(const Rec32& r, const float* slow_diag, const InstDev& in, ContactFwd<float>& s) {
+    double x[3];
+    body2_point(in, (double)r.a[0], (double)r.a[1], (double)r.a[2], x);
+    s.xi1 = mk3<float>(r.a[0], r.a[1], r.a[2]);
+    s.x = mk3<float>((float)x[0], (float)x[1], (float)x[2]);
+    s.d = r.b[0];
+    s.g = mk3<float>(r.b[1], r.b[2], r.b[3]);
+    s.H[3] = r.c[0]; s.H[4] = r.c[1]; s.H[5] = r.c[2];
+    const int slow = __builtin_bit_cast(int, r.c[3]);
+    s.slow = slow >= 0;
+    if (s.slow) {
+        s.H[0] = slow_diag[4 * (int64_t)slow];
+        s.H[1] = slow_diag[4 * (int64_t)slow + 1];
+        s.H[2] = slow_diag[4 * (int64_t)slow + 2];
+    } else {
+        s.H[0] = s.H[1] = s.H[2] = 0.0f;
+    }
+}
+
+ATAC_HD void record_decode(const Rec64& r, const float*, const InstDev& in, ContactFwd<double>& s) {
+    double x[3];
+    body2_point(in, (double)r.a[0], (double)r.a[1], (double)r.a[2], x);
+    s.xi1 = mk3<double>(r.a[0], r.a[1], r.a[2]);
+    s.x = mk3<double>(x[0], x[1], x[2]);
+    s.d = r.d;
+    s.g = mk3<double>(r.g[0], r.g[1], r.g[2]);
+    for (int k = 0; k < 6; ++k) s.H[k] = r.H[k];
+    s.slow = false;
 }
 
 // Values of BodySDFObj::collision (BodySDFObj.cpp:251-273) given s.x, s.d, s.g
@@ -516,7 +696,8 @@ ATAC_HD void contact_jvp(const InstR<R>& I, const ContactFwd<R>& s, const Tangen
 // out[0:6] = gb^T Km[:, q1], out[6:12] = gb^T Km[:, q2], out[12:18] = gb^T Dm[:, phi1],
 // out[18:24] = gb^T Dm[:, phi2], out[24] = gb^T d f / d kn, out[25] = gb^T d f / d kt
 // (kn, kt effective). Hand-written reverse sweep of the tangent map above.
-template <typename R>
+// ACC: add into out instead of assigning.
+template <typename R, bool ACC = false>
 ATAC_HD void contact_vjp(const InstR<R>& I, const ContactFwd<R>& s, const R gb[12], R out[26]) {
     const v3<R> f1a = mk3<R>(gb[0], gb[1], gb[2]), f1l = mk3<R>(gb[3], gb[4], gb[5]);
     const v3<R> f2a = mk3<R>(gb[6], gb[7], gb[8]), f2l = mk3<R>(gb[9], gb[10], gb[11]);
@@ -594,9 +775,15 @@ ATAC_HD void contact_vjp(const InstR<R>& I, const ContactFwd<R>& s, const R gb[1
     om1b += cross(s.xi1, y2);
     v1b += y2;
     const v3<R> o[8] = {om1b, v1b, om2b, v2b, w1pb, v1pb, w2pb, v2pb};
-    for (int k = 0; k < 8; ++k) { out[3 * k] = o[k].x; out[3 * k + 1] = o[k].y; out[3 * k + 2] = o[k].z; }
-    out[24] = knb;
-    out[25] = ktb;
+    if (ACC) {
+        for (int k = 0; k < 8; ++k) { out[3 * k] += o[k].x; out[3 * k + 1] += o[k].y; out[3 * k + 2] += o[k].z; }
+        out[24] += knb;
+        out[25] += ktb;
+    } else {
+        for (int k = 0; k < 8; ++k) { out[3 * k] = o[k].x; out[3 * k + 1] = o[k].y; out[3 * k + 2] = o[k].z; }
+        out[24] = knb;
+        out[25] = ktb;
+    }
 }
 
 // Neumaier-compensated FP32 accumulator

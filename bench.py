@@ -31,8 +31,13 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "contact-point SDF queries/s (force+adjoint) at 1 GPU; % HBM roofline; err vs FP64"
-INST_DEV_BYTES = 512        # sizeof(InstDev)
-TILE_BYTES = 24 + 4 + 4     # TileDev + processing-order entry + tile_active write
+# Algorithmic bytes of one detection launch (see roofline in main):
+DETECT_INST_BYTES = 72 + 24 + 24 + 8 + 24 + 8 + 4   # instance fields detection reads: A, b, cmax, col_th,
+                                                    # point pointers, point count, grid index
+TILE_BYTES = 24 + 4 + 4 + 4                         # TileDev + queue entry + tile_active + tile_chunk0 writes
+RECORD_BYTES = 48                                   # FP32 active-contact record (written once per active point)
+CHUNK_BYTES = 8                                     # work-item entry, one per kChunk records
+K_CHUNK = 256
 
 
 def log(*a):
@@ -121,6 +126,7 @@ def main():
     ap.add_argument("--tile-points", type=int, default=0)
     ap.add_argument("--detect-block", type=int, default=0)
     ap.add_argument("--order", type=int, default=0)
+    ap.add_argument("--points-per-lane", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -136,7 +142,8 @@ def main():
     log(f"workload {w.name}: {len(w.instances)} instances, {w.n_queries} queries, generated in "
         f"{time.perf_counter() - t0:.1f}s")
     t0 = time.perf_counter()
-    tuning = {"tile_points": args.tile_points, "detect_block": args.detect_block, "order": args.order}
+    tuning = {"tile_points": args.tile_points, "detect_block": args.detect_block, "order": args.order,
+              "points_per_lane": args.points_per_lane}
     b = batch_from_workload(w, tuning=tuning)
     info = b.info()
     log(f"batch ready in {time.perf_counter() - t0:.1f}s: {info}")
@@ -172,10 +179,18 @@ def main():
     log(f"stage ms: {stage_ms}")
 
     touched = b.touched_values()
-    det_bytes = 12 * n_q + 4 * touched + INST_DEV_BYTES * b.n + TILE_BYTES * info["n_tiles"] + 4 * n_active
+    # detection: every point read once (12 B), every grid value its stencils touch
+    # once (4 B), the instance fields and tile entries, and one record per active point
+    n_chunks_lb = -(-n_active // K_CHUNK)
+    det_bytes = (12 * n_q + 4 * touched + DETECT_INST_BYTES * b.n + TILE_BYTES * info["n_tiles"]
+                 + RECORD_BYTES * n_active + CHUNK_BYTES * n_chunks_lb)
     det_ms = stage_ms["detect"]
     achieved = det_bytes / (det_ms * 1e-3) / 1e9
-    step_bytes = (det_bytes + 20 * n_active * 2 + 8 * 12 * b.n * 2 + 8 * 26 * b.n)
+    # the step: detection, then the contact and adjoint passes each read the records
+    # once, write one partial row per work item, and the reductions write fm, read
+    # g_fm and write the 26 gradients per instance
+    step_bytes = (det_bytes + 2 * RECORD_BYTES * n_active + 8 * (12 + 26) * n_chunks_lb
+                  + 8 * 12 * b.n * 2 + 8 * 26 * b.n)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
                 "kernel": "detect_kernel", "algorithmic_bytes_per_launch": int(det_bytes),
@@ -214,8 +229,10 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": w.name, "instances": b.n, "queries": n_q, "grids": f"{len(w.grids)} x "
                    f"{'x'.join(str(v) for v in w.grids[0].n)} FP32", "active_fraction": round(n_active / n_q, 5),
-                   "step": "detect + contact wrench + reduce + adjoint (VJP) + reduce, HIP graph replay",
+                   "step": "detect + slow-path samples + contact wrench + reduce + adjoint (VJP) + reduce, "
+                           "HIP graph replay",
                    "tile_points": info["tile_points"], "detect_block": info["detect_block"],
+                   "points_per_lane": info["points_per_lane"],
                    "persistent_blocks": info["persistent_blocks"], "n_cu": n_cu, "parallelism": "none"},
         "roofline": roofline,
         "cpu_baseline": cpu,

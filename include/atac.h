@@ -52,18 +52,20 @@ typedef struct {
 } atac_instance;
 
 typedef struct {
-    int32_t tile_points;   /* points per tile (0: 4096); at most 65536 */
+    int32_t tile_points;   /* points per tile (0: 4096); a multiple of 64, at most 65536 */
     int32_t detect_block;  /* threads per detection workgroup: 128, 256, 512, 1024 (0: 256) */
     int32_t blocks_per_cu; /* persistent workgroups per CU (0: occupancy limit) */
     int32_t order;         /* work queues: 0 one per XCD, items grouped by grid (default);
                               1 one queue in instance order; 2 one queue grouped by grid */
+    int32_t points_per_lane; /* consecutive points a detection lane loads at once: 1, 2, 4 (0: 4) */
 } atac_tuning;
 
 /* flags of atac_batch_eval / atac_eval_plain */
 #define ATAC_EVAL_FORCE 0u       /* fm only: ForceGeneralSDFContact::computeForce */
 #define ATAC_EVAL_DERIV 1u       /* + Km, Dm: computeForceWithDerivative */
 #define ATAC_EVAL_P6 2u          /* + contact-scale column of dfm_dp */
-#define ATAC_EVAL_CHECKPOINT 4u  /* keep (d, grad phi) of active points for atac_batch_vjp */
+#define ATAC_EVAL_CHECKPOINT 4u  /* accepted for compatibility: every evaluation keeps its active-contact
+                                    records (d, grad phi, Hessian) for atac_batch_vjp */
 #define ATAC_EVAL_PRECISE 8u     /* FP64 contact arithmetic (detection stays FP32) */
 #define ATAC_EVAL_ALL_ACTIVE 16u /* every point is a contact, as for the reference's contact-list
                                     overloads (ForceGeneralSDFContact.cpp:147, :221, :449) */
@@ -133,11 +135,12 @@ int atac_batch_info(atac_batch b, int64_t* n_points, int32_t* n_tiles, int32_t* 
  * wrench (and Km, Dm, p6 column) over the active ones. Stream-ordered. */
 int atac_batch_eval(atac_batch b, uint32_t flags, const atac_outputs* out, void* stream);
 
-/* Adjoint of the last checkpointed evaluation: g_fm (device, 12 per instance) ->
- * gradients. Stream-ordered after that evaluation. */
+/* Adjoint of the last evaluation of the batch (at its precision), from the
+ * active-contact records it kept: g_fm (device, 12 per instance) -> gradients.
+ * Stream-ordered after that evaluation. */
 int atac_batch_vjp(atac_batch b, const double* g_fm, const atac_gradients* out, void* stream);
 
-/* Captures [reset, detect, contact, reduce] (and the adjoint when g_fm != NULL)
+/* Captures [reset, detect, slow, contact, reduce] (and the adjoint when g_fm != NULL)
  * into a HIP graph bound to these outputs; atac_batch_graph_launch replays it. */
 int atac_batch_graph_capture(atac_batch b, uint32_t flags, const atac_outputs* out, const double* g_fm,
                              const atac_gradients* grads);
@@ -145,8 +148,8 @@ int atac_batch_graph_launch(atac_batch b, void* stream);
 
 /* Per-stage kernel timing with HIP events recorded on the launch stream around
  * each kernel of atac_batch_eval / atac_batch_vjp (not inside captured graphs).
- * Reads and clears what was recorded since the last call (ms_sum, launches: 5
- * entries each for detect, contact, reduce, vjp, vjp-reduce; may be NULL), then
+ * Reads and clears what was recorded since the last call (ms_sum, launches: 6
+ * entries each for detect, slow, contact, reduce, vjp, vjp-reduce; may be NULL), then
  * enables (enable != 0) or disables recording. Synchronous when reading. */
 int atac_batch_timing(atac_batch b, int32_t enable, double* ms_sum, int32_t* launches);
 

@@ -11,9 +11,18 @@
 
 using namespace atac;
 
-static GridDev host_grid(const float* vals, const int32_t n[3], const double minb[3], const double maxb[3],
-                         const double d[3]) {
+// The grid with its corner-packed copy and block bounds built on the host by the
+// same per-cell / per-block functions the set-up kernels run.
+struct HostGrid {
+    std::vector<float4> cells;
+    std::vector<float2> blk;
     GridDev G;
+};
+
+static void host_grid(const float* vals, const int32_t n[3], const double minb[3], const double maxb[3],
+                      const double d[3], HostGrid& hg) {
+    GridDev& G = hg.G;
+    std::memset(&G, 0, sizeof G);
     G.v = vals;
     for (int a = 0; a < 3; ++a) {
         G.n[a] = n[a];
@@ -21,9 +30,58 @@ static GridDev host_grid(const float* vals, const int32_t n[3], const double min
         G.maxb[a] = maxb[a];
         G.d[a] = d[a];
         G.inv_d[a] = 1.0 / d[a];
+        G.nb[a] = (n[a] - 1) / kBlk + 1;
     }
-    G.pad = 0;
-    return G;
+    if (n[0] >= 2 && n[1] >= 2 && n[2] >= 2) {
+        const int64_t ci = n[0] - 1, cj = n[1] - 1, ck = n[2] - 1;
+        hg.cells.resize(2 * ci * cj * ck);
+        for (int64_t c = 0; c < ci * cj * ck; ++c)
+            pack_cell(G, (int)(c % ci), (int)((c / ci) % cj), (int)(c / (ci * cj)), &hg.cells[2 * c]);
+        const int64_t nb = (int64_t)G.nb[0] * G.nb[1] * G.nb[2];
+        hg.blk.resize(nb);
+        for (int64_t b = 0; b < nb; ++b)
+            hg.blk[b] = block_bounds(G, (int)(b % G.nb[0]), (int)((b / G.nb[0]) % G.nb[1]),
+                                     (int)(b / ((int64_t)G.nb[0] * G.nb[1])));
+        G.cells = hg.cells.data();
+        G.blk = hg.blk.data();
+        G.fast = 1;
+    }
+}
+
+template <typename R> struct RecOfH;
+template <> struct RecOfH<float> { using type = Rec32; };
+template <> struct RecOfH<double> { using type = Rec64; };
+
+// One active point through the batched path's record stages: the detection
+// pass's fast-path record or slow marking, the slow pass, then the decode the
+// contact and adjoint passes start from.
+template <typename R>
+static bool sample_via_record(const GridDev& G, const InstDev& in, int64_t o, ContactFwd<R>& s) {
+    using Rec = typename RecOfH<R>::type;
+    Rec r;
+    std::memset(&r, 0, sizeof r);
+    const float x = in.px[o], y = in.py[o], z = in.pz[o];
+    double cc[3], fd[3];
+    int idx[3];
+    grid_coords(in, x, y, z, cc);
+    r.a[0] = x; r.a[1] = y; r.a[2] = z;
+    const int32_t off = (int32_t)o;
+    std::memcpy(&r.a[3], &off, sizeof off);
+    float diag[4] = {0, 0, 0, 0};
+    bool slow = false;
+    if (fast_cell_c(G, in, cc, idx, fd)) {
+        const Corners cr = G.fast ? load_cell(G, idx[0], idx[1], idx[2]) : load_corners(G, idx[0], idx[1], idx[2]);
+        record_fast(G, in, cr, fd, r);
+    } else {
+        slow = true;
+        record_slow(G, in, r, diag);
+        if constexpr (sizeof(R) == sizeof(float)) {
+            const int32_t si = 0;
+            std::memcpy(&r.c[3], &si, sizeof si);
+        }
+    }
+    record_decode(r, diag, in, s);
+    return slow;
 }
 
 template <typename R>
@@ -37,13 +95,13 @@ static void eval_t(const GridDev& G, const InstDev& in, uint32_t flags, double* 
     if (vjp) std::memset(vjp, 0, 26 * sizeof(double));
     int64_t na = 0;
     for (int64_t o = 0; o < in.pt_count; ++o) {
-        const bool act = detect_value(G, in, in.px[o], in.py[o], in.pz[o]) < 0.0f;
+        const bool act = detect_active(G, in, in.px[o], in.py[o], in.pz[o]);
         if (active) active[o] = act ? 1 : 0;
         if (!act) continue;
         ++na;
         ContactFwd<R> s;
-        contact_sample<R>(G, in, (double)in.px[o], (double)in.py[o], (double)in.pz[o], s);
-        if (slow) slow[o] = s.slow ? 1 : 0;
+        const bool sl = sample_via_record<R>(G, in, o, s);
+        if (slow) slow[o] = sl ? 1 : 0;
         R q[12];
         contact_forward<R>(I, s, q);
         for (int k = 0; k < 12; ++k) fm[k] += (double)s.f[k];
@@ -77,7 +135,9 @@ int hc_eval(const float* vals, const int32_t n[3], const double minb[3], const d
             const float* px, const float* py, const float* pz, int64_t npts, const atac_instance* inst, uint32_t flags,
             double* fm, double* p6, double* Km, double* Dm, int64_t* n_active, uint8_t* active, uint8_t* slow,
             const double* g_fm, double* vjp) {
-    GridDev G = host_grid(vals, n, minb, maxb, d);
+    HostGrid hg;
+    host_grid(vals, n, minb, maxb, d, hg);
+    const GridDev& G = hg.G;
     InstDev in;
     std::memset(&in, 0, sizeof in);
     compose_instance(*inst, G, px, py, pz, npts, in);
@@ -91,11 +151,46 @@ int hc_eval(const float* vals, const int32_t n[3], const double minb[3], const d
 // Detection values of every point (query_SDF + col_th, +inf outside).
 int hc_detect(const float* vals, const int32_t n[3], const double minb[3], const double maxb[3], const double d[3],
               const float* px, const float* py, const float* pz, int64_t npts, const atac_instance* inst, float* out) {
-    GridDev G = host_grid(vals, n, minb, maxb, d);
+    HostGrid hg;
+    host_grid(vals, n, minb, maxb, d, hg);
+    InstDev in;
+    std::memset(&in, 0, sizeof in);
+    compose_instance(*inst, hg.G, px, py, pz, npts, in);
+    for (int64_t o = 0; o < npts; ++o) out[o] = detect_value(hg.G, in, px[o], py[o], pz[o]);
+    return 0;
+}
+
+// The bounded detection decision (block bounds, corner-packed cells) next to the
+// full interpolation's: decided[o] = detect_active, full[o] = detect_value < 0,
+// settled[o] = 1 when the block bound alone settled the point.
+int hc_detect_decisions(const float* vals, const int32_t n[3], const double minb[3], const double maxb[3],
+                        const double d[3], const float* px, const float* py, const float* pz, int64_t npts,
+                        const atac_instance* inst, uint8_t* decided, uint8_t* full, uint8_t* settled) {
+    HostGrid hg;
+    host_grid(vals, n, minb, maxb, d, hg);
+    const GridDev& G = hg.G;
     InstDev in;
     std::memset(&in, 0, sizeof in);
     compose_instance(*inst, G, px, py, pz, npts, in);
-    for (int64_t o = 0; o < npts; ++o) out[o] = detect_value(G, in, px[o], py[o], pz[o]);
+    for (int64_t o = 0; o < npts; ++o) {
+        decided[o] = detect_active(G, in, px[o], py[o], pz[o]) ? 1 : 0;
+        full[o] = detect_value(G, in, px[o], py[o], pz[o]) < 0.0f ? 1 : 0;
+        settled[o] = 0;
+        if (!G.fast) continue;
+        double cc[3];
+        grid_coords(in, px[o], py[o], pz[o], cc);
+        if (!(cc[0] >= 0.0 && cc[0] <= in.cmax[0] && cc[1] >= 0.0 && cc[1] <= in.cmax[1] && cc[2] >= 0.0 &&
+              cc[2] <= in.cmax[2]))
+            continue;
+        int i = (int)cc[0], j = (int)cc[1], k = (int)cc[2];
+        i = i < G.n[0] - 1 ? i : G.n[0] - 1;
+        j = j < G.n[1] - 1 ? j : G.n[1] - 1;
+        k = k < G.n[2] - 1 ? k : G.n[2] - 1;
+        const float cth = (float)in.col_th;
+        const float2 b = G.blk[i / kBlk + G.nb[0] * (j / kBlk + G.nb[1] * (k / kBlk))];
+        const float tol = 4e-6f * (fmaxf(fabsf(b.x), fabsf(b.y)) + fabsf(cth));
+        settled[o] = (b.x + cth > tol || b.y + cth < -tol) ? 1 : 0;
+    }
     return 0;
 }
 }

@@ -135,10 +135,16 @@ def test_vjp_central_fd_peg_in_hole(setup):
     _fd_check(w, ref, 1e-6, 1e-4)
 
 
-def test_vjp_requires_checkpoint(setup):
-    w, ref, b = setup
-    b.eval()
+def test_vjp_requires_an_evaluation(setup):
+    """The adjoint differentiates the last evaluation's records: a batch never
+    evaluated (or re-planned since) has none."""
     import atac
+    w, ref, _ = setup
+    b = batch_from_workload(w)
+    with pytest.raises(atac.AtacError):
+        b.vjp(torch.zeros((b.n, 12), dtype=torch.float64, device="cuda"))
+    b.eval()
+    b.retune({"tile_points": 2048})
     with pytest.raises(atac.AtacError):
         b.vjp(torch.zeros((b.n, 12), dtype=torch.float64, device="cuda"))
 

@@ -135,3 +135,57 @@ def test_slow_path_near_faces_and_box():
     assert h["n_active"] == ref["n_active"]
     assert rel_err(h["fm"], ref["fm"], ref["scale_fm"]) <= TOL
     assert rel_err(h["Km"], ref["Km"], ref["scale_Km"]) <= TOL
+
+
+def _decisions_agree(grid, pts, inst):
+    dec, full, settled = H.detect_decisions(grid, pts, inst)
+    assert np.array_equal(dec, full), f"{int((dec != full).sum())} bounded decisions differ from the full ones"
+    return settled
+
+
+def test_bounded_detection_matches_full_interpolation(peg):
+    """The block-bound shortcut of the detection test decides every point exactly
+    as the full 8-corner FP32 interpolation (on real poses, including points
+    deep inside, outside the box and near the surface)."""
+    w, _ = peg
+    pts = W.round_points_f32(w.point_sets[0]).astype(np.float32)
+    settled = []
+    for ins in w.instances[:16]:
+        settled.append(_decisions_agree(w.grids[0], pts, _ai(ins)).mean())
+    assert 0.0 < np.mean(settled) < 1.0
+
+
+def test_bounded_detection_adversarial():
+    """Fields whose value sits at, or within a few FP32 ulps of, the threshold -col_th
+    (where the bound must defer to the interpolation), non-finite corners, and
+    points on the box faces."""
+    rng = np.random.default_rng(5)
+    col_th = 0.01
+    base = np.float32(-col_th)
+    n = 9
+    zs = np.zeros(6)
+    ins = make_instance(0, 0, 0, np.eye(4), zs, np.eye(4), zs, col_th, 1e4, 0.0, 0.0, 0.0)
+    # points: uniform in the box, on every face and corner, and on the max face exactly
+    lo, hi = -1.0, 1.0
+    p = rng.uniform(lo, hi, (4000, 3))
+    faces = rng.uniform(lo, hi, (600, 3))
+    for a in range(3):
+        faces[a * 200:(a * 200 + 100), a] = lo
+        faces[(a * 200 + 100):(a + 1) * 200, a] = hi
+    pts = np.concatenate([p, faces, np.array([[hi, hi, hi], [lo, lo, lo], [hi, lo, hi]])]).astype(np.float32)
+    dxyz = (hi - lo) / (n - 1)
+    cases = []
+    for k in (-3, -1, 0, 1, 3):       # constant fields a few ulps either side of the threshold
+        cases.append(np.full((n, n, n), base + np.float32(k) * np.spacing(base), dtype=np.float32))
+    noisy = (base + rng.normal(0, 1e-7, (n, n, n))).astype(np.float32)
+    cases.append(noisy)
+    mixed = (base + rng.normal(0, 0.02, (n, n, n))).astype(np.float32)
+    cases.append(mixed)
+    bad = mixed.copy()
+    bad[2, 3, 4] = np.nan
+    bad[5, 5, 5] = np.inf
+    bad[0, 0, 0] = -np.inf
+    cases.append(bad)
+    for vals in cases:
+        g = W.SDFGrid(vals, np.array([lo] * 3), np.array([hi] * 3), np.array([dxyz] * 3))
+        _decisions_agree(g, pts, ins)
