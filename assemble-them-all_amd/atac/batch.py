@@ -185,6 +185,7 @@ class ContactBatch:
                 "order": tu.order, "points_per_lane": tu.points_per_lane}
 
     def retune(self, tuning):
+        self.generation += 1
         tu = self._tuning(tuning)
         check(lib().atac_batch_retune(self.handle, C.byref(tu)))
 
@@ -219,9 +220,12 @@ class ContactBatch:
         return ((L.EVAL_DERIV if deriv else 0) | (L.EVAL_P6 if p6 else 0) | (L.EVAL_CHECKPOINT if checkpoint else 0)
                 | (L.EVAL_PRECISE if precise else 0) | (L.EVAL_ALL_ACTIVE if all_active else 0))
 
+    generation = 0   # evaluations (and re-plans) so far; the adjoint differentiates the last one
+
     def eval(self, deriv=False, p6=False, checkpoint=False, precise=False, all_active=False, out=None):
         out = out if out is not None else self.alloc_outputs(deriv, p6)
         o = self._outputs_struct(out)
+        self.generation += 1
         check(lib().atac_batch_eval(self.handle, self.flags(deriv, p6, checkpoint, precise, all_active), C.byref(o),
                                     _stream()))
         return out
@@ -260,6 +264,7 @@ class ContactBatch:
                                              C.byref(gs) if gs is not None else None))
 
     def graph_launch(self):
+        self.generation += 1
         check(lib().atac_batch_graph_launch(self.handle, _stream()))
 
     def active_ids(self, i):

@@ -89,15 +89,17 @@ struct atac_batch_s {
     int32_t* d_tile_active = nullptr;
     int32_t* d_tile_chunk0 = nullptr;
     ChunkDev* d_chunks = nullptr;
-    int32_t* d_counters = nullptr;    // nq detection queue heads, then CNT_WORDS
+    int32_t* d_counters = nullptr;    // CNT_WORDS
     // per-evaluation buffers, allocated on first use
+    uint32_t* d_act = nullptr;        // n_slots: active point offsets, tile slices
     Rec32* d_rec32 = nullptr;         // n_slots
     Rec64* d_rec64 = nullptr;         // n_slots
-    uint32_t* d_slow_slot = nullptr;  // n_slots
+    SlowDev* d_slow = nullptr;        // n_slots
     float* d_slow_diag = nullptr;     // 4 x n_slots
     double* d_part = nullptr;         // chunk_cap x part_width
     int64_t part_width = 0;
-    int grid_detect = 0, grid_contact[2][3] = {{0, 0, 0}, {0, 0, 0}}, grid_vjp[2] = {0, 0}, grid_slow = 0;
+    int grid_detect = 0, grid_contact[2][3] = {{0, 0, 0}, {0, 0, 0}}, grid_vjp[2] = {0, 0};
+    int grid_slow = 0;
     size_t lds_detect = 0;
     int wave_span = 0;
     bool rec_valid = false;           // records of the last evaluation are in place
@@ -344,7 +346,8 @@ constexpr int kXcds = 8;   // MI355X: 8 XCDs, each with its own L2
 static void release_plan(atac_batch b) {
     dfree(b->d_tiles); dfree(b->d_order); dfree(b->d_tq_begin); dfree(b->d_counters);
     dfree(b->d_tile_active); dfree(b->d_tile_chunk0); dfree(b->d_chunks);
-    dfree(b->d_rec32); dfree(b->d_rec64); dfree(b->d_slow_slot); dfree(b->d_slow_diag); dfree(b->d_part);
+    dfree(b->d_act); dfree(b->d_rec32); dfree(b->d_rec64); dfree(b->d_slow); dfree(b->d_slow_diag);
+    dfree(b->d_part);
     b->part_width = 0;
     b->rec_valid = false;
     if (b->graph_exec) { (void)hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
@@ -362,9 +365,10 @@ static int64_t part_width_of(int mode) {
 static int ensure_buffers(atac_batch b, int mode, bool precise) {
     int rc = ATAC_OK;
     if (b->n_slots > 0) {
-        if (precise && !b->d_rec64) rc = dalloc(&b->d_rec64, (size_t)b->n_slots);
+        if (!b->d_act) rc = dalloc(&b->d_act, (size_t)b->n_slots);
+        if (!rc && precise && !b->d_rec64) rc = dalloc(&b->d_rec64, (size_t)b->n_slots);
         if (!rc && !precise && !b->d_rec32) rc = dalloc(&b->d_rec32, (size_t)b->n_slots);
-        if (!rc && !b->d_slow_slot) rc = dalloc(&b->d_slow_slot, (size_t)b->n_slots);
+        if (!rc && !b->d_slow) rc = dalloc(&b->d_slow, (size_t)b->n_slots);
         if (!rc && !precise && !b->d_slow_diag) rc = dalloc(&b->d_slow_diag, (size_t)b->n_slots * 4);
     }
     const int64_t w = part_width_of(mode);
@@ -391,8 +395,8 @@ static int plan(atac_batch b, const atac_tuning* t) {
     if (tu.tile_points <= 0) tu.tile_points = 4096;
     if (tu.detect_block <= 0) tu.detect_block = 256;
     if (tu.points_per_lane <= 0) tu.points_per_lane = 4;
-    if (tu.tile_points > 65536 || tu.tile_points % 64 != 0)
-        return fail(ATAC_ERR_INVALID, "tile_points must be a multiple of 64 and <= 65536");
+    if (tu.tile_points > 32768 || tu.tile_points % 64 != 0)
+        return fail(ATAC_ERR_INVALID, "tile_points must be a multiple of 64 and <= 32768");
     if (tu.detect_block != 128 && tu.detect_block != 256 && tu.detect_block != 512 && tu.detect_block != 1024)
         return fail(ATAC_ERR_INVALID, "detect_block must be 128, 256, 512 or 1024");
     if (tu.points_per_lane != 1 && tu.points_per_lane != 2 && tu.points_per_lane != 4)
@@ -455,7 +459,7 @@ static int plan(atac_batch b, const atac_tuning* t) {
     int rc = dalloc(&b->d_tiles, b->n_tiles);
     if (!rc) rc = dalloc(&b->d_order, b->n_tiles);
     if (!rc) rc = dalloc(&b->d_tq_begin, b->nq + 1);
-    if (!rc) rc = dalloc(&b->d_counters, b->nq + CNT_WORDS);
+    if (!rc) rc = dalloc(&b->d_counters, CNT_WORDS);
     if (!rc) rc = dalloc(&b->d_tile_active, b->n_tiles);
     if (!rc) rc = dalloc(&b->d_tile_chunk0, b->n_tiles);
     if (!rc) rc = dalloc(&b->d_chunks, (size_t)b->chunk_cap);
@@ -472,10 +476,18 @@ static int plan(atac_batch b, const atac_tuning* t) {
     b->wave_span = span;
     b->lds_detect = lds;
     int nb = 0;
-    ATAC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, detect_kernel_ptr(B, tu.points_per_lane, false), B, lds));
+    ATAC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, detect_kernel_ptr(B, tu.points_per_lane), B, lds));
     if (tu.blocks_per_cu > 0) nb = std::min(nb, tu.blocks_per_cu);
     nb = std::max(nb, 1);
-    b->grid_detect = std::max(1, std::min(b->n_tiles, nb * b->n_cu));
+    // workgroup g serves queue g % nq. Default: one workgroup per tile (the
+    // dispatcher balances the uneven tiles); with blocks_per_cu > 0, that many
+    // persistent workgroups per CU walk their queue's tiles
+    int max_q = 0;
+    for (int q = 0; q < b->nq; ++q) max_q = std::max(max_q, tq[q + 1] - tq[q]);
+    if (tu.blocks_per_cu > 0)
+        b->grid_detect = std::max(b->nq, std::min(b->n_tiles, nb * b->n_cu));
+    else
+        b->grid_detect = std::max(b->nq, b->nq * max_q);
     const int64_t items = std::max<int64_t>(1, (b->chunk_cap + 3) / 4);
     for (int pr = 0; pr < 2; ++pr) {
         for (int m = 0; m < 3; ++m) {
@@ -487,7 +499,7 @@ static int plan(atac_batch b, const atac_tuning* t) {
         ATAC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nv, vjp_kernel_ptr(pr == 1), 256, 0));
         b->grid_vjp[pr] = (int)std::max<int64_t>(1, std::min<int64_t>(items, (int64_t)std::max(nv, 1) * b->n_cu));
     }
-    b->grid_slow = std::max(1, b->n_cu);
+    b->grid_slow = std::max(1, 8 * b->n_cu);
     return ATAC_OK;
 }
 
@@ -583,12 +595,11 @@ static QueueDev tile_queues(atac_batch b) {
     QueueDev q{};
     q.order = b->d_order;
     q.begin = b->d_tq_begin;
-    q.counter = b->d_counters;
     q.nq = b->nq;
     return q;
 }
 
-static int32_t* counters(atac_batch b) { return b->d_counters + b->nq; }
+static int32_t* counters(atac_batch b) { return b->d_counters; }
 
 static int enqueue_eval(atac_batch b, uint32_t flags, const atac_outputs* out, hipStream_t st) {
     if (flags & ~(ATAC_EVAL_DERIV | ATAC_EVAL_P6 | ATAC_EVAL_CHECKPOINT | ATAC_EVAL_PRECISE | ATAC_EVAL_ALL_ACTIVE))
@@ -601,29 +612,30 @@ static int enqueue_eval(atac_batch b, uint32_t flags, const atac_outputs* out, h
     if (rc) return rc;
     if (b->inst.empty()) return ATAC_OK;
     void* rec = precise ? (void*)b->d_rec64 : (void*)b->d_rec32;
-    ATAC_HIP(launch_reset(b->d_counters, b->nq + CNT_WORDS, st));
+    ATAC_HIP(launch_reset(b->d_counters, CNT_WORDS, st));
     if (b->n_tiles > 0) {
         DetectArgs da{};
         da.q = tile_queues(b);
         da.tiles = b->d_tiles; da.wave_span = b->wave_span; da.inst = b->d_inst; da.grids = b->d_grids;
         da.all_active = (flags & ATAC_EVAL_ALL_ACTIVE) ? 1 : 0;
-        da.rec = rec; da.tile_active = b->d_tile_active; da.tile_chunk0 = b->d_tile_chunk0; da.chunks = b->d_chunks;
-        da.slow_slot = b->d_slow_slot; da.counters = counters(b);
+        da.act = b->d_act; da.tile_active = b->d_tile_active; da.tile_chunk0 = b->d_tile_chunk0; da.chunks = b->d_chunks;
+        da.slow = b->d_slow; da.counters = counters(b);
         hipEvent_t t0 = stage_begin(b, st);
-        ATAC_HIP(launch_detect(da, b->tune.detect_block, b->tune.points_per_lane, precise, b->grid_detect, b->lds_detect, st));
+        ATAC_HIP(launch_detect(da, b->tune.detect_block, b->tune.points_per_lane, b->grid_detect, b->lds_detect, st));
         stage_end(b, ST_DETECT, t0, st);
         SlowArgs sa{};
-        sa.inst = b->d_inst; sa.grids = b->d_grids; sa.rec = rec; sa.slow_diag = b->d_slow_diag;
-        sa.slow_slot = b->d_slow_slot; sa.counters = counters(b);
-        hipEvent_t t1 = stage_begin(b, st);
-        ATAC_HIP(launch_slow(sa, precise, b->grid_slow, st));
-        stage_end(b, ST_SLOW, t1, st);
-        ContactArgs ca{};
-        ca.chunks = b->d_chunks; ca.tiles = b->d_tiles; ca.inst = b->d_inst; ca.rec = rec; ca.slow_diag = b->d_slow_diag;
-        ca.tile_active = b->d_tile_active; ca.counters = counters(b); ca.partial = b->d_part;
+        sa.inst = b->d_inst; sa.grids = b->d_grids; sa.act = b->d_act; sa.rec = rec; sa.slow_diag = b->d_slow_diag;
+        sa.slow = b->d_slow; sa.counters = counters(b);
         hipEvent_t t2 = stage_begin(b, st);
+        ATAC_HIP(launch_slow(sa, precise, b->grid_slow, st));
+        stage_end(b, ST_SLOW, t2, st);
+        ContactArgs ca{};
+        ca.chunks = b->d_chunks; ca.inst = b->d_inst; ca.grids = b->d_grids; ca.act = b->d_act; ca.rec = rec;
+        ca.slow_diag = b->d_slow_diag;
+        ca.counters = counters(b); ca.partial = b->d_part;
+        hipEvent_t t3 = stage_begin(b, st);
         ATAC_HIP(launch_contact(ca, mode, precise, b->grid_contact[precise ? 1 : 0][mode], st));
-        stage_end(b, ST_CONTACT, t2, st);
+        stage_end(b, ST_CONTACT, t3, st);
     }
     ReduceArgs ra{};
     ra.inst = b->d_inst; ra.n_inst = (int32_t)b->inst.size(); ra.nacc = (int32_t)part_width_of(mode);
@@ -649,12 +661,11 @@ static int enqueue_vjp(atac_batch b, const double* g_fm, const atac_gradients* g
     if (rc) return rc;
     if (b->inst.empty()) return ATAC_OK;
     const bool precise = b->rec_precise;
-    ATAC_HIP(launch_reset(counters(b) + CNT_VJP_HEAD, 1, st));
     if (b->n_tiles > 0) {
         VjpArgs va{};
-        va.chunks = b->d_chunks; va.tiles = b->d_tiles; va.inst = b->d_inst;
+        va.chunks = b->d_chunks; va.inst = b->d_inst;
         va.rec = precise ? (const void*)b->d_rec64 : (const void*)b->d_rec32; va.slow_diag = b->d_slow_diag;
-        va.tile_active = b->d_tile_active; va.counters = counters(b); va.g_fm = g_fm; va.partial = b->d_part;
+        va.counters = counters(b); va.g_fm = g_fm; va.partial = b->d_part;
         hipEvent_t t0 = stage_begin(b, st);
         ATAC_HIP(launch_vjp(va, precise, b->grid_vjp[precise ? 1 : 0], st));
         stage_end(b, ST_VJP, t0, st);
@@ -727,21 +738,13 @@ int atac_batch_active_ids(atac_batch b, int32_t inst, int64_t* ids, int64_t capa
         ATAC_HIP(hipMemcpy(ta.data(), b->d_tile_active + d.tile_begin, sizeof(int32_t) * d.tile_count, hipMemcpyDeviceToHost));
     const atac_points_s& P = *b->pts[b->inst[inst].points];
     const int64_t seg_begin = P.seg[b->inst[inst].segment];
-    // every record starts with (xi1, bits(point offset within the instance))
-    const size_t stride = b->rec_precise ? sizeof(Rec64) : sizeof(Rec32);
-    const char* base = b->rec_precise ? (const char*)b->d_rec64 : (const char*)b->d_rec32;
     std::vector<int64_t> out;
     for (int k = 0; k < d.tile_count; ++k) {
         const TileDev& t = b->tiles[d.tile_begin + k];
         if (!ta[k]) continue;
-        std::vector<float> a4((size_t)ta[k] * 4);
-        ATAC_HIP(hipMemcpy2D(a4.data(), 4 * sizeof(float), base + (size_t)t.slot * stride, stride, 4 * sizeof(float),
-                             (size_t)ta[k], hipMemcpyDeviceToHost));
-        for (int j = 0; j < ta[k]; ++j) {
-            int32_t off;
-            std::memcpy(&off, &a4[4 * (size_t)j + 3], sizeof off);
-            out.push_back(P.perm[seg_begin + off]);
-        }
+        std::vector<uint32_t> off(ta[k]);
+        ATAC_HIP(hipMemcpy(off.data(), b->d_act + t.slot, sizeof(uint32_t) * ta[k], hipMemcpyDeviceToHost));
+        for (uint32_t o : off) out.push_back(P.perm[seg_begin + (o & ~kActSlow)]);
     }
     std::sort(out.begin(), out.end());
     *count = (int64_t)out.size();

@@ -5,11 +5,10 @@
 
 namespace atac {
 
-// Work queues: items order[begin[q] .. begin[q+1]) belong to queue q.
+// Tile queues: tiles order[begin[q] .. begin[q+1]) belong to queue q.
 struct QueueDev {
     const int32_t* order;
     const int32_t* begin;   // nq + 1 entries
-    int32_t* counter;       // nq heads (reset before each launch)
     int32_t nq;
     int32_t pad;
 };
@@ -21,41 +20,40 @@ struct DetectArgs {
     int32_t all_active;         // every point is a contact (fixed contact lists)
     const InstDev* inst;
     const GridDev* grids;
-    void* rec;                  // Rec32 or Rec64 per slot
+    uint32_t* act;              // per slot: offset of the active point within its instance (| kActSlow)
     int32_t* tile_active;       // n_tiles
     int32_t* tile_chunk0;       // n_tiles: first work item of the tile
     ChunkDev* chunks;           // work items, pushed per tile
-    uint32_t* slow_slot;        // slow list: record slots
+    SlowDev* slow;              // slow list
     int32_t* counters;          // CNT_*
 };
 
 struct SlowArgs {
     const InstDev* inst;
     const GridDev* grids;
-    void* rec;
+    const uint32_t* act;
+    void* rec;                  // Rec32 or Rec64 per slot
     float* slow_diag;           // 4 per slow contact (FP32 records)
-    const uint32_t* slow_slot;
+    const SlowDev* slow;
     const int32_t* counters;
 };
 
 struct ContactArgs {
     const ChunkDev* chunks;
-    const TileDev* tiles;
     const InstDev* inst;
-    const void* rec;
+    const GridDev* grids;
+    const uint32_t* act;
+    void* rec;                  // fast-path records are written here, slow ones read
     const float* slow_diag;
-    const int32_t* tile_active;
     int32_t* counters;
     double* partial;            // per work item: fm 12 | p6 12 | Km 144 | Dm 144 (as the mode has them)
 };
 
 struct VjpArgs {
     const ChunkDev* chunks;
-    const TileDev* tiles;
     const InstDev* inst;
     const void* rec;
     const float* slow_diag;
-    const int32_t* tile_active;
     int32_t* counters;
     const double* g_fm;         // n_inst x 12
     double* partial;            // per work item: kVjpOut
@@ -108,8 +106,8 @@ struct TouchedArgs {
     const int64_t* bit_offset;  // per grid, in 32-bit words
 };
 
-hipError_t launch_detect(const DetectArgs& a, int block, int ppl, bool precise, int grid, size_t lds, hipStream_t st);
-const void* detect_kernel_ptr(int block, int ppl, bool precise);
+hipError_t launch_detect(const DetectArgs& a, int block, int ppl, int grid, size_t lds, hipStream_t st);
+const void* detect_kernel_ptr(int block, int ppl);
 hipError_t launch_slow(const SlowArgs& a, bool precise, int grid, hipStream_t st);
 hipError_t launch_contact(const ContactArgs& a, int mode, bool precise, int grid, hipStream_t st);
 const void* contact_kernel_ptr(int mode, bool precise);

@@ -80,22 +80,22 @@ struct LaneAcc<double> {
     __device__ double value() const { return s; }
 };
 
-// ------------------------------------------------------------------ work queues
-// Items (tiles or instances) are split into nq contiguous queues; workgroup b
-// starts on queue b % nq (the XCD it was dispatched to, with round-robin
-// dispatch over the XCDs) and moves to the next queue when its own is drained.
-// Called by one thread; qid/tried persist in that thread across calls. Returns
-// -1 when every queue is empty, so every workgroup's loop ends.
-__device__ __forceinline__ int next_item(const QueueDev& Q, int& qid, int& tried) {
-    while (tried < Q.nq) {
-        const int s = atomicAdd(Q.counter + qid, 1);
-        const int b = Q.begin[qid], len = Q.begin[qid + 1] - b;
-        if (s < len) return Q.order[b + s];
-        qid = (qid + 1 == Q.nq) ? 0 : qid + 1;
-        ++tried;
+// ------------------------------------------------------------------ tile assignment
+// Tiles are split into nq contiguous queues of balanced point counts. Workgroup
+// b belongs to queue b % nq (the XCD it is dispatched to, with round-robin
+// dispatch over the XCDs, so a queue's tiles share that XCD's L2) and takes the
+// queue's tiles r, r + m, r + 2m, ... where r = b / nq is its rank among the m
+// workgroups of the queue. The assignment is static: no shared counter, and
+// every workgroup's tile sequence is known from the start.
+struct TileWalk {
+    int k, end, step;
+    __device__ TileWalk(const QueueDev& Q) {
+        const int nq = Q.nq, q = (int)blockIdx.x % nq, r = (int)blockIdx.x / nq;
+        step = ((int)gridDim.x - q + nq - 1) / nq;
+        k = as_const(Q.begin)[q] + r;
+        end = as_const(Q.begin)[q + 1];
     }
-    return -1;
-}
+};
 
 // ------------------------------------------------------------------ records
 template <typename R> struct RecOf;
@@ -120,77 +120,85 @@ __device__ __forceinline__ Rec32 load_rec(const Rec32* p) {
 __device__ __forceinline__ void store_rec(Rec64* p, const Rec64& r) { *p = r; }
 __device__ __forceinline__ Rec64 load_rec(const Rec64* p) { return *p; }
 
-// Marks a record as waiting for the slow path: its instance is parked in the
-// depth field until the slow pass fills the sample in.
-__device__ __forceinline__ void mark_slow(Rec32& r, int inst, int slow) {
-    r.b[0] = __int_as_float(inst);
-    r.c[3] = __int_as_float(slow);
-}
-__device__ __forceinline__ void mark_slow(Rec64& r, int inst, int) { r.d = (double)inst; }
-__device__ __forceinline__ int parked_inst(const Rec32& r) { return __float_as_int(r.b[0]); }
-__device__ __forceinline__ int parked_inst(const Rec64& r) { return (int)r.d; }
+// A slow-path FP32 record points at its diagonal Hessian entries in slow_diag.
+__device__ __forceinline__ void mark_slow_index(Rec32& r, int slow) { r.c[3] = __int_as_float(slow); }
+__device__ __forceinline__ void mark_slow_index(Rec64&, int) {}
 
 // ------------------------------------------------------------------ batched detection
-// Points of one tile per work item. Phase 1 streams the tile (PPL consecutive
-// points per lane, vector loads, each wavefront a contiguous span of the tile)
-// and compacts the offsets of the points with d < 0 into per-wave LDS segments
+// PPL consecutive points from offset o (nontemporal vector loads); nothing is
+// read when o >= end (o is a multiple of PPL and the point arrays are padded to
+// a multiple of 4).
+template <int PPL, typename P>
+__device__ __forceinline__ void load_points(P px, P py, P pz, int o, int end, float (&x)[PPL], float (&y)[PPL],
+                                            float (&z)[PPL]) {
+    if (o >= end) {
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) x[k] = y[k] = z[k] = 0.0f;
+        return;
+    }
+    if constexpr (PPL == 4) {
+        const f4v vx = __builtin_nontemporal_load(reinterpret_cast<global_ptr<f4v>>(px + o));
+        const f4v vy = __builtin_nontemporal_load(reinterpret_cast<global_ptr<f4v>>(py + o));
+        const f4v vz = __builtin_nontemporal_load(reinterpret_cast<global_ptr<f4v>>(pz + o));
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) { x[k] = vx[k]; y[k] = vy[k]; z[k] = vz[k]; }
+    } else if constexpr (PPL == 2) {
+        const f2v vx = __builtin_nontemporal_load(reinterpret_cast<global_ptr<f2v>>(px + o));
+        const f2v vy = __builtin_nontemporal_load(reinterpret_cast<global_ptr<f2v>>(py + o));
+        const f2v vz = __builtin_nontemporal_load(reinterpret_cast<global_ptr<f2v>>(pz + o));
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) { x[k] = vx[k]; y[k] = vy[k]; z[k] = vz[k]; }
+    } else {
+        x[0] = __builtin_nontemporal_load(px + o);
+        y[0] = __builtin_nontemporal_load(py + o);
+        z[0] = __builtin_nontemporal_load(pz + o);
+    }
+}
+
+// Points of one tile per work item. The tile is streamed (PPL consecutive
+// points per lane, vector loads, each wavefront a contiguous span of the tile);
+// the offsets of the points with d < 0 are compacted into per-wave LDS segments
 // in ascending order (a wave-wide exclusive prefix of the per-lane counts built
-// from ballots of their bits). Phase 2 walks the compacted list with every lane
-// busy and writes each active point's record: the fast-path sample from its
-// 8 corners, or the point queued for the slow pass. The tile then pushes its
-// work items for the contact and adjoint passes.
-template <int BLOCK, int PPL, bool PREC>
+// from ballots of their bits), then copied to the tile's slots of the active
+// list, and the tile pushes its work items for the record, contact and adjoint
+// passes. Only the detection arithmetic lives here, so the kernel stays small
+// enough for a high occupancy on this latency-bound stream.
+template <int BLOCK, int PPL>
 __global__ __launch_bounds__(BLOCK) void detect_kernel(DetectArgs a) {
-    using Rec = typename RecOf<typename std::conditional<PREC, double, float>::type>::type;
     constexpr int NW = BLOCK / 64;
     constexpr int NB = PPL == 1 ? 1 : (PPL == 2 ? 2 : 3);   // bits of a per-lane count <= PPL
-    extern __shared__ uint16_t s_queue[];   // tile offsets (tiles hold at most 65536 points)
-    __shared__ int s_item;
+    extern __shared__ uint16_t s_queue[];   // tile offsets of the active points
     __shared__ int s_wcount[NW];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int span = a.wave_span;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
-    int qid = blockIdx.x % a.q.nq, tried = 0;
-    for (;;) {
-        if (tid == 0) s_item = next_item(a.q, qid, tried);
-        __syncthreads();
-        const int tile = s_item;
-        if (tile < 0) break;
-        const TileDev t = a.tiles[tile];
-        const InstDev& in = a.inst[t.inst];
-        const GridDev G = a.grids[in.grid];
-        const float* px = in.px + t.begin;
-        const float* py = in.py + t.begin;
-        const float* pz = in.pz + t.begin;
+    const auto order = as_const(a.q.order);
+    const TileWalk walk(a.q);
+    for (int kk = walk.k; kk < walk.end; kk += walk.step) {
+        const int tile = order[kk];
+        const TileDev t = as_const(a.tiles)[tile];
+        const InstDev in = as_const(a.inst)[t.inst];
+        const GridDev G = as_const(a.grids)[in.grid];
+        const auto px = as_global(in.px) + t.begin;
+        const auto py = as_global(in.py) + t.begin;
+        const auto pz = as_global(in.pz) + t.begin;
         uint16_t* q = s_queue + w * span;
         const int wb = w * span, we = min(t.count, wb + span);
         int wcount = 0;
+        // points of the next iteration are loaded while this one is tested
+        float nx[PPL], ny[PPL], nz[PPL];
+        load_points<PPL>(px, py, pz, wb + lane * PPL, we, nx, ny, nz);
         for (int o0 = wb; o0 < we; o0 += 64 * PPL) {
             const int o = o0 + lane * PPL;
-            unsigned m = 0;
-            if (o < we) {
-                float x[PPL], y[PPL], z[PPL];
-                if constexpr (PPL == 4) {
-                    const f4v vx = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(px + o));
-                    const f4v vy = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(py + o));
-                    const f4v vz = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(pz + o));
+            float x[PPL], y[PPL], z[PPL];
 #pragma unroll
-                    for (int k = 0; k < PPL; ++k) { x[k] = vx[k]; y[k] = vy[k]; z[k] = vz[k]; }
-                } else if constexpr (PPL == 2) {
-                    const f2v vx = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(px + o));
-                    const f2v vy = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(py + o));
-                    const f2v vz = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(pz + o));
+            for (int k = 0; k < PPL; ++k) { x[k] = nx[k]; y[k] = ny[k]; z[k] = nz[k]; }
+            load_points<PPL>(px, py, pz, o + 64 * PPL, we, nx, ny, nz);
+            unsigned valid = 0;
 #pragma unroll
-                    for (int k = 0; k < PPL; ++k) { x[k] = vx[k]; y[k] = vy[k]; z[k] = vz[k]; }
-                } else {
-                    x[0] = __builtin_nontemporal_load(px + o);
-                    y[0] = __builtin_nontemporal_load(py + o);
-                    z[0] = __builtin_nontemporal_load(pz + o);
-                }
-#pragma unroll
-                for (int k = 0; k < PPL; ++k)
-                    if (o + k < we && (a.all_active || detect_active(G, in, x[k], y[k], z[k]))) m |= 1u << k;
-            }
+            for (int k = 0; k < PPL; ++k)
+                if (o + k < we) valid |= 1u << k;
+            const unsigned m = detect_points<PPL>(G, in, x, y, z, valid, a.all_active != 0);
             const int c = __popc(m);
             int pre = 0, tot = 0;
 #pragma unroll
@@ -210,26 +218,21 @@ __global__ __launch_bounds__(BLOCK) void detect_kernel(DetectArgs a) {
         int cnt = 0;
 #pragma unroll
         for (int k = 0; k < NW; ++k) cnt += s_wcount[k];
-        Rec* rec = static_cast<Rec*>(a.rec) + t.slot;
+        uint32_t* act = a.act + t.slot;
+        const uint32_t base = (uint32_t)t.begin;
         for (int qi = tid; qi < cnt; qi += BLOCK) {
             int rem = qi, ww = 0;
             while (rem >= s_wcount[ww]) { rem -= s_wcount[ww]; ++ww; }
-            const int off = s_queue[ww * span + rem];
-            const float x = px[off], y = py[off], z = pz[off];
-            double cc[3], fd[3];
-            int idx[3];
-            grid_coords(in, x, y, z, cc);
-            Rec r;
-            r.a[0] = x; r.a[1] = y; r.a[2] = z; r.a[3] = __int_as_float(off + (int)(t.begin));
-            if (fast_cell_c(G, in, cc, idx, fd)) {
-                const Corners cr = G.fast ? load_cell(G, idx[0], idx[1], idx[2]) : load_corners(G, idx[0], idx[1], idx[2]);
-                record_fast(G, in, cr, fd, r);
-            } else {
+            const uint32_t off = base + s_queue[ww * span + rem];
+            // the slow-path test of an active point (its FD stencil leaves the cell)
+            double cc[3];
+            grid_coords(in, px[off - base], py[off - base], pz[off - base], cc);
+            const bool slow = !fast_cell_test(G, in, cc);
+            act[qi] = off | (slow ? kActSlow : 0u);
+            if (slow) {
                 const int si = atomicAdd(a.counters + CNT_SLOW, 1);
-                a.slow_slot[si] = (uint32_t)(t.slot + qi);
-                mark_slow(r, t.inst, si);
+                a.slow[si] = SlowDev{(uint32_t)(t.slot + qi), t.inst};
             }
-            store_rec(rec + qi, r);
         }
         if (tid == 0) {
             a.tile_active[tile] = cnt;
@@ -237,38 +240,63 @@ __global__ __launch_bounds__(BLOCK) void detect_kernel(DetectArgs a) {
                 const int nch = (cnt + kChunk - 1) / kChunk;
                 const int c0 = atomicAdd(a.counters + CNT_CHUNKS, nch);
                 a.tile_chunk0[tile] = c0;
-                for (int k = 0; k < nch; ++k) a.chunks[c0 + k] = ChunkDev{tile, k};
+                for (int k = 0; k < nch; ++k)
+                    a.chunks[c0 + k] = ChunkDev{t.inst, min(kChunk, cnt - k * kChunk), t.slot + (int64_t)k * kChunk};
             }
         }
         __syncthreads();
     }
 }
 
-// Slow-path samples of the queued records (a few per thousand contacts).
+// Wavefront-granular passes take work items in a fixed stride: wavefront g of
+// the launch takes items g, g + G, g + 2G, ... (G = wavefronts in the launch),
+// the items being near-equal (at most kChunk records). No shared counter is
+// touched, and the item index is wave-uniform (readfirstlane), so every branch
+// on it is uniform.
+__device__ __forceinline__ int wave_id() {
+    return (int)blockIdx.x * (int)(blockDim.x >> 6) + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
+__device__ __forceinline__ int wave_count() { return (int)gridDim.x * (int)(blockDim.x >> 6); }
+
+// Records of the slow-path contacts (a few per thousand): the 20 FD-stencil
+// evaluations of a point run on 20 lanes at once (three points per wavefront),
+// and one lane per point combines them and writes the record.
 template <bool PREC>
 __global__ __launch_bounds__(64) void slow_kernel(SlowArgs a) {
     using Rec = typename RecOf<typename std::conditional<PREC, double, float>::type>::type;
+    constexpr int GROUPS = 64 / kStencil;
+    __shared__ double s_v[GROUPS][kStencil];
+    const int lane = threadIdx.x, grp = lane / kStencil, st = lane % kStencil;
     const int n = a.counters[CNT_SLOW];
-    for (int i = blockIdx.x * 64 + threadIdx.x; i < n; i += gridDim.x * 64) {
-        Rec* p = static_cast<Rec*>(a.rec) + a.slow_slot[i];
-        Rec r = load_rec(p);
-        const InstDev& in = a.inst[parked_inst(r)];
-        record_slow(a.grids[in.grid], in, r, a.slow_diag + 4 * (int64_t)i);
-        store_rec(p, r);
+    for (int base = (int)blockIdx.x * GROUPS; base < n; base += (int)gridDim.x * GROUPS) {
+        const int i = base + grp;
+        const bool mine = grp < GROUPS && i < n;
+        SlowDev sd{0u, 0};
+        Rec r;
+        if (mine) {
+            sd = a.slow[i];
+            const InstDev& in = a.inst[sd.inst];
+            const uint32_t off = a.act[sd.slot] & ~kActSlow;
+            r.a[0] = in.px[off]; r.a[1] = in.py[off]; r.a[2] = in.pz[off]; r.a[3] = __uint_as_float(off);
+            double x[3], q[3];
+            bool accurate;
+            body2_point(in, (double)r.a[0], (double)r.a[1], (double)r.a[2], x);
+            stencil_point(x, st, q, accurate);
+            s_v[grp][st] = query_sdf64(a.grids[in.grid], q[0], q[1], q[2], accurate);
+        }
+        __syncthreads();
+        if (mine && st == 0) {
+            double v[kStencil], val, g[3], H[6];
+#pragma unroll
+            for (int k = 0; k < kStencil; ++k) v[k] = s_v[grp][k];
+            stencil_combine(v, val, g, H);
+            float* diag = a.slow_diag ? a.slow_diag + 4 * (int64_t)i : nullptr;
+            record_slow_fill(a.inst[sd.inst], val, g, H, r, diag);
+            mark_slow_index(r, i);
+            store_rec(static_cast<Rec*>(a.rec) + sd.slot, r);
+        }
+        __syncthreads();
     }
-}
-
-// Next work item of a wavefront-granular pass, called with every lane active.
-// Every lane issues the atomic (lane 0 adds 1, the others 0; the compiler
-// folds them into one atomic per wavefront) and the index comes back
-// wave-uniform through readfirstlane: no lane-0-only branch feeds the loop
-// condition, so the pass loop is uniform and the lanes of a wavefront cannot be
-// split across its iterations (a lane-0-only fetch let the compiler retire lane
-// 0 from the loop while the other lanes kept iterating on a stale index).
-__device__ __forceinline__ int next_chunk(int32_t* head) {
-    const int lane = threadIdx.x & 63;
-    const int c = atomicAdd(head, lane == 0 ? 1 : 0);
-    return __builtin_amdgcn_readfirstlane(c);
 }
 
 // Writes n (<= 64) wave-uniform values to dst[0..n) with one store per lane.
@@ -293,29 +321,56 @@ __global__ __launch_bounds__(256) void contact_kernel(ContactArgs a) {
     constexpr bool DERIV = MODE >= 2;
     constexpr int NF = P6 ? 24 : 12;
     constexpr int NACC = NF + (DERIV ? 288 : 0);
+    constexpr int U = kChunk / 64;
     __shared__ double s_km[DERIV ? 4 : 1][DERIV ? 288 : 1];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int total = a.counters[CNT_CHUNKS];
-    for (;;) {
-        const int c = next_chunk(a.counters + CNT_CONTACT_HEAD);
-        if (c >= total) break;
-        const ChunkDev ch = a.chunks[c];
-        const TileDev t = a.tiles[ch.tile];
-        const InstDev& in = a.inst[t.inst];
-        const int n = min(kChunk, a.tile_active[ch.tile] - ch.k * kChunk);
-        const Rec* rec = static_cast<const Rec*>(a.rec) + t.slot + (int64_t)ch.k * kChunk;
+    for (int c = wave_id(); c < total; c += wave_count()) {
+        const ChunkDev ch = as_const(a.chunks)[c];
+        const InstDev in = as_const(a.inst)[ch.inst];
+        const GridDev G = as_const(a.grids)[in.grid];
+        const int n = ch.n;
+        Rec* rec = static_cast<Rec*>(a.rec) + ch.rec0;
+        const auto act = as_global(a.act) + ch.rec0;
         const InstR<R> I = make_inst<R>(in);
         LaneAcc<R> acc[NF];
 #pragma unroll
         for (int k = 0; k < NF; ++k) acc[k].zero();
         if (DERIV)
             for (int k = lane; k < 288; k += 64) s_km[DERIV ? w : 0][DERIV ? k : 0] = 0.0;
-        for (int j0 = 0; j0 < n; j0 += 64) {
-            const int j = j0 + lane;
+        // this lane's active-list entries and points, all loaded first
+        uint32_t e[U];
+        float qx[U], qy[U], qz[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) e[u] = (u * 64 + lane < n) ? act[u * 64 + lane] : kActSlow;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t off = e[u] & ~kActSlow;
+            qx[u] = as_global(in.px)[off];
+            qy[u] = as_global(in.py)[off];
+            qz[u] = as_global(in.pz)[off];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = u * 64 + lane;
             const bool have = j < n;
             ContactFwd<R> s;
             if (have) {
-                record_decode(load_rec(rec + j), a.slow_diag, in, s);
+                Rec r;
+                if (e[u] & kActSlow) {
+                    r = load_rec(rec + j);   // written by the slow pass
+                } else {
+                    // fast-path record: the sample from the packed cell, kept for the adjoint
+                    double cc[3], fd[3];
+                    int idx[3];
+                    grid_coords(in, qx[u], qy[u], qz[u], cc);
+                    cell_of_c(G, cc, idx, fd);
+                    const Corners cr = G.fast ? load_cell(G, idx[0], idx[1], idx[2]) : load_corners(G, idx[0], idx[1], idx[2]);
+                    r.a[0] = qx[u]; r.a[1] = qy[u]; r.a[2] = qz[u]; r.a[3] = __uint_as_float(e[u]);
+                    record_fast(G, in, cr, fd, r);
+                    store_rec(rec + j, r);
+                }
+                record_decode(r, a.slow_diag, in, s);
                 R p6[12];
                 contact_forward<R>(I, s, P6 ? p6 : nullptr);
 #pragma unroll
@@ -364,26 +419,29 @@ __global__ __launch_bounds__(256) void vjp_kernel(VjpArgs a) {
     using Rec = typename RecOf<R>::type;
     const int lane = threadIdx.x & 63;
     const int total = a.counters[CNT_CHUNKS];
-    for (;;) {
-        const int c = next_chunk(a.counters + CNT_VJP_HEAD);
-        if (c >= total) break;
-        const ChunkDev ch = a.chunks[c];
-        const TileDev t = a.tiles[ch.tile];
-        const InstDev& in = a.inst[t.inst];
-        const int n = min(kChunk, a.tile_active[ch.tile] - ch.k * kChunk);
-        const Rec* rec = static_cast<const Rec*>(a.rec) + t.slot + (int64_t)ch.k * kChunk;
+    for (int c = wave_id(); c < total; c += wave_count()) {
+        const ChunkDev ch = as_const(a.chunks)[c];
+        const InstDev in = as_const(a.inst)[ch.inst];
+        const int n = ch.n;
+        const Rec* rec = static_cast<const Rec*>(a.rec) + ch.rec0;
         const InstR<R> I = make_inst<R>(in);
         R gb[12];
 #pragma unroll
-        for (int k = 0; k < 12; ++k) gb[k] = (R)a.g_fm[(int64_t)t.inst * 12 + k];
+        for (int k = 0; k < 12; ++k) gb[k] = (R)a.g_fm[(int64_t)ch.inst * 12 + k];
         R acc[kVjpOut];
 #pragma unroll
         for (int k = 0; k < kVjpOut; ++k) acc[k] = (R)0;
+        // the next iteration's record is loaded while this one is computed
+        Rec cur;
+        if (lane < n) cur = load_rec(rec + lane);
         for (int j = lane; j < n; j += 64) {
+            Rec nxt;
+            if (j + 64 < n) nxt = load_rec(rec + j + 64);
             ContactFwd<R> s;
-            record_decode(load_rec(rec + j), a.slow_diag, in, s);
+            record_decode(cur, a.slow_diag, in, s);
             contact_forward<R>(I, s, nullptr);
             contact_vjp<R, true>(I, s, gb, acc);
+            cur = nxt;
         }
         double* dst = a.partial + (int64_t)c * kVjpOut;
         double row[kVjpOut];
@@ -396,32 +454,52 @@ __global__ __launch_bounds__(256) void vjp_kernel(VjpArgs a) {
 }
 
 // ------------------------------------------------------------------ per-instance reduction
-// One wavefront per instance: the partial rows of its tiles' work items, in
-// tile order then item order; every instance's outputs are written (zeros when
+// One wavefront per instance. The lanes first load the instance's tile entries
+// (active count, first work item) 64 tiles at a time; then, lanes over
+// components, the work items' partial rows are summed in tile order and each
+// tile's items in order, the tile entries broadcast from their lanes, so every
+// row load's address is known up front. The order depends only on the tiling:
+// the same result on every run. Every instance's outputs are written (zeros when
 // it has no active contact).
 __global__ __launch_bounds__(256) void reduce_kernel(ReduceArgs a) {
     const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int i = (int)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     if (i >= a.n_inst) return;
-    const InstDev& in = a.inst[i];
-    const int tb = in.tile_begin, te = in.tile_begin + in.tile_count;
-    for (int comp = lane; comp < a.nacc; comp += 64) {
+    const InstDev in = as_const(a.inst)[i];
+    const int tb = in.tile_begin, nt = in.tile_count;
+    const auto part = as_global(a.partial);
+    int nact = 0;
+    for (int comp0 = 0; comp0 < a.nacc; comp0 += 64) {
+        const int comp = comp0 + lane;
+        const bool mine = comp < a.nacc;
         double sum = 0.0;
-        for (int t = tb; t < te; ++t) {
-            const int cnt = a.tile_active[t];
-            if (cnt <= 0) continue;
-            const int c0 = a.tile_chunk0[t], nch = (cnt + kChunk - 1) / kChunk;
-            for (int k = 0; k < nch; ++k) sum += a.partial[(int64_t)(c0 + k) * a.nacc + comp];
+        for (int g0 = 0; g0 < nt; g0 += 64) {
+            int cnt = 0, c0 = 0;
+            if (g0 + lane < nt) {
+                cnt = a.tile_active[tb + g0 + lane];
+                if (cnt > 0) c0 = a.tile_chunk0[tb + g0 + lane];
+            }
+            if (comp0 == 0) nact += cnt;
+            const int ng = min(64, nt - g0);
+            for (int t = 0; t < ng; ++t) {
+                const int tc = __builtin_amdgcn_readlane(cnt, t);
+                if (tc <= 0) continue;
+                const int tc0 = __builtin_amdgcn_readlane(c0, t);
+                const int nch = (tc + kChunk - 1) / kChunk;
+                for (int k = 0; k < nch; ++k)
+                    if (mine) sum += part[(int64_t)(tc0 + k) * a.nacc + comp];
+            }
         }
-        for (int sg = 0; sg < a.nseg; ++sg) {
-            const int rel = comp - a.begin[sg];
-            if (rel >= 0 && rel < a.width[sg] && a.dst[sg]) a.dst[sg][(int64_t)i * a.width[sg] + rel] = sum;
+        if (mine) {
+            for (int sg = 0; sg < a.nseg; ++sg) {
+                const int rel = comp - a.begin[sg];
+                if (rel >= 0 && rel < a.width[sg] && a.dst[sg]) a.dst[sg][(int64_t)i * a.width[sg] + rel] = sum;
+            }
         }
     }
-    if (a.n_active && lane == 0) {
-        int n = 0;
-        for (int t = tb; t < te; ++t) n += a.tile_active[t];
-        a.n_active[i] = n;
+    if (a.n_active) {
+        const int n = (int)wave_sum((double)nact);
+        if (lane == 0) a.n_active[i] = n;
     }
 }
 
@@ -606,28 +684,24 @@ __global__ void popcount_kernel(const uint32_t* bits, int64_t nwords, unsigned l
 
 // ------------------------------------------------------------------ launchers
 template <int BLOCK>
-static const void* detect_ptr_b(int ppl, bool precise) {
-    if (precise) {
-        return ppl == 1 ? (const void*)&detect_kernel<BLOCK, 1, true>
-             : ppl == 2 ? (const void*)&detect_kernel<BLOCK, 2, true> : (const void*)&detect_kernel<BLOCK, 4, true>;
-    }
-    return ppl == 1 ? (const void*)&detect_kernel<BLOCK, 1, false>
-         : ppl == 2 ? (const void*)&detect_kernel<BLOCK, 2, false> : (const void*)&detect_kernel<BLOCK, 4, false>;
+static const void* detect_ptr_b(int ppl) {
+    return ppl == 1 ? (const void*)&detect_kernel<BLOCK, 1>
+         : ppl == 2 ? (const void*)&detect_kernel<BLOCK, 2> : (const void*)&detect_kernel<BLOCK, 4>;
 }
 
-const void* detect_kernel_ptr(int block, int ppl, bool precise) {
+const void* detect_kernel_ptr(int block, int ppl) {
     if (ppl != 1 && ppl != 2 && ppl != 4) return nullptr;
     switch (block) {
-        case 128: return detect_ptr_b<128>(ppl, precise);
-        case 256: return detect_ptr_b<256>(ppl, precise);
-        case 512: return detect_ptr_b<512>(ppl, precise);
-        case 1024: return detect_ptr_b<1024>(ppl, precise);
+        case 128: return detect_ptr_b<128>(ppl);
+        case 256: return detect_ptr_b<256>(ppl);
+        case 512: return detect_ptr_b<512>(ppl);
+        case 1024: return detect_ptr_b<1024>(ppl);
         default: return nullptr;
     }
 }
 
-hipError_t launch_detect(const DetectArgs& a, int block, int ppl, bool precise, int grid, size_t lds, hipStream_t st) {
-    const void* f = detect_kernel_ptr(block, ppl, precise);
+hipError_t launch_detect(const DetectArgs& a, int block, int ppl, int grid, size_t lds, hipStream_t st) {
+    const void* f = detect_kernel_ptr(block, ppl);
     if (!f) return hipErrorInvalidValue;
     DetectArgs args = a;
     void* params[] = {&args};

@@ -91,22 +91,31 @@ struct Rec64 {
     double H[6];
 };
 
-// A work item of the contact and adjoint passes: records
-// [tile.slot + k*kChunk, min(+kChunk, tile_active)) of one tile.
+// A work item of the record, contact and adjoint passes: records
+// [rec0, rec0 + n) (n <= kChunk) of one tile of instance `inst`.
 struct ChunkDev {
-    int32_t tile;
-    int32_t k;
+    int32_t inst;
+    int32_t n;
+    int64_t rec0;
 };
 
 constexpr int kChunk = 256;         // records per work item
+
+// Active list: per slot, the point's offset within its instance, with kActSlow
+// set when its FD stencil leaves the cell (its record comes from the slow pass).
+constexpr uint32_t kActSlow = 0x80000000u;
+
+// Slow-list entry: the record slot and the instance of a slow-path contact.
+struct SlowDev {
+    uint32_t slot;
+    int32_t inst;
+};
 
 // Device counters of one evaluation (reset before detection).
 enum {
     CNT_CHUNKS = 0,                 // work items pushed by detection
     CNT_SLOW = 1,                   // contacts in the slow list
-    CNT_CONTACT_HEAD = 2,           // next work item of the contact pass
-    CNT_VJP_HEAD = 3,               // next work item of the adjoint pass
-    CNT_WORDS = 4
+    CNT_WORDS = 2
 };
 
 constexpr int kFmOut = 12;          // wrench of both bodies

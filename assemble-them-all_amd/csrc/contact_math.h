@@ -29,6 +29,33 @@
 
 namespace atac {
 
+// Address-space views for device code. Data that no kernel writes while it runs
+// (instances, grids, tiles) is read through the constant space, so wave-uniform
+// reads become scalar loads into SGPRs; grid and point arrays are read through
+// the global space (plain global loads instead of generic flat ones).
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+using const_ptr = const __attribute__((address_space(4))) T*;
+template <typename T>
+using global_ptr = const __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ const_ptr<T> as_const(const T* p) { return (const_ptr<T>)p; }
+template <typename T>
+__device__ __forceinline__ global_ptr<T> as_global(const T* p) { return (global_ptr<T>)p; }
+#define ATAC_GLOBAL(p) ::atac::as_global(p)
+#else
+// host pass (and the host-compiled check library): plain pointers
+template <typename T>
+using const_ptr = const T*;
+template <typename T>
+using global_ptr = const T*;
+template <typename T>
+ATAC_HD const T* as_const(const T* p) { return p; }
+template <typename T>
+ATAC_HD const T* as_global(const T* p) { return p; }
+#define ATAC_GLOBAL(p) (p)
+#endif
+
 // ------------------------------------------------------------------ small vectors
 template <typename T>
 struct v3 {
@@ -112,34 +139,58 @@ ATAC_HD double query_sdf64(const GridDev& G, double x0, double x1, double x2, bo
     return sqrt(e0 * e0 + e1 * e1 + e2 * e2) + trilinear64_inside(G, b0, b1, b2);
 }
 
-// FP64 value, FD gradient and FD Hessian at x, operation for operation as
-// query_SDF / query_dSDF / query_ddSDF (BodySDFObj.cpp:123-205); the six
-// axis-stencil values are shared between gradient and Hessian diagonal, as they
-// are bitwise the same evaluations in the reference.
-ATAC_HD void sdf_stencil64(const GridDev& G, const double x[3], double& val, double g[3], double H[6]) {
+// The FD stencils of query_dSDF / query_ddSDF (BodySDFObj.cpp:176-205) as 20
+// evaluation points: 0 = x (outside_accurate = false, the value), 1 = x, 2..7 =
+// x +- h along x, y, z, 8..19 = the half-step mixed stencils of xy, xz, yz
+// (+ +, - -, + -, - +). h = 1e-5.
+constexpr int kStencil = 20;
+
+ATAC_HD void stencil_point(const double x[3], int s, double p[3], bool& accurate) {
     const double h = kFdEps, hh = kFdEps / 2;
-    val = query_sdf64(G, x[0], x[1], x[2], false);
-    double xs = query_sdf64(G, x[0], x[1], x[2], true);
-    double px = query_sdf64(G, x[0] + h, x[1], x[2], true), mx = query_sdf64(G, x[0] - h, x[1], x[2], true);
-    double py = query_sdf64(G, x[0], x[1] + h, x[2], true), my = query_sdf64(G, x[0], x[1] - h, x[2], true);
-    double pz = query_sdf64(G, x[0], x[1], x[2] + h, true), mz = query_sdf64(G, x[0], x[1], x[2] - h, true);
-    g[0] = (px - mx) / (2 * h);
-    g[1] = (py - my) / (2 * h);
-    g[2] = (pz - mz) / (2 * h);
-    const double inv = 1.0 / (h * h);
-    H[0] = (px + mx - 2 * xs) / (h * h);
-    H[1] = (py + my - 2 * xs) / (h * h);
-    H[2] = (pz + mz - 2 * xs) / (h * h);
-    H[3] = (query_sdf64(G, (x[0] + hh), (x[1] + hh), x[2], true) + query_sdf64(G, (x[0] - hh), (x[1] - hh), x[2], true) -
-            query_sdf64(G, (x[0] + hh), (x[1] - hh), x[2], true) - query_sdf64(G, (x[0] - hh), (x[1] + hh), x[2], true)) /
-           (h * h);
-    H[4] = (query_sdf64(G, (x[0] + hh), x[1], (x[2] + hh), true) + query_sdf64(G, (x[0] - hh), x[1], (x[2] - hh), true) -
-            query_sdf64(G, (x[0] + hh), x[1], (x[2] - hh), true) - query_sdf64(G, (x[0] - hh), x[1], (x[2] + hh), true)) /
-           (h * h);
-    H[5] = (query_sdf64(G, x[0], (x[1] + hh), (x[2] + hh), true) + query_sdf64(G, x[0], (x[1] - hh), (x[2] - hh), true) -
-            query_sdf64(G, x[0], (x[1] + hh), (x[2] - hh), true) - query_sdf64(G, x[0], (x[1] - hh), (x[2] + hh), true)) /
-           (h * h);
-    (void)inv;
+    p[0] = x[0]; p[1] = x[1]; p[2] = x[2];
+    accurate = s != 0;
+    if (s >= 2 && s < 8) {
+        const int ax = (s - 2) >> 1;
+        p[ax] = (s & 1) ? x[ax] - h : x[ax] + h;
+    } else if (s >= 8) {
+        const int pair = (s - 8) >> 2, k = (s - 8) & 3;   // pair 0 = xy, 1 = xz, 2 = yz
+        const int a0 = pair == 2 ? 1 : 0, a1 = pair == 0 ? 1 : 2;
+        const double s0 = (k == 0 || k == 2) ? hh : -hh, s1 = (k == 0 || k == 3) ? hh : -hh;
+        p[a0] = x[a0] + s0;
+        p[a1] = x[a1] + s1;
+    }
+}
+
+// Value, gradient and Hessian from the 20 stencil values, in the reference's
+// operation order.
+ATAC_HD void stencil_combine(const double v[kStencil], double& val, double g[3], double H[6]) {
+    const double h = kFdEps;
+    val = v[0];
+    const double xs = v[1];
+    g[0] = (v[2] - v[3]) / (2 * h);
+    g[1] = (v[4] - v[5]) / (2 * h);
+    g[2] = (v[6] - v[7]) / (2 * h);
+    H[0] = (v[2] + v[3] - 2 * xs) / (h * h);
+    H[1] = (v[4] + v[5] - 2 * xs) / (h * h);
+    H[2] = (v[6] + v[7] - 2 * xs) / (h * h);
+    H[3] = (v[8] + v[9] - v[10] - v[11]) / (h * h);
+    H[4] = (v[12] + v[13] - v[14] - v[15]) / (h * h);
+    H[5] = (v[16] + v[17] - v[18] - v[19]) / (h * h);
+}
+
+// FP64 value, FD gradient and FD Hessian at x, operation for operation as
+// query_SDF / query_dSDF / query_ddSDF (BodySDFObj.cpp:123-205); the axis-stencil
+// values are shared between gradient and Hessian diagonal, as they are bitwise
+// the same evaluations in the reference.
+ATAC_HD void sdf_stencil64(const GridDev& G, const double x[3], double& val, double g[3], double H[6]) {
+    double v[kStencil];
+    for (int s = 0; s < kStencil; ++s) {
+        double p[3];
+        bool accurate;
+        stencil_point(x, s, p, accurate);
+        v[s] = query_sdf64(G, p[0], p[1], p[2], accurate);
+    }
+    stencil_combine(v, val, g, H);
 }
 
 // Cell of x when the whole FD stencil stays inside it (fast path), else false.
@@ -169,7 +220,7 @@ struct Corners {
 };
 
 ATAC_HD Corners load_corners(const GridDev& G, int i, int j, int k) {
-    const float* v = G.v;
+    const auto v = ATAC_GLOBAL(G.v);
     int64_t b = grid_index(G, i, j, k);
     int64_t sj = G.n[0], sk = (int64_t)G.n[0] * G.n[1];
     Corners c;
@@ -186,7 +237,7 @@ ATAC_HD Corners load_corners(const GridDev& G, int i, int j, int k) {
 
 // The corners of interior cell (i, j, k) from the corner-packed copy.
 ATAC_HD Corners load_cell(const GridDev& G, int i, int j, int k) {
-    const float4* q = G.cells + 2 * ((int64_t)i + (int64_t)(G.n[0] - 1) * ((int64_t)j + (int64_t)(G.n[1] - 1) * k));
+    const auto q = ATAC_GLOBAL(G.cells) + 2 * ((int64_t)i + (int64_t)(G.n[0] - 1) * ((int64_t)j + (int64_t)(G.n[1] - 1) * k));
     const float4 lo = q[0], hi = q[1];
     Corners c;
     c.v000 = lo.x; c.v100 = lo.y; c.v010 = lo.z; c.v110 = lo.w;
@@ -278,7 +329,7 @@ ATAC_HD float detect_value(const GridDev& G, const InstDev& in, float px, float 
     const int64_t di = i < G.n[0] - 1 ? 1 : 0;
     const int64_t dj = j < G.n[1] - 1 ? sj : 0;
     const int64_t dk = k < G.n[2] - 1 ? sk : 0;
-    const float* v = G.v + ((int64_t)i + sj * j + sk * k);
+    const auto v = ATAC_GLOBAL(G.v) + ((int64_t)i + sj * j + sk * k);
     const float a = lerpf(v[0], v[di], fx), b = lerpf(v[dj], v[dj + di], fx);
     const float c = lerpf(v[dk], v[dk + di], fx), d = lerpf(v[dk + dj], v[dk + dj + di], fx);
     return lerpf(lerpf(a, b, fy), lerpf(c, d, fy), fz) + (float)in.col_th;
@@ -292,10 +343,9 @@ ATAC_HD float detect_value(const GridDev& G, const InstDev& in, float px, float 
 // bounds clear the threshold by 4e-6 of that scale (5x the rounding bound) settles
 // the point; otherwise the value is interpolated exactly as detect_value does,
 // from the corner-packed cell (interior cells) or the 8 nodes (clamped cells).
-ATAC_HD bool detect_active(const GridDev& G, const InstDev& in, float px, float py, float pz) {
-    if (!G.fast) return detect_value(G, in, px, py, pz) < 0.0f;
-    double cc[3];
+ATAC_HD bool detect_active(const GridDev& G, const InstDev& in, float px, float py, float pz, double cc[3]) {
     grid_coords(in, px, py, pz, cc);
+    if (!G.fast) return detect_value(G, in, px, py, pz) < 0.0f;
     const double c0 = cc[0], c1 = cc[1], c2 = cc[2];
     const bool inside = c0 >= 0.0 && c0 <= in.cmax[0] && c1 >= 0.0 && c1 <= in.cmax[1] && c2 >= 0.0 && c2 <= in.cmax[2];
     if (!inside) return false;
@@ -304,18 +354,26 @@ ATAC_HD bool detect_active(const GridDev& G, const InstDev& in, float px, float 
     j = j < G.n[1] - 1 ? j : G.n[1] - 1;
     k = k < G.n[2] - 1 ? k : G.n[2] - 1;
     const float cth = (float)in.col_th;
-    const float2 b = G.blk[i / kBlk + G.nb[0] * (j / kBlk + G.nb[1] * (k / kBlk))];
+    const float2 b = ATAC_GLOBAL(G.blk)[i / kBlk + G.nb[0] * (j / kBlk + G.nb[1] * (k / kBlk))];
     const float tol = 4e-6f * (fmaxf(fabsf(b.x), fabsf(b.y)) + fabsf(cth));
     if (b.x + cth > tol) return false;
     if (b.y + cth < -tol) return true;
     if (i > G.n[0] - 2 || j > G.n[1] - 2 || k > G.n[2] - 2) return detect_value(G, in, px, py, pz) < 0.0f;
     const float fx = (float)(c0 - (double)i), fy = (float)(c1 - (double)j), fz = (float)(c2 - (double)k);
-    const float4* q = G.cells + 2 * ((int64_t)i + (int64_t)(G.n[0] - 1) * ((int64_t)j + (int64_t)(G.n[1] - 1) * k));
+    const auto q = ATAC_GLOBAL(G.cells) + 2 * ((int64_t)i + (int64_t)(G.n[0] - 1) * ((int64_t)j + (int64_t)(G.n[1] - 1) * k));
     const float4 lo = q[0], hi = q[1];
     const float a = lerpf(lo.x, lo.y, fx), bb = lerpf(lo.z, lo.w, fx);
     const float c = lerpf(hi.x, hi.y, fx), d = lerpf(hi.z, hi.w, fx);
     return lerpf(lerpf(a, bb, fy), lerpf(c, d, fy), fz) + cth < 0.0f;
 }
+
+ATAC_HD bool detect_active(const GridDev& G, const InstDev& in, float px, float py, float pz) {
+    double cc[3];
+    return detect_active(G, in, px, py, pz, cc);
+}
+
+// fast_cell_c without the outputs: does the FD stencil of c stay in its cell?
+ATAC_HD bool fast_cell_test(const GridDev& G, const InstDev& in, const double c[3]);
 
 // fast_cell in grid-index coordinates c = (x - min_box) / d (the detection map):
 // the cell is interior, and the FD stencil (+-1e-5 along each axis) stays inside
@@ -331,6 +389,87 @@ ATAC_HD bool fast_cell_c(const GridDev& G, const InstDev& in, const double c[3],
         fd[a] = fr;
     }
     return true;
+}
+
+ATAC_HD bool fast_cell_test(const GridDev& G, const InstDev& in, const double c[3]) {
+    int idx[3];
+    double fd[3];
+    return fast_cell_c(G, in, c, idx, fd);
+}
+
+// Detection of PPL points at once, the same decisions as detect_active (bit for
+// bit), with every grid read of a phase issued before any is used: the block
+// bounds of all inside points, then the packed cells of the points the bounds
+// leave undecided. Returns the active points (bit k).
+template <int PPL>
+ATAC_HD unsigned detect_points(const GridDev& G, const InstDev& in, const float (&x)[PPL], const float (&y)[PPL],
+                               const float (&z)[PPL], unsigned valid, bool all_active) {
+    const float cth = (float)in.col_th;
+    unsigned inside = 0, interior = 0;
+    int cell[PPL];
+    float fx[PPL], fy[PPL], fz[PPL];
+    float2 bb[PPL];
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) {
+        double c[3];
+        grid_coords(in, x[k], y[k], z[k], c);
+        const bool ins = ((valid >> k) & 1u) && c[0] >= 0.0 && c[0] <= in.cmax[0] && c[1] >= 0.0 &&
+                         c[1] <= in.cmax[1] && c[2] >= 0.0 && c[2] <= in.cmax[2];
+        int i = ins ? (int)c[0] : 0, j = ins ? (int)c[1] : 0, kk = ins ? (int)c[2] : 0;
+        i = i < G.n[0] - 1 ? i : G.n[0] - 1;
+        j = j < G.n[1] - 1 ? j : G.n[1] - 1;
+        kk = kk < G.n[2] - 1 ? kk : G.n[2] - 1;
+        if (ins) inside |= 1u << k;
+        if (ins && G.fast && i <= G.n[0] - 2 && j <= G.n[1] - 2 && kk <= G.n[2] - 2) interior |= 1u << k;
+        fx[k] = (float)(c[0] - (double)i);
+        fy[k] = (float)(c[1] - (double)j);
+        fz[k] = (float)(c[2] - (double)kk);
+        cell[k] = G.fast ? i + (G.n[0] - 1) * (j + (G.n[1] - 1) * kk) : 0;
+        bb[k] = make_float2(-INFINITY, INFINITY);
+        if (ins && G.fast) bb[k] = ATAC_GLOBAL(G.blk)[i / kBlk + G.nb[0] * (j / kBlk + G.nb[1] * (kk / kBlk))];
+    }
+    unsigned settled_in = 0, undecided = 0;
+    float4 lo[PPL], hi[PPL];
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) {
+        const float tol = 4e-6f * (fmaxf(fabsf(bb[k].x), fabsf(bb[k].y)) + fabsf(cth));
+        if (!((inside >> k) & 1u)) continue;
+        if (bb[k].y + cth < -tol) settled_in |= 1u << k;
+        else if (!(bb[k].x + cth > tol)) undecided |= 1u << k;
+        if (((undecided & interior) >> k) & 1u) {
+            const auto q = ATAC_GLOBAL(G.cells) + 2 * (int64_t)cell[k];
+            lo[k] = q[0];
+            hi[k] = q[1];
+        }
+    }
+    unsigned act = all_active ? valid : settled_in;
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) {
+        if (all_active || !((undecided >> k) & 1u)) continue;
+        bool a;
+        if ((interior >> k) & 1u) {
+            const float aa = lerpf(lo[k].x, lo[k].y, fx[k]), bq = lerpf(lo[k].z, lo[k].w, fx[k]);
+            const float cq = lerpf(hi[k].x, hi[k].y, fx[k]), dq = lerpf(hi[k].z, hi[k].w, fx[k]);
+            a = lerpf(lerpf(aa, bq, fy[k]), lerpf(cq, dq, fy[k]), fz[k]) + cth < 0.0f;
+        } else {
+            a = detect_value(G, in, x[k], y[k], z[k]) < 0.0f;
+        }
+        if (a) act |= 1u << k;
+    }
+    return act;
+}
+
+// Cell and fractions of grid coordinates c known to pass fast_cell_c (the
+// detection pass tested it); the index is clamped to the interior cells, so
+// the corner reads stay inside the grid whatever c is.
+ATAC_HD void cell_of_c(const GridDev& G, const double c[3], int idx[3], double fd[3]) {
+    for (int a = 0; a < 3; ++a) {
+        const double fl = floor(c[a]);
+        fd[a] = c[a] - fl;
+        int i = (int)fl;
+        i = i < 0 ? 0 : (i > G.n[a] - 2 ? G.n[a] - 2 : i);
+        idx[a] = i;
+    }
 }
 
 // FP64 value, closed-form gradient and mixed partials of the trilinear cell, from
@@ -479,23 +618,26 @@ ATAC_HD void record_fast(const GridDev& G, const InstDev& in, const Corners& cr,
 
 // Slow-path sample (FD stencils operation for operation in FP64) of a record's
 // point; fills the record (and, for FP32 records, the diagonal Hessian entry).
-ATAC_HD void record_slow(const GridDev& G, const InstDev& in, Rec32& r, float* diag) {
-    double x[3], val, g[3], H[6];
-    body2_point(in, (double)r.a[0], (double)r.a[1], (double)r.a[2], x);
-    sdf_stencil64(G, x, val, g, H);
+ATAC_HD void record_slow_fill(const InstDev& in, double val, const double g[3], const double H[6], Rec32& r,
+                              float* diag) {
     r.b[0] = (float)(val + in.col_th);
     r.b[1] = (float)g[0]; r.b[2] = (float)g[1]; r.b[3] = (float)g[2];
     r.c[0] = (float)H[3]; r.c[1] = (float)H[4]; r.c[2] = (float)H[5];
     diag[0] = (float)H[0]; diag[1] = (float)H[1]; diag[2] = (float)H[2]; diag[3] = 0.0f;
 }
 
-ATAC_HD void record_slow(const GridDev& G, const InstDev& in, Rec64& r, float*) {
-    double x[3], val, g[3], H[6];
-    body2_point(in, (double)r.a[0], (double)r.a[1], (double)r.a[2], x);
-    sdf_stencil64(G, x, val, g, H);
+ATAC_HD void record_slow_fill(const InstDev& in, double val, const double g[3], const double H[6], Rec64& r, float*) {
     r.d = val + in.col_th;
     for (int k = 0; k < 3; ++k) r.g[k] = g[k];
     for (int k = 0; k < 6; ++k) r.H[k] = H[k];
+}
+
+template <typename Rec>
+ATAC_HD void record_slow(const GridDev& G, const InstDev& in, Rec& r, float* diag) {
+    double x[3], val, g[3], H[6];
+    body2_point(in, (double)r.a[0], (double)r.a[1], (double)r.a[2], x);
+    sdf_stencil64(G, x, val, g, H);
+    record_slow_fill(in, val, g, H, r, diag);
 }
 
 // Forward state of a record: the contact point, its SDF-frame position (formed

@@ -35,8 +35,9 @@ METRIC = "contact-point SDF queries/s (force+adjoint) at 1 GPU; % HBM roofline; 
 DETECT_INST_BYTES = 72 + 24 + 24 + 8 + 24 + 8 + 4   # instance fields detection reads: A, b, cmax, col_th,
                                                     # point pointers, point count, grid index
 TILE_BYTES = 24 + 4 + 4 + 4                         # TileDev + queue entry + tile_active + tile_chunk0 writes
-RECORD_BYTES = 48                                   # FP32 active-contact record (written once per active point)
-CHUNK_BYTES = 8                                     # work-item entry, one per kChunk records
+ACT_BYTES = 4                                       # active-list entry (point offset) per active point
+CHUNK_BYTES = 16                                    # work-item entry, one per kChunk active points
+RECORD_BYTES = 48                                   # FP32 active-contact record
 K_CHUNK = 256
 
 
@@ -180,17 +181,19 @@ def main():
 
     touched = b.touched_values()
     # detection: every point read once (12 B), every grid value its stencils touch
-    # once (4 B), the instance fields and tile entries, and one record per active point
+    # once (4 B), the instance fields and tile entries read, and the active list
+    # (one offset per active point) and its work items written
     n_chunks_lb = -(-n_active // K_CHUNK)
     det_bytes = (12 * n_q + 4 * touched + DETECT_INST_BYTES * b.n + TILE_BYTES * info["n_tiles"]
-                 + RECORD_BYTES * n_active + CHUNK_BYTES * n_chunks_lb)
+                 + ACT_BYTES * n_active + CHUNK_BYTES * n_chunks_lb)
     det_ms = stage_ms["detect"]
     achieved = det_bytes / (det_ms * 1e-3) / 1e9
-    # the step: detection, then the contact and adjoint passes each read the records
-    # once, write one partial row per work item, and the reductions write fm, read
-    # g_fm and write the 26 gradients per instance
-    step_bytes = (det_bytes + 2 * RECORD_BYTES * n_active + 8 * (12 + 26) * n_chunks_lb
-                  + 8 * 12 * b.n * 2 + 8 * 26 * b.n)
+    # the step: detection; the record pass reads the active list and points and
+    # writes one record per active point; the contact and adjoint passes each read
+    # the records and write one partial row per work item; the reductions write fm,
+    # read g_fm and write the 26 gradients per instance
+    step_bytes = (det_bytes + (ACT_BYTES + 12 + RECORD_BYTES) * n_active + 2 * RECORD_BYTES * n_active
+                  + 8 * (12 + 26) * n_chunks_lb + 8 * 12 * b.n * 2 + 8 * 26 * b.n)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
                 "kernel": "detect_kernel", "algorithmic_bytes_per_launch": int(det_bytes),
@@ -229,8 +232,8 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": w.name, "instances": b.n, "queries": n_q, "grids": f"{len(w.grids)} x "
                    f"{'x'.join(str(v) for v in w.grids[0].n)} FP32", "active_fraction": round(n_active / n_q, 5),
-                   "step": "detect + slow-path samples + contact wrench + reduce + adjoint (VJP) + reduce, "
-                           "HIP graph replay",
+                   "step": "detect + records + slow-path samples + contact wrench + reduce + adjoint (VJP) "
+                           "+ reduce, HIP graph replay",
                    "tile_points": info["tile_points"], "detect_block": info["detect_block"],
                    "points_per_lane": info["points_per_lane"],
                    "persistent_blocks": info["persistent_blocks"], "n_cu": n_cu, "parallelism": "none"},

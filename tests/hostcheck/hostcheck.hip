@@ -165,13 +165,31 @@ int hc_detect(const float* vals, const int32_t n[3], const double minb[3], const
 // settled[o] = 1 when the block bound alone settled the point.
 int hc_detect_decisions(const float* vals, const int32_t n[3], const double minb[3], const double maxb[3],
                         const double d[3], const float* px, const float* py, const float* pz, int64_t npts,
-                        const atac_instance* inst, uint8_t* decided, uint8_t* full, uint8_t* settled) {
+                        const atac_instance* inst, uint8_t* decided, uint8_t* full, uint8_t* settled,
+                        uint8_t* batched, uint8_t* slow) {
     HostGrid hg;
     host_grid(vals, n, minb, maxb, d, hg);
     const GridDev& G = hg.G;
     InstDev in;
     std::memset(&in, 0, sizeof in);
     compose_instance(*inst, G, px, py, pz, npts, in);
+    // the detection kernel's 4-points-at-a-time form, ragged tail included
+    for (int64_t o0 = 0; o0 < npts; o0 += 4) {
+        float x[4], y[4], z[4];
+        unsigned valid = 0;
+        for (int k = 0; k < 4; ++k) {
+            const int64_t o = o0 + k < npts ? o0 + k : npts - 1;
+            x[k] = px[o]; y[k] = py[o]; z[k] = pz[o];
+            if (o0 + k < npts) valid |= 1u << k;
+        }
+        const unsigned act = detect_points<4>(G, in, x, y, z, valid, false);
+        for (int k = 0; k < 4 && o0 + k < npts; ++k) {
+            batched[o0 + k] = (act >> k) & 1u;
+            double cc[3];
+            grid_coords(in, x[k], y[k], z[k], cc);
+            slow[o0 + k] = ((act >> k) & 1u) && !fast_cell_test(G, in, cc);
+        }
+    }
     for (int64_t o = 0; o < npts; ++o) {
         decided[o] = detect_active(G, in, px[o], py[o], pz[o]) ? 1 : 0;
         full[o] = detect_value(G, in, px[o], py[o], pz[o]) < 0.0f ? 1 : 0;

@@ -167,3 +167,50 @@ def test_graph_with_adjoint(setup):
         for k in eager:
             assert np.array_equal(grads[k].cpu().numpy(), eager[k]), (rep, k)
         torch.ones(100, device="cuda").sum().item()
+
+
+def test_autograd_function(setup):
+    """contact_wrench (torch.autograd.Function over the hand-written adjoint): the
+    forward equals the batch evaluation; the backward's phi and kn gradients and the
+    pairing of the pose gradients with tangent directions E xi^ equal the oracle's
+    Km / Dm contractions; evaluating again between forward and backward raises."""
+    from atac.autograd import contact_wrench
+    w, ref, b = setup
+    n = b.n
+    dev = "cuda"
+    E1 = torch.tensor(np.stack([ins.E1 for ins in w.instances]), device=dev, requires_grad=True)
+    E2 = torch.tensor(np.stack([ins.E2 for ins in w.instances]), device=dev, requires_grad=True)
+    phi1 = torch.tensor(np.stack([ins.phi1 for ins in w.instances]), device=dev, requires_grad=True)
+    phi2 = torch.tensor(np.stack([ins.phi2 for ins in w.instances]), device=dev, requires_grad=True)
+    kn = torch.tensor([ins.kn for ins in w.instances], device=dev, dtype=torch.float64, requires_grad=True)
+    kt = torch.tensor([ins.kt for ins in w.instances], device=dev, dtype=torch.float64, requires_grad=True)
+    fm = contact_wrench(b, E1, E2, phi1, phi2, kn, kt)
+    direct = b.eval()["fm"]
+    torch.cuda.synchronize()
+    assert torch.equal(fm.detach(), direct)
+    # the direct evaluation above replaced the forward's records
+    with pytest.raises(RuntimeError):
+        fm.sum().backward()
+    fm = contact_wrench(b, E1, E2, phi1, phi2, kn, kt)
+    rng = np.random.default_rng(17)
+    g = rng.normal(size=(n, 12))
+    (fm * torch.tensor(g, device=dev)).sum().backward()
+    gE1, gE2 = E1.grad.cpu().numpy(), E2.grad.cpu().numpy()
+    gp1, gp2 = phi1.grad.cpu().numpy(), phi2.grad.cpu().numpy()
+    for i, r in enumerate(ref):
+        if not r["n_active"]:
+            continue
+        Kc, Dc = g[i] @ r["Km"], g[i] @ r["Dm"]
+        s = np.linalg.norm(g[i]) * r["scale_Km"]
+        for _ in range(3):
+            xi = rng.normal(size=6)
+            hat = np.zeros((4, 4))
+            hat[:3, :3] = [[0, -xi[2], xi[1]], [xi[2], 0, -xi[0]], [-xi[1], xi[0], 0]]
+            hat[:3, 3] = xi[3:]
+            p1 = np.sum(gE1[i] * (w.instances[i].E1 @ hat))
+            p2 = np.sum(gE2[i] * (w.instances[i].E2 @ hat))
+            assert abs(p1 - Kc[:6] @ xi) <= TOL * s * np.linalg.norm(xi)
+            assert abs(p2 - Kc[6:] @ xi) <= TOL * s * np.linalg.norm(xi)
+        sd = np.linalg.norm(g[i]) * max(r["scale_Dm"], 1e-30)
+        assert rel_err(np.concatenate([gp1[i], gp2[i]]), Dc, sd) <= TOL
+    assert torch.isfinite(kn.grad).all() and torch.isfinite(kt.grad).all()
