@@ -193,9 +193,16 @@ int atac_grid_create(const float* values, int32_t ni, int32_t nj, int32_t nk, co
         G.inv_d[a] = 1.0 / dxyz[a];
         G.nb[a] = (G.n[a] - 1) / kBlk + 1;
     }
-    if (ni >= 2 && nj >= 2 && nk >= 2) {
-        const int64_t nc = (int64_t)(ni - 1) * (nj - 1) * (nk - 1);
+    {
+        // corner-packed cells, one per node (the last layer clamped), addressed
+        // by 32-bit byte offsets in the detection kernel
+        const int64_t nc = (int64_t)ni * nj * nk;
         const int64_t nb = (int64_t)G.nb[0] * G.nb[1] * G.nb[2];
+        if (nc * 32 >= ((int64_t)1 << 32)) {
+            dfree(g->d_values);
+            delete g;
+            return fail(ATAC_ERR_INVALID, "atac_grid_create: grids hold fewer than 2^27 nodes");
+        }
         rc = dalloc(&g->d_cells, (size_t)(2 * nc));
         if (!rc) rc = dalloc(&g->d_blk, (size_t)nb);
         if (!rc) {

@@ -125,53 +125,51 @@ __device__ __forceinline__ void mark_slow_index(Rec32& r, int slow) { r.c[3] = _
 __device__ __forceinline__ void mark_slow_index(Rec64&, int) {}
 
 // ------------------------------------------------------------------ batched detection
-// PPL consecutive points from offset o (nontemporal vector loads); nothing is
-// read when o >= end (o is a multiple of PPL and the point arrays are padded to
-// a multiple of 4).
+// PPL consecutive points from offset o (nontemporal vector loads). Past the end
+// (o >= end) the tile's first PPL points are read instead, for the caller to
+// mask out: the loads are unconditional and their values are used as loaded, so
+// they stay in flight until the points are tested (a tile holds at least one
+// point; the point arrays are padded to a multiple of 4 and o is a multiple of
+// PPL).
 template <int PPL, typename P>
 __device__ __forceinline__ void load_points(P px, P py, P pz, int o, int end, float (&x)[PPL], float (&y)[PPL],
                                             float (&z)[PPL]) {
-    if (o >= end) {
-#pragma unroll
-        for (int k = 0; k < PPL; ++k) x[k] = y[k] = z[k] = 0.0f;
-        return;
-    }
+    const uint32_t oo = (uint32_t)(o < end ? o : 0) * 4u;   // byte offset
     if constexpr (PPL == 4) {
-        const f4v vx = __builtin_nontemporal_load(reinterpret_cast<global_ptr<f4v>>(px + o));
-        const f4v vy = __builtin_nontemporal_load(reinterpret_cast<global_ptr<f4v>>(py + o));
-        const f4v vz = __builtin_nontemporal_load(reinterpret_cast<global_ptr<f4v>>(pz + o));
+        const f4v vx = nt_load_at<f4v>(px, oo), vy = nt_load_at<f4v>(py, oo), vz = nt_load_at<f4v>(pz, oo);
 #pragma unroll
         for (int k = 0; k < PPL; ++k) { x[k] = vx[k]; y[k] = vy[k]; z[k] = vz[k]; }
     } else if constexpr (PPL == 2) {
-        const f2v vx = __builtin_nontemporal_load(reinterpret_cast<global_ptr<f2v>>(px + o));
-        const f2v vy = __builtin_nontemporal_load(reinterpret_cast<global_ptr<f2v>>(py + o));
-        const f2v vz = __builtin_nontemporal_load(reinterpret_cast<global_ptr<f2v>>(pz + o));
+        const f2v vx = nt_load_at<f2v>(px, oo), vy = nt_load_at<f2v>(py, oo), vz = nt_load_at<f2v>(pz, oo);
 #pragma unroll
         for (int k = 0; k < PPL; ++k) { x[k] = vx[k]; y[k] = vy[k]; z[k] = vz[k]; }
     } else {
-        x[0] = __builtin_nontemporal_load(px + o);
-        y[0] = __builtin_nontemporal_load(py + o);
-        z[0] = __builtin_nontemporal_load(pz + o);
+        x[0] = nt_load_at<float>(px, oo);
+        y[0] = nt_load_at<float>(py, oo);
+        z[0] = nt_load_at<float>(pz, oo);
     }
 }
 
-// Points of one tile per work item. The tile is streamed (PPL consecutive
-// points per lane, vector loads, each wavefront a contiguous span of the tile);
-// the offsets of the points with d < 0 are compacted into per-wave LDS segments
-// in ascending order (a wave-wide exclusive prefix of the per-lane counts built
-// from ballots of their bits), then copied to the tile's slots of the active
-// list, and the tile pushes its work items for the record, contact and adjoint
-// passes. Only the detection arithmetic lives here, so the kernel stays small
-// enough for a high occupancy on this latency-bound stream.
+// Points of one tile per work item. The tile is streamed: PPL consecutive
+// points per lane (vector loads, each wavefront a contiguous span of the tile),
+// in software-pipelined steps (detect_step): a step's reads are the cells of the
+// previous step's open points, the block bounds of its own points and the next
+// step's points, issued in that order, so each step waits once, for reads
+// issued in the same step. The active offsets (bit 15: slow path; tiles hold at
+// most 32768 points) are compacted into per-wave LDS segments (a wave-wide
+// exclusive prefix of the per-lane counts built from ballots of their bits),
+// copied to the tile's slots of the active list, and the tile pushes its work
+// items for the record, contact and adjoint passes.
 template <int BLOCK, int PPL>
 __global__ __launch_bounds__(BLOCK) void detect_kernel(DetectArgs a) {
     constexpr int NW = BLOCK / 64;
-    constexpr int NB = PPL == 1 ? 1 : (PPL == 2 ? 2 : 3);   // bits of a per-lane count <= PPL
-    extern __shared__ uint16_t s_queue[];   // tile offsets of the active points
+    constexpr int NB = PPL == 1 ? 2 : (PPL == 2 ? 3 : 4);   // bits of a per-lane count <= 2 PPL
+    extern __shared__ uint16_t s_queue[];   // [NW][span] active offsets, bit 15 = slow path
     __shared__ int s_wcount[NW];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int span = a.wave_span;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
+    const bool all_active = a.all_active != 0;
     const auto order = as_const(a.q.order);
     const TileWalk walk(a.q);
     for (int kk = walk.k; kk < walk.end; kk += walk.step) {
@@ -179,27 +177,17 @@ __global__ __launch_bounds__(BLOCK) void detect_kernel(DetectArgs a) {
         const TileDev t = as_const(a.tiles)[tile];
         const InstDev in = as_const(a.inst)[t.inst];
         const GridDev G = as_const(a.grids)[in.grid];
+        const StepConsts K = step_consts(G, in);
         const auto px = as_global(in.px) + t.begin;
         const auto py = as_global(in.py) + t.begin;
         const auto pz = as_global(in.pz) + t.begin;
         uint16_t* q = s_queue + w * span;
         const int wb = w * span, we = min(t.count, wb + span);
         int wcount = 0;
-        // points of the next iteration are loaded while this one is tested
-        float nx[PPL], ny[PPL], nz[PPL];
-        load_points<PPL>(px, py, pz, wb + lane * PPL, we, nx, ny, nz);
-        for (int o0 = wb; o0 < we; o0 += 64 * PPL) {
-            const int o = o0 + lane * PPL;
-            float x[PPL], y[PPL], z[PPL];
-#pragma unroll
-            for (int k = 0; k < PPL; ++k) { x[k] = nx[k]; y[k] = ny[k]; z[k] = nz[k]; }
-            load_points<PPL>(px, py, pz, o + 64 * PPL, we, nx, ny, nz);
-            unsigned valid = 0;
-#pragma unroll
-            for (int k = 0; k < PPL; ++k)
-                if (o + k < we) valid |= 1u << k;
-            const unsigned m = detect_points<PPL>(G, in, x, y, z, valid, a.all_active != 0);
-            const int c = __popc(m);
+        // appends points oa + k (bits ma, slow bits sa), then ob + k (mb, sb), in
+        // lane order
+        auto emit = [&](unsigned ma, int oa, unsigned sa, unsigned mb, int ob, unsigned sb) {
+            const int c = __popc(ma) + __popc(mb);
             int pre = 0, tot = 0;
 #pragma unroll
             for (int bit = 0; bit < NB; ++bit) {
@@ -210,8 +198,46 @@ __global__ __launch_bounds__(BLOCK) void detect_kernel(DetectArgs a) {
             int pos = wcount + pre;
 #pragma unroll
             for (int k = 0; k < PPL; ++k)
-                if ((m >> k) & 1u) q[pos++] = (uint16_t)(o + k);
+                if ((ma >> k) & 1u) q[pos++] = (uint16_t)((oa + k) | (((sa >> k) & 1u) << 15));
+#pragma unroll
+            for (int k = 0; k < PPL; ++k)
+                if ((mb >> k) & 1u) q[pos++] = (uint16_t)((ob + k) | (((sb >> k) & 1u) << 15));
             wcount += tot;
+        };
+        OpenPoints<PPL> op;
+        op.mask = 0u;
+        op.o = 0;
+        // one step: points (cx, cy, cz) at o0 are tested while the next ones load
+        // into (nx, ny, nz); the loop alternates the two buffers, so no register
+        // copy waits for the loads in flight
+        auto step = [&](int o0, const float (&cx)[PPL], const float (&cy)[PPL], const float (&cz)[PPL],
+                        float (&nx)[PPL], float (&ny)[PPL], float (&nz)[PPL]) {
+            const int o = o0 + lane * PPL;
+            unsigned valid = 0;
+#pragma unroll
+            for (int k = 0; k < PPL; ++k)
+                if (o + k < we) valid |= 1u << k;
+            const int po = op.o;
+            unsigned sl, pm, psl;
+            const unsigned m = detect_step<PPL>(G, in, K, cx, cy, cz, valid, all_active, sl, op, pm, psl, [&]() {
+                __builtin_amdgcn_sched_barrier(0);
+                load_points<PPL>(px, py, pz, o + 64 * PPL, we, nx, ny, nz);
+                __builtin_amdgcn_sched_barrier(0);
+            });
+            op.o = o;
+            emit(pm, po, psl, m, o, sl);
+        };
+        float x0[PPL], y0[PPL], z0[PPL], x1[PPL], y1[PPL], z1[PPL];
+        load_points<PPL>(px, py, pz, wb + lane * PPL, we, x0, y0, z0);
+        for (int o0 = wb; o0 < we; o0 += 2 * 64 * PPL) {
+            step(o0, x0, y0, z0, x1, y1, z1);
+            if (o0 + 64 * PPL >= we) break;
+            step(o0 + 64 * PPL, x1, y1, z1, x0, y0, z0);
+        }
+        {   // the open points of the last step
+            unsigned psl;
+            const unsigned pm = open_finish<PPL>(G, K, op, psl);
+            emit(pm, op.o, psl, 0u, 0, 0u);
         }
         if (lane == 0) s_wcount[w] = wcount;
         __syncthreads();
@@ -223,12 +249,9 @@ __global__ __launch_bounds__(BLOCK) void detect_kernel(DetectArgs a) {
         for (int qi = tid; qi < cnt; qi += BLOCK) {
             int rem = qi, ww = 0;
             while (rem >= s_wcount[ww]) { rem -= s_wcount[ww]; ++ww; }
-            const uint32_t off = base + s_queue[ww * span + rem];
-            // the slow-path test of an active point (its FD stencil leaves the cell)
-            double cc[3];
-            grid_coords(in, px[off - base], py[off - base], pz[off - base], cc);
-            const bool slow = !fast_cell_test(G, in, cc);
-            act[qi] = off | (slow ? kActSlow : 0u);
+            const uint32_t e = s_queue[ww * span + rem];
+            const bool slow = (e >> 15) != 0u;
+            act[qi] = (base + (e & 0x7fffu)) | (slow ? kActSlow : 0u);
             if (slow) {
                 const int si = atomicAdd(a.counters + CNT_SLOW, 1);
                 a.slow[si] = SlowDev{(uint32_t)(t.slot + qi), t.inst};
@@ -505,7 +528,7 @@ __global__ __launch_bounds__(256) void reduce_kernel(ReduceArgs a) {
 
 // ------------------------------------------------------------------ grid set-up
 __global__ void build_cells_kernel(GridDev G, float4* cells) {
-    const int64_t ci = G.n[0] - 1, cj = G.n[1] - 1, ck = G.n[2] - 1, nc = ci * cj * ck;
+    const int64_t ci = G.n[0], cj = G.n[1], ck = G.n[2], nc = ci * cj * ck;
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x)
         pack_cell(G, (int)(c % ci), (int)((c / ci) % cj), (int)(c / (ci * cj)), cells + 2 * c);
 }

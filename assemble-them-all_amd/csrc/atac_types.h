@@ -10,13 +10,14 @@ namespace atac {
 struct GridDev {
     const float* v;
     int32_t n[3];
-    int32_t fast;       // 1: cells and blk below exist (every n >= 2)
+    int32_t fast;       // 1: cells and blk below exist (every grid of the library has them)
     double minb[3];
     double maxb[3];
     double d[3];
     double inv_d[3];
-    // corner-packed copy: cell (i, j, k) < n - 1 holds its 8 corners
-    // (v000 v100 v010 v110 | v001 v101 v011 v111) in 2 x float4, cell-index order
+    // corner-packed copy: cell (i, j, k) < n holds the 8 nodes a point in it
+    // interpolates (v000 v100 v010 v110 | v001 v101 v011 v111, offsets clamped at
+    // the last node) in 2 x float4, node-index order
     const float4* cells;
     // conservative bounds of the detection value per block of kBlk^3 cells:
     // (min, max) of every node the block's cells read (with the detection's clamp

@@ -56,6 +56,17 @@ ATAC_HD const T* as_global(const T* p) { return p; }
 #define ATAC_GLOBAL(p) (p)
 #endif
 
+// Program-order pins for device code: `pin(v)` makes v computed at this point
+// (the compiler may not sink its computation below it); no instruction is
+// scheduled across `sched_fence()`. No-ops on the host.
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ void pin(unsigned& v) { asm volatile("" : "+v"(v)); }
+__device__ __forceinline__ void sched_fence() { __builtin_amdgcn_sched_barrier(0); }
+#else
+ATAC_HD void pin(unsigned&) {}
+ATAC_HD void sched_fence() {}
+#endif
+
 // ------------------------------------------------------------------ small vectors
 template <typename T>
 struct v3 {
@@ -235,9 +246,9 @@ ATAC_HD Corners load_corners(const GridDev& G, int i, int j, int k) {
     return c;
 }
 
-// The corners of interior cell (i, j, k) from the corner-packed copy.
+// The corners of cell (i, j, k) from the corner-packed copy.
 ATAC_HD Corners load_cell(const GridDev& G, int i, int j, int k) {
-    const auto q = ATAC_GLOBAL(G.cells) + 2 * ((int64_t)i + (int64_t)(G.n[0] - 1) * ((int64_t)j + (int64_t)(G.n[1] - 1) * k));
+    const auto q = ATAC_GLOBAL(G.cells) + 2 * ((int64_t)i + (int64_t)G.n[0] * ((int64_t)j + (int64_t)G.n[1] * k));
     const float4 lo = q[0], hi = q[1];
     Corners c;
     c.v000 = lo.x; c.v100 = lo.y; c.v010 = lo.z; c.v110 = lo.w;
@@ -246,10 +257,17 @@ ATAC_HD Corners load_cell(const GridDev& G, int i, int j, int k) {
 }
 
 // Set-up of the corner-packed copy and the block bounds (one cell / one block).
+// Cell (i, j, k), 0 <= i < n: the 8 nodes detect_value reads for a point in it,
+// the offsets clamped at the last node (a cell of the last layer repeats it).
 ATAC_HD void pack_cell(const GridDev& G, int i, int j, int k, float4 out[2]) {
-    const Corners q = load_corners(G, i, j, k);
-    out[0] = make_float4(q.v000, q.v100, q.v010, q.v110);
-    out[1] = make_float4(q.v001, q.v101, q.v011, q.v111);
+    const auto v = G.v;
+    const int64_t b = grid_index(G, i, j, k);
+    const int64_t sj = G.n[0], sk = (int64_t)G.n[0] * G.n[1];
+    const int64_t di = i < G.n[0] - 1 ? 1 : 0;
+    const int64_t dj = j < G.n[1] - 1 ? sj : 0;
+    const int64_t dk = k < G.n[2] - 1 ? sk : 0;
+    out[0] = make_float4(v[b], v[b + di], v[b + dj], v[b + dj + di]);
+    out[1] = make_float4(v[b + dk], v[b + dk + di], v[b + dk + dj], v[b + dk + dj + di]);
 }
 
 ATAC_HD float2 block_bounds(const GridDev& G, int bi, int bj, int bk) {
@@ -335,14 +353,60 @@ ATAC_HD float detect_value(const GridDev& G, const InstDev& in, float px, float 
     return lerpf(lerpf(a, b, fy), lerpf(c, d, fy), fz) + (float)in.col_th;
 }
 
+// Nontemporal load of a V at a 32-bit byte offset from a wave-uniform base.
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename V, typename T>
+__device__ __forceinline__ V nt_load_at(global_ptr<T> base, uint32_t byte_off) {
+    using GV = const __attribute__((address_space(1))) V*;
+    using GC = const __attribute__((address_space(1))) char*;
+    return __builtin_nontemporal_load(reinterpret_cast<GV>(reinterpret_cast<GC>(base) + byte_off));
+}
+#else
+template <typename V, typename T>
+ATAC_HD V nt_load_at(const T* base, uint32_t byte_off) {
+    return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+#endif
+
+// The trilinear detection value of a cell's 8 corners (corner-packed order, the
+// lerp order of detect_value) at fractions (fx, fy, fz), plus col_th.
+ATAC_HD float cell_value(float4 lo, float4 hi, float fx, float fy, float fz, float cth) {
+    const float a = lerpf(lo.x, lo.y, fx), b = lerpf(lo.z, lo.w, fx);
+    const float c = lerpf(hi.x, hi.y, fx), d = lerpf(hi.z, hi.w, fx);
+    return lerpf(lerpf(a, b, fy), lerpf(c, d, fy), fz) + cth;
+}
+
+// Exact detection of a point the block bounds left open: the value is interpolated
+// as detect_value does, from the corner-packed cell (whose last layer holds the
+// clamped corners), or from the 8 nodes on a grid without the table.
+ATAC_HD bool detect_interp(const GridDev& G, const InstDev& in, float px, float py, float pz, const double cc[3]) {
+    const double c0 = cc[0], c1 = cc[1], c2 = cc[2];
+    const bool inside = c0 >= 0.0 && c0 <= in.cmax[0] && c1 >= 0.0 && c1 <= in.cmax[1] && c2 >= 0.0 && c2 <= in.cmax[2];
+    if (!inside) return false;
+    if (!G.fast) return detect_value(G, in, px, py, pz) < 0.0f;
+    int i = (int)c0, j = (int)c1, k = (int)c2;
+    i = i < G.n[0] - 1 ? i : G.n[0] - 1;
+    j = j < G.n[1] - 1 ? j : G.n[1] - 1;
+    k = k < G.n[2] - 1 ? k : G.n[2] - 1;
+    const float fx = (float)(c0 - (double)i), fy = (float)(c1 - (double)j), fz = (float)(c2 - (double)k);
+    const auto q = ATAC_GLOBAL(G.cells) + 2 * ((int64_t)i + (int64_t)G.n[0] * ((int64_t)j + (int64_t)G.n[1] * k));
+    return cell_value(q[0], q[1], fx, fy, fz, (float)in.col_th) < 0.0f;
+}
+
 // The detection decision detect_value(...) < 0, bit for bit, with fewer grid
 // reads. The trilinear value is a convex combination of its 8 corners, so it lies
 // in the (min, max) of the block holding the cell, up to the FP32 rounding of the
 // 7 lerps and the col_th add: at most 13 units of 2^-24 of max|corner| + |col_th|
 // (each lerp adds <= 4 units of its largest input; the add 1 unit). A block whose
 // bounds clear the threshold by 4e-6 of that scale (5x the rounding bound) settles
-// the point; otherwise the value is interpolated exactly as detect_value does,
-// from the corner-packed cell (interior cells) or the 8 nodes (clamped cells).
+// the point; otherwise detect_interp decides it.
+ATAC_HD int block_decision(float2 b, float cth) {   // 1 active, 0 inactive, -1 open
+    const float tol = 4e-6f * (fmaxf(fabsf(b.x), fabsf(b.y)) + fabsf(cth));
+    if (b.x + cth > tol) return 0;
+    if (b.y + cth < -tol) return 1;
+    return -1;
+}
+
 ATAC_HD bool detect_active(const GridDev& G, const InstDev& in, float px, float py, float pz, double cc[3]) {
     grid_coords(in, px, py, pz, cc);
     if (!G.fast) return detect_value(G, in, px, py, pz) < 0.0f;
@@ -353,18 +417,10 @@ ATAC_HD bool detect_active(const GridDev& G, const InstDev& in, float px, float 
     i = i < G.n[0] - 1 ? i : G.n[0] - 1;
     j = j < G.n[1] - 1 ? j : G.n[1] - 1;
     k = k < G.n[2] - 1 ? k : G.n[2] - 1;
-    const float cth = (float)in.col_th;
-    const float2 b = ATAC_GLOBAL(G.blk)[i / kBlk + G.nb[0] * (j / kBlk + G.nb[1] * (k / kBlk))];
-    const float tol = 4e-6f * (fmaxf(fabsf(b.x), fabsf(b.y)) + fabsf(cth));
-    if (b.x + cth > tol) return false;
-    if (b.y + cth < -tol) return true;
-    if (i > G.n[0] - 2 || j > G.n[1] - 2 || k > G.n[2] - 2) return detect_value(G, in, px, py, pz) < 0.0f;
-    const float fx = (float)(c0 - (double)i), fy = (float)(c1 - (double)j), fz = (float)(c2 - (double)k);
-    const auto q = ATAC_GLOBAL(G.cells) + 2 * ((int64_t)i + (int64_t)(G.n[0] - 1) * ((int64_t)j + (int64_t)(G.n[1] - 1) * k));
-    const float4 lo = q[0], hi = q[1];
-    const float a = lerpf(lo.x, lo.y, fx), bb = lerpf(lo.z, lo.w, fx);
-    const float c = lerpf(hi.x, hi.y, fx), d = lerpf(hi.z, hi.w, fx);
-    return lerpf(lerpf(a, bb, fy), lerpf(c, d, fy), fz) + cth < 0.0f;
+    const int dec = block_decision(ATAC_GLOBAL(G.blk)[i / kBlk + G.nb[0] * (j / kBlk + G.nb[1] * (k / kBlk))],
+                                   (float)in.col_th);
+    if (dec >= 0) return dec == 1;
+    return detect_interp(G, in, px, py, pz, cc);
 }
 
 ATAC_HD bool detect_active(const GridDev& G, const InstDev& in, float px, float py, float pz) {
@@ -397,66 +453,167 @@ ATAC_HD bool fast_cell_test(const GridDev& G, const InstDev& in, const double c[
     return fast_cell_c(G, in, c, idx, fd);
 }
 
-// Detection of PPL points at once, the same decisions as detect_active (bit for
-// bit), with every grid read of a phase issued before any is used: the block
-// bounds of all inside points, then the packed cells of the points the bounds
-// leave undecided. Returns the active points (bit k).
+// 16 bytes at a 32-bit byte offset from a wave-uniform base (one scalar base
+// and a 32-bit vector offset: no 64-bit address arithmetic per lane).
+template <typename T>
+ATAC_HD float4 load_f4_at(const T* base, uint32_t byte_off) {
+    return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+template <typename T>
+ATAC_HD float2 load_f2_at(const T* base, uint32_t byte_off) {
+    return *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+__device__ __forceinline__ float4 load_f4_at(global_ptr<T> base, uint32_t byte_off) {
+    using G4 = const __attribute__((address_space(1))) float4*;
+    using GC = const __attribute__((address_space(1))) char*;
+    return *reinterpret_cast<G4>(reinterpret_cast<GC>(base) + byte_off);
+}
+template <typename T>
+__device__ __forceinline__ float2 load_f2_at(global_ptr<T> base, uint32_t byte_off) {
+    using G2 = const __attribute__((address_space(1))) float2*;
+    using GC = const __attribute__((address_space(1))) char*;
+    return *reinterpret_cast<G2>(reinterpret_cast<GC>(base) + byte_off);
+}
+#endif
+
+// Points a detection step left open, carried to the next step (bit k of
+// `mask`): byte offset of their corner-packed cell and their fractions; `slow`:
+// the FD stencil leaves the cell; `o`: offset of the step's first point (set by
+// the caller).
 template <int PPL>
-ATAC_HD unsigned detect_points(const GridDev& G, const InstDev& in, const float (&x)[PPL], const float (&y)[PPL],
-                               const float (&z)[PPL], unsigned valid, bool all_active) {
-    const float cth = (float)in.col_th;
-    unsigned inside = 0, interior = 0;
-    int cell[PPL];
+struct OpenPoints {
+    uint32_t cell[PPL];
     float fx[PPL], fy[PPL], fz[PPL];
-    float2 bb[PPL];
+    unsigned mask, slow;
+    int o;
+};
+
+// Per-tile constants of the detection step (wave-uniform).
+struct StepConsts {
+    double m[3];        // FD-stencil margin per axis (fast_cell_c)
+    int32_t last[3];    // n - 1: last node
+    float cth;          // col_th
+};
+
+ATAC_HD StepConsts step_consts(const GridDev& G, const InstDev& in) {
+    StepConsts k;
+    for (int a = 0; a < 3; ++a) {
+        k.m[a] = kFdEps * G.inv_d[a] + kCellMargin;
+        k.last[a] = G.n[a] - 1;
+    }
+    k.cth = (float)in.col_th;
+    return k;
+}
+
+// One detection step over PPL points of a grid with the cell table (G.fast),
+// software-pipelined: detect_active's decisions, bit for bit. Once the points'
+// grid coordinates are computed, the cell reads of the previous step's open
+// points (`open` on entry) go out, then the block-bound reads of this step's
+// inside points, then `issue()` (the caller's next loads, behind them, so the
+// one wait here covers both reads and nothing later). The previous step's open
+// points are then interpolated: the active ones go to prev_act (bit k), with
+// slow-path flags in prev_slow. Then the bounds are applied: points they settle
+// as active are returned (bit k), with `slow` the ones whose FD stencil leaves
+// their cell (fast_cell_test); the ones they leave open go to `open` for the
+// next step (or open_finish after the last). With all_active every valid point
+// is returned and no grid value is read.
+template <int PPL, typename F>
+ATAC_HD unsigned detect_step(const GridDev& G, const InstDev& in, const StepConsts& K, const float (&x)[PPL],
+                             const float (&y)[PPL], const float (&z)[PPL], unsigned valid, bool all_active,
+                             unsigned& slow, OpenPoints<PPL>& open, unsigned& prev_act, unsigned& prev_slow,
+                             F&& issue) {
+    unsigned inside = 0, off_cell = 0;
+    uint32_t boff[PPL], coff[PPL];
+    float fr[3][PPL];
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
         double c[3];
         grid_coords(in, x[k], y[k], z[k], c);
-        const bool ins = ((valid >> k) & 1u) && c[0] >= 0.0 && c[0] <= in.cmax[0] && c[1] >= 0.0 &&
-                         c[1] <= in.cmax[1] && c[2] >= 0.0 && c[2] <= in.cmax[2];
-        int i = ins ? (int)c[0] : 0, j = ins ? (int)c[1] : 0, kk = ins ? (int)c[2] : 0;
-        i = i < G.n[0] - 1 ? i : G.n[0] - 1;
-        j = j < G.n[1] - 1 ? j : G.n[1] - 1;
-        kk = kk < G.n[2] - 1 ? kk : G.n[2] - 1;
+        bool ins = ((valid >> k) & 1u) != 0u, fast = true;
+        uint32_t idx[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const bool in_a = c[a] >= 0.0 && c[a] <= in.cmax[a];
+            ins = ins && in_a;
+            int i = (int)(in_a ? c[a] : 0.0);   // inside: floor
+            i = i < K.last[a] ? i : K.last[a];  // clamped to the last node
+            const double f = c[a] - (double)i;
+            // fast_cell_c for an inside point: interior cell, stencil inside it and the box
+            fast = fast && i < K.last[a] && f >= K.m[a] && f <= 1.0 - K.m[a] && c[a] + K.m[a] <= in.cmax[a];
+            fr[a][k] = (float)f;
+            idx[a] = (uint32_t)i;
+        }
         if (ins) inside |= 1u << k;
-        if (ins && G.fast && i <= G.n[0] - 2 && j <= G.n[1] - 2 && kk <= G.n[2] - 2) interior |= 1u << k;
-        fx[k] = (float)(c[0] - (double)i);
-        fy[k] = (float)(c[1] - (double)j);
-        fz[k] = (float)(c[2] - (double)kk);
-        cell[k] = G.fast ? i + (G.n[0] - 1) * (j + (G.n[1] - 1) * kk) : 0;
-        bb[k] = make_float2(-INFINITY, INFINITY);
-        if (ins && G.fast) bb[k] = ATAC_GLOBAL(G.blk)[i / kBlk + G.nb[0] * (j / kBlk + G.nb[1] * (kk / kBlk))];
+        if (!fast) off_cell |= 1u << k;
+        boff[k] = ((idx[0] / kBlk) + (uint32_t)G.nb[0] * ((idx[1] / kBlk) + (uint32_t)G.nb[1] * (idx[2] / kBlk))) * 8u;
+        coff[k] = (idx[0] + (uint32_t)G.n[0] * (idx[1] + (uint32_t)G.n[1] * idx[2])) * 32u;
     }
-    unsigned settled_in = 0, undecided = 0;
+    pin(off_cell);   // the FP64 coordinates die here, before the wait
+    pin(inside);
+    sched_fence();   // the points are in: the reads below wait for nothing earlier
     float4 lo[PPL], hi[PPL];
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
-        const float tol = 4e-6f * (fmaxf(fabsf(bb[k].x), fabsf(bb[k].y)) + fabsf(cth));
-        if (!((inside >> k) & 1u)) continue;
-        if (bb[k].y + cth < -tol) settled_in |= 1u << k;
-        else if (!(bb[k].x + cth > tol)) undecided |= 1u << k;
-        if (((undecided & interior) >> k) & 1u) {
-            const auto q = ATAC_GLOBAL(G.cells) + 2 * (int64_t)cell[k];
-            lo[k] = q[0];
-            hi[k] = q[1];
+        if ((open.mask >> k) & 1u) {
+            lo[k] = load_f4_at(ATAC_GLOBAL(G.cells), open.cell[k]);
+            hi[k] = load_f4_at(ATAC_GLOBAL(G.cells), open.cell[k] + 16u);
         }
     }
-    unsigned act = all_active ? valid : settled_in;
+    float2 bb[PPL];
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
-        if (all_active || !((undecided >> k) & 1u)) continue;
-        bool a;
-        if ((interior >> k) & 1u) {
-            const float aa = lerpf(lo[k].x, lo[k].y, fx[k]), bq = lerpf(lo[k].z, lo[k].w, fx[k]);
-            const float cq = lerpf(hi[k].x, hi[k].y, fx[k]), dq = lerpf(hi[k].z, hi[k].w, fx[k]);
-            a = lerpf(lerpf(aa, bq, fy[k]), lerpf(cq, dq, fy[k]), fz[k]) + cth < 0.0f;
-        } else {
-            a = detect_value(G, in, x[k], y[k], z[k]) < 0.0f;
-        }
-        if (a) act |= 1u << k;
+        bb[k] = make_float2(0.0f, 0.0f);
+        if (((inside >> k) & 1u) && !all_active) bb[k] = load_f2_at(ATAC_GLOBAL(G.blk), boff[k]);
     }
+    issue();
+    unsigned pa = 0;
+#pragma unroll
+    for (int k = 0; k < PPL; ++k)
+        if (((open.mask >> k) & 1u) && cell_value(lo[k], hi[k], open.fx[k], open.fy[k], open.fz[k], K.cth) < 0.0f)
+            pa |= 1u << k;
+    prev_act = pa;
+    prev_slow = pa & open.slow;
+    unsigned act = 0, opn = 0;
+    if (all_active) {
+        act = valid;
+    } else {
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) {
+            if (!((inside >> k) & 1u)) continue;
+            const int dec = block_decision(bb[k], K.cth);
+            if (dec == 1) act |= 1u << k;
+            else if (dec < 0) opn |= 1u << k;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) {
+        open.cell[k] = coff[k];
+        open.fx[k] = fr[0][k];
+        open.fy[k] = fr[1][k];
+        open.fz[k] = fr[2][k];
+    }
+    open.mask = opn;
+    open.slow = off_cell;
+    slow = act & off_cell;
     return act;
+}
+
+// The open points of the last step (the same interpolation as detect_step).
+template <int PPL>
+ATAC_HD unsigned open_finish(const GridDev& G, const StepConsts& K, const OpenPoints<PPL>& open, unsigned& slow) {
+    unsigned pa = 0;
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) {
+        if ((open.mask >> k) & 1u) {
+            const float4 lo = load_f4_at(ATAC_GLOBAL(G.cells), open.cell[k]);
+            const float4 hi = load_f4_at(ATAC_GLOBAL(G.cells), open.cell[k] + 16u);
+            if (cell_value(lo, hi, open.fx[k], open.fy[k], open.fz[k], K.cth) < 0.0f) pa |= 1u << k;
+        }
+    }
+    slow = pa & open.slow;
+    return pa;
 }
 
 // Cell and fractions of grid coordinates c known to pass fast_cell_c (the

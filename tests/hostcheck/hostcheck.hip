@@ -32,8 +32,8 @@ static void host_grid(const float* vals, const int32_t n[3], const double minb[3
         G.inv_d[a] = 1.0 / d[a];
         G.nb[a] = (n[a] - 1) / kBlk + 1;
     }
-    if (n[0] >= 2 && n[1] >= 2 && n[2] >= 2) {
-        const int64_t ci = n[0] - 1, cj = n[1] - 1, ck = n[2] - 1;
+    {
+        const int64_t ci = n[0], cj = n[1], ck = n[2];
         hg.cells.resize(2 * ci * cj * ck);
         for (int64_t c = 0; c < ci * cj * ck; ++c)
             pack_cell(G, (int)(c % ci), (int)((c / ci) % cj), (int)(c / (ci * cj)), &hg.cells[2 * c]);
@@ -166,29 +166,41 @@ int hc_detect(const float* vals, const int32_t n[3], const double minb[3], const
 int hc_detect_decisions(const float* vals, const int32_t n[3], const double minb[3], const double maxb[3],
                         const double d[3], const float* px, const float* py, const float* pz, int64_t npts,
                         const atac_instance* inst, uint8_t* decided, uint8_t* full, uint8_t* settled,
-                        uint8_t* batched, uint8_t* slow) {
+                        uint8_t* batched, uint8_t* slow, uint8_t* slow_ref) {
     HostGrid hg;
     host_grid(vals, n, minb, maxb, d, hg);
     const GridDev& G = hg.G;
     InstDev in;
     std::memset(&in, 0, sizeof in);
     compose_instance(*inst, G, px, py, pz, npts, in);
-    // the detection kernel's 4-points-at-a-time form, ragged tail included
+    // the detection kernels: 4 points at a time through the block bounds, then
+    // the open points interpolated one by one (resolve_kernel), ragged tail included
     for (int64_t o0 = 0; o0 < npts; o0 += 4) {
         float x[4], y[4], z[4];
         unsigned valid = 0;
         for (int k = 0; k < 4; ++k) {
-            const int64_t o = o0 + k < npts ? o0 + k : npts - 1;
+            const int64_t o = o0 + k < npts ? o0 + k : 0;
             x[k] = px[o]; y[k] = py[o]; z[k] = pz[o];
             if (o0 + k < npts) valid |= 1u << k;
         }
-        const unsigned act = detect_points<4>(G, in, x, y, z, valid, false);
+        unsigned open, sl;
+        const unsigned act = detect_blocks<4>(G, in, x, y, z, valid, false, open, sl, [] {});
         for (int k = 0; k < 4 && o0 + k < npts; ++k) {
-            batched[o0 + k] = (act >> k) & 1u;
-            double cc[3];
-            grid_coords(in, x[k], y[k], z[k], cc);
-            slow[o0 + k] = ((act >> k) & 1u) && !fast_cell_test(G, in, cc);
+            bool a = (act >> k) & 1u, s = (sl >> k) & 1u;
+            if ((open >> k) & 1u) {
+                double cc[3];
+                grid_coords(in, x[k], y[k], z[k], cc);
+                a = detect_interp(G, in, x[k], y[k], z[k], cc);
+                s = a && !fast_cell_test(G, in, cc);
+            }
+            batched[o0 + k] = a ? 1 : 0;
+            slow[o0 + k] = s ? 1 : 0;
         }
+    }
+    for (int64_t o = 0; o < npts; ++o) {
+        double cc[3];
+        grid_coords(in, px[o], py[o], pz[o], cc);
+        slow_ref[o] = batched[o] && !fast_cell_test(G, in, cc);
     }
     for (int64_t o = 0; o < npts; ++o) {
         decided[o] = detect_active(G, in, px[o], py[o], pz[o]) ? 1 : 0;

@@ -81,7 +81,8 @@ def detect_values(grid, pts_f32, inst):
 
 def detect_decisions(grid, pts_f32, inst):
     """(bounded decision, full-interpolation decision, settled-by-block-bound, the
-    4-point batched decision, its slow-path flag) per point."""
+    4-point batched decision, its slow-path flag, that flag recomputed point by point
+    from fast_cell_test) per point."""
     vals, n, mn, mx, d = _grid_args(grid)
     p = np.ascontiguousarray(pts_f32, dtype=np.float32).reshape(-1, 3)
     px, py, pz = [np.ascontiguousarray(p[:, k]) for k in range(3)]
@@ -90,7 +91,9 @@ def detect_decisions(grid, pts_f32, inst):
     settled = np.zeros(len(p), dtype=np.uint8)
     batched = np.zeros(len(p), dtype=np.uint8)
     slow = np.zeros(len(p), dtype=np.uint8)
+    slow_ref = np.zeros(len(p), dtype=np.uint8)
     lib().hc_detect_decisions(_vp(vals), _vp(n), _vp(mn), _vp(mx), _vp(d), _vp(px), _vp(py), _vp(pz),
                               C.c_int64(len(p)), C.byref(inst), _vp(dec), _vp(full), _vp(settled), _vp(batched),
-                              _vp(slow))
-    return dec.astype(bool), full.astype(bool), settled.astype(bool), batched.astype(bool), slow.astype(bool)
+                              _vp(slow), _vp(slow_ref))
+    return (dec.astype(bool), full.astype(bool), settled.astype(bool), batched.astype(bool), slow.astype(bool),
+            slow_ref.astype(bool))

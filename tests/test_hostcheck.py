@@ -138,9 +138,10 @@ def test_slow_path_near_faces_and_box():
 
 
 def _decisions_agree(grid, pts, inst):
-    dec, full, settled, batched, slow = H.detect_decisions(grid, pts, inst)
+    dec, full, settled, batched, slow, slow_ref = H.detect_decisions(grid, pts, inst)
     assert np.array_equal(dec, full), f"{int((dec != full).sum())} bounded decisions differ from the full ones"
     assert np.array_equal(batched, full), f"{int((batched != full).sum())} batched decisions differ"
+    assert np.array_equal(slow, slow_ref), f"{int((slow != slow_ref).sum())} slow-path flags differ"
     assert not np.any(slow & ~full)
     return settled
 
