@@ -391,7 +391,7 @@ static int ensure_buffers(atac_batch b, int mode, bool precise) {
     return rc;
 }
 
-// LDS of the detection kernel: one u16 offset segment per wavefront.
+// Points of a tile per wavefront of the detection kernel.
 static int wave_span_of(int tile_points, int block, int ppl) {
     const int nw = block / 64, step = 64 * ppl;
     return ((tile_points + nw - 1) / nw + step - 1) / step * step;
@@ -404,14 +404,15 @@ static int plan(atac_batch b, const atac_tuning* t) {
     if (tu.points_per_lane <= 0) tu.points_per_lane = 4;
     if (tu.tile_points > 32768 || tu.tile_points % 64 != 0)
         return fail(ATAC_ERR_INVALID, "tile_points must be a multiple of 64 and <= 32768");
-    if (tu.detect_block != 128 && tu.detect_block != 256 && tu.detect_block != 512 && tu.detect_block != 1024)
-        return fail(ATAC_ERR_INVALID, "detect_block must be 128, 256, 512 or 1024");
+    if (tu.detect_block != 64 && tu.detect_block != 128 && tu.detect_block != 256 && tu.detect_block != 512 &&
+        tu.detect_block != 1024)
+        return fail(ATAC_ERR_INVALID, "detect_block must be 64, 128, 256, 512 or 1024");
     if (tu.points_per_lane != 1 && tu.points_per_lane != 2 && tu.points_per_lane != 4)
         return fail(ATAC_ERR_INVALID, "points_per_lane must be 1, 2 or 4");
     if (tu.blocks_per_cu < 0 || tu.order < 0 || tu.order > 2) return fail(ATAC_ERR_INVALID, "bad tuning");
     const int B = tu.detect_block;
     const int span = wave_span_of(tu.tile_points, B, tu.points_per_lane);
-    const size_t lds = (size_t)span * (B / 64) * sizeof(uint16_t);
+    const size_t lds = detect_lds_per_wave(span, tu.points_per_lane) * (B / 64);
     if (lds + 1024 > b->lds_limit)
         return fail(ATAC_ERR_INVALID, "tile_points " + std::to_string(tu.tile_points) + " with detect_block " +
                                           std::to_string(B) + " needs " + std::to_string(lds) +

@@ -13,6 +13,16 @@ struct QueueDev {
     int32_t pad;
 };
 
+// Candidate buffer of a detection wavefront (entries x, y, z, tile offset |
+// open << 15): room for what one step adds (<= 64 PPL) on top of < 64 waiting.
+__host__ __device__ constexpr int detect_ring_cap(int ppl) { return 64 + 64 * ppl; }
+
+// LDS of the detection kernel per wavefront: the output segment (span u16
+// offsets) and the candidate buffer (16 B entries).
+__host__ __device__ constexpr size_t detect_lds_per_wave(int span, int ppl) {
+    return (size_t)span * 2 + (size_t)detect_ring_cap(ppl) * 16;
+}
+
 struct DetectArgs {
     QueueDev q;                 // tiles
     const TileDev* tiles;

@@ -151,23 +151,30 @@ __device__ __forceinline__ void load_points(P px, P py, P pz, int o, int end, fl
 }
 
 // Points of one tile per work item. The tile is streamed: PPL consecutive
-// points per lane (vector loads, each wavefront a contiguous span of the tile),
-// in software-pipelined steps (detect_step): a step's reads are the cells of the
-// previous step's open points, the block bounds of its own points and the next
-// step's points, issued in that order, so each step waits once, for reads
-// issued in the same step. The active offsets (bit 15: slow path; tiles hold at
-// most 32768 points) are compacted into per-wave LDS segments (a wave-wide
-// exclusive prefix of the per-lane counts built from ballots of their bits),
-// copied to the tile's slots of the active list, and the tile pushes its work
-// items for the record, contact and adjoint passes.
+// points per lane (vector loads, each wavefront a contiguous span of the tile;
+// the next points are loaded behind the block-bound reads of the current ones,
+// so a step waits once), and only the block bounds are read (detect_candidates).
+// The candidates a step finds (settled active or open; a few per cent, near the
+// surface) are appended, with their coordinates, to the wavefront's LDS buffer in
+// ascending order (a wave-wide exclusive prefix of the per-lane counts built from
+// ballots of their bits); whenever 64 are waiting, the wavefront finishes them
+// one per lane (candidate_active: cell read and interpolation of the open ones,
+// FD-stencil test) instead of every stream step branching into that work for a
+// few lanes. The active offsets (bit 15: slow path; tiles hold at most 32768
+// points) go to a per-wave LDS segment, then to the tile's slots of the active
+// list, and the tile pushes its work items for the record, contact and adjoint
+// passes.
 template <int BLOCK, int PPL>
 __global__ __launch_bounds__(BLOCK) void detect_kernel(DetectArgs a) {
     constexpr int NW = BLOCK / 64;
-    constexpr int NB = PPL == 1 ? 2 : (PPL == 2 ? 3 : 4);   // bits of a per-lane count <= 2 PPL
-    extern __shared__ uint16_t s_queue[];   // [NW][span] active offsets, bit 15 = slow path
+    constexpr int NB = PPL == 1 ? 1 : (PPL == 2 ? 2 : 3);   // bits of a per-lane count <= PPL
+    constexpr int CAP = detect_ring_cap(PPL);
+    extern __shared__ __align__(16) unsigned char s_lds[];
     __shared__ int s_wcount[NW];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int span = a.wave_span;
+    uint16_t* s_queue = reinterpret_cast<uint16_t*>(s_lds);   // [NW][span] active offsets, bit 15 = slow path
+    float4* cand = reinterpret_cast<float4*>(s_lds + (size_t)NW * span * 2) + w * CAP;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
     const bool all_active = a.all_active != 0;
     const auto order = as_const(a.q.order);
@@ -177,36 +184,25 @@ __global__ __launch_bounds__(BLOCK) void detect_kernel(DetectArgs a) {
         const TileDev t = as_const(a.tiles)[tile];
         const InstDev in = as_const(a.inst)[t.inst];
         const GridDev G = as_const(a.grids)[in.grid];
-        const StepConsts K = step_consts(G, in);
         const auto px = as_global(in.px) + t.begin;
         const auto py = as_global(in.py) + t.begin;
         const auto pz = as_global(in.pz) + t.begin;
         uint16_t* q = s_queue + w * span;
         const int wb = w * span, we = min(t.count, wb + span);
-        int wcount = 0;
-        // appends points oa + k (bits ma, slow bits sa), then ob + k (mb, sb), in
-        // lane order
-        auto emit = [&](unsigned ma, int oa, unsigned sa, unsigned mb, int ob, unsigned sb) {
-            const int c = __popc(ma) + __popc(mb);
-            int pre = 0, tot = 0;
-#pragma unroll
-            for (int bit = 0; bit < NB; ++bit) {
-                const uint64_t bb = __ballot((c >> bit) & 1);
-                pre += __popcll(bb & lt_mask) << bit;
-                tot += __popcll(bb) << bit;
+        int wcount = 0, ncand = 0;   // wave-uniform
+        // finishes candidates [b, b + n), n <= 64, one per lane, into the output segment
+        auto finish = [&](int b, int n) {
+            bool act = false, slow = false;
+            uint32_t tag = 0;
+            if (lane < n) {
+                const float4 e = cand[b + lane];
+                tag = __float_as_uint(e.w);
+                act = candidate_active(G, in, e.x, e.y, e.z, (tag >> 15) != 0u, slow);
             }
-            int pos = wcount + pre;
-#pragma unroll
-            for (int k = 0; k < PPL; ++k)
-                if ((ma >> k) & 1u) q[pos++] = (uint16_t)((oa + k) | (((sa >> k) & 1u) << 15));
-#pragma unroll
-            for (int k = 0; k < PPL; ++k)
-                if ((mb >> k) & 1u) q[pos++] = (uint16_t)((ob + k) | (((sb >> k) & 1u) << 15));
-            wcount += tot;
+            const uint64_t bal = __ballot(act);
+            if (act) q[wcount + __popcll(bal & lt_mask)] = (uint16_t)((tag & 0x7fffu) | (slow ? 0x8000u : 0u));
+            wcount += __popcll(bal);
         };
-        OpenPoints<PPL> op;
-        op.mask = 0u;
-        op.o = 0;
         // one step: points (cx, cy, cz) at o0 are tested while the next ones load
         // into (nx, ny, nz); the loop alternates the two buffers, so no register
         // copy waits for the loads in flight
@@ -217,15 +213,37 @@ __global__ __launch_bounds__(BLOCK) void detect_kernel(DetectArgs a) {
 #pragma unroll
             for (int k = 0; k < PPL; ++k)
                 if (o + k < we) valid |= 1u << k;
-            const int po = op.o;
-            unsigned sl, pm, psl;
-            const unsigned m = detect_step<PPL>(G, in, K, cx, cy, cz, valid, all_active, sl, op, pm, psl, [&]() {
+            unsigned op;
+            const unsigned m = detect_candidates<PPL>(G, in, cx, cy, cz, valid, all_active, op, [] {}, [&]() {
                 __builtin_amdgcn_sched_barrier(0);
                 load_points<PPL>(px, py, pz, o + 64 * PPL, we, nx, ny, nz);
                 __builtin_amdgcn_sched_barrier(0);
             });
-            op.o = o;
-            emit(pm, po, psl, m, o, sl);
+            const int c = __popc(m);
+            int pre = 0, tot = 0;
+#pragma unroll
+            for (int bit = 0; bit < NB; ++bit) {
+                const uint64_t bb = __ballot((c >> bit) & 1);
+                pre += __popcll(bb & lt_mask) << bit;
+                tot += __popcll(bb) << bit;
+            }
+            int pos = ncand + pre;
+#pragma unroll
+            for (int k = 0; k < PPL; ++k)
+                if ((m >> k) & 1u)
+                    cand[pos++] =
+                        make_float4(cx[k], cy[k], cz[k], __uint_as_float((uint32_t)(o + k) | (((op >> k) & 1u) << 15)));
+            ncand += tot;
+            if (ncand >= 64) {
+                int b = 0;
+                for (; ncand - b >= 64; b += 64) finish(b, 64);
+                // the < 64 left move to the front, one per lane (b >= 64 > left: no overlap)
+                const int left = ncand - b;
+                float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (lane < left) e = cand[b + lane];
+                if (lane < left) cand[lane] = e;
+                ncand = left;
+            }
         };
         float x0[PPL], y0[PPL], z0[PPL], x1[PPL], y1[PPL], z1[PPL];
         load_points<PPL>(px, py, pz, wb + lane * PPL, we, x0, y0, z0);
@@ -234,11 +252,7 @@ __global__ __launch_bounds__(BLOCK) void detect_kernel(DetectArgs a) {
             if (o0 + 64 * PPL >= we) break;
             step(o0 + 64 * PPL, x1, y1, z1, x0, y0, z0);
         }
-        {   // the open points of the last step
-            unsigned psl;
-            const unsigned pm = open_finish<PPL>(G, K, op, psl);
-            emit(pm, op.o, psl, 0u, 0, 0u);
-        }
+        if (ncand > 0) finish(0, ncand);
         if (lane == 0) s_wcount[w] = wcount;
         __syncthreads();
         int cnt = 0;
@@ -715,6 +729,7 @@ static const void* detect_ptr_b(int ppl) {
 const void* detect_kernel_ptr(int block, int ppl) {
     if (ppl != 1 && ppl != 2 && ppl != 4) return nullptr;
     switch (block) {
+        case 64: return detect_ptr_b<64>(ppl);
         case 128: return detect_ptr_b<128>(ppl);
         case 256: return detect_ptr_b<256>(ppl);
         case 512: return detect_ptr_b<512>(ppl);

@@ -478,142 +478,109 @@ __device__ __forceinline__ float2 load_f2_at(global_ptr<T> base, uint32_t byte_o
 }
 #endif
 
-// Points a detection step left open, carried to the next step (bit k of
-// `mask`): byte offset of their corner-packed cell and their fractions; `slow`:
-// the FD stencil leaves the cell; `o`: offset of the step's first point (set by
-// the caller).
-template <int PPL>
-struct OpenPoints {
-    uint32_t cell[PPL];
-    float fx[PPL], fy[PPL], fz[PPL];
-    unsigned mask, slow;
-    int o;
-};
-
-// Per-tile constants of the detection step (wave-uniform).
-struct StepConsts {
-    double m[3];        // FD-stencil margin per axis (fast_cell_c)
-    int32_t last[3];    // n - 1: last node
-    float cth;          // col_th
-};
-
-ATAC_HD StepConsts step_consts(const GridDev& G, const InstDev& in) {
-    StepConsts k;
-    for (int a = 0; a < 3; ++a) {
-        k.m[a] = kFdEps * G.inv_d[a] + kCellMargin;
-        k.last[a] = G.n[a] - 1;
-    }
-    k.cth = (float)in.col_th;
-    return k;
-}
-
-// One detection step over PPL points of a grid with the cell table (G.fast),
-// software-pipelined: detect_active's decisions, bit for bit. Once the points'
-// grid coordinates are computed, the cell reads of the previous step's open
-// points (`open` on entry) go out, then the block-bound reads of this step's
-// inside points, then `issue()` (the caller's next loads, behind them, so the
-// one wait here covers both reads and nothing later). The previous step's open
-// points are then interpolated: the active ones go to prev_act (bit k), with
-// slow-path flags in prev_slow. Then the bounds are applied: points they settle
-// as active are returned (bit k), with `slow` the ones whose FD stencil leaves
-// their cell (fast_cell_test); the ones they leave open go to `open` for the
-// next step (or open_finish after the last). With all_active every valid point
-// is returned and no grid value is read.
-template <int PPL, typename F>
-ATAC_HD unsigned detect_step(const GridDev& G, const InstDev& in, const StepConsts& K, const float (&x)[PPL],
-                             const float (&y)[PPL], const float (&z)[PPL], unsigned valid, bool all_active,
-                             unsigned& slow, OpenPoints<PPL>& open, unsigned& prev_act, unsigned& prev_slow,
-                             F&& issue) {
-    unsigned inside = 0, off_cell = 0;
-    uint32_t boff[PPL], coff[PPL];
-    float fr[3][PPL];
+// Stream pass of the detection of PPL points (detect_active without the cell
+// reads): once the points' coordinates are computed, `before()` runs (the caller
+// issues reads that can share this step's wait), the block-bound reads of the
+// inside points are issued, then `issue()` (the caller's next loads go out
+// behind them, so waiting for the bounds does not wait for them), then the
+// bounds are applied. Returns the candidates (bit
+// k): the points they settle as active and the ones they leave open (also in
+// `open`); candidate_active finishes both. Only what every point needs is done
+// here, so a wavefront's lanes stay busy: the per-candidate work runs later,
+// densely, one candidate per lane. With all_active every valid point is a
+// settled candidate and nothing is read.
+template <int PPL, typename F0, typename F>
+ATAC_HD unsigned detect_candidates(const GridDev& G, const InstDev& in, const float (&x)[PPL], const float (&y)[PPL],
+                                   const float (&z)[PPL], unsigned valid, bool all_active, unsigned& open,
+                                   F0&& before, F&& issue) {
+    const float cth = (float)in.col_th;
+    unsigned inside = 0;
+    uint32_t boff[PPL];
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
         double c[3];
         grid_coords(in, x[k], y[k], z[k], c);
-        bool ins = ((valid >> k) & 1u) != 0u, fast = true;
+        bool ins = ((valid >> k) & 1u) != 0u;
         uint32_t idx[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             const bool in_a = c[a] >= 0.0 && c[a] <= in.cmax[a];
             ins = ins && in_a;
-            int i = (int)(in_a ? c[a] : 0.0);   // inside: floor
-            i = i < K.last[a] ? i : K.last[a];  // clamped to the last node
-            const double f = c[a] - (double)i;
-            // fast_cell_c for an inside point: interior cell, stencil inside it and the box
-            fast = fast && i < K.last[a] && f >= K.m[a] && f <= 1.0 - K.m[a] && c[a] + K.m[a] <= in.cmax[a];
-            fr[a][k] = (float)f;
-            idx[a] = (uint32_t)i;
+            const int i = (int)(in_a ? c[a] : 0.0);   // inside: floor
+            idx[a] = (uint32_t)(i < G.n[a] - 1 ? i : G.n[a] - 1);
         }
         if (ins) inside |= 1u << k;
-        if (!fast) off_cell |= 1u << k;
         boff[k] = ((idx[0] / kBlk) + (uint32_t)G.nb[0] * ((idx[1] / kBlk) + (uint32_t)G.nb[1] * (idx[2] / kBlk))) * 8u;
-        coff[k] = (idx[0] + (uint32_t)G.n[0] * (idx[1] + (uint32_t)G.n[1] * idx[2])) * 32u;
     }
-    pin(off_cell);   // the FP64 coordinates die here, before the wait
     pin(inside);
     sched_fence();   // the points are in: the reads below wait for nothing earlier
-    float4 lo[PPL], hi[PPL];
-#pragma unroll
-    for (int k = 0; k < PPL; ++k) {
-        if ((open.mask >> k) & 1u) {
-            lo[k] = load_f4_at(ATAC_GLOBAL(G.cells), open.cell[k]);
-            hi[k] = load_f4_at(ATAC_GLOBAL(G.cells), open.cell[k] + 16u);
-        }
-    }
+    before();
     float2 bb[PPL];
+    const bool use_blk = G.fast && !all_active;
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
-        bb[k] = make_float2(0.0f, 0.0f);
-        if (((inside >> k) & 1u) && !all_active) bb[k] = load_f2_at(ATAC_GLOBAL(G.blk), boff[k]);
+        bb[k] = make_float2(-INFINITY, INFINITY);   // no table: the point stays open
+        if (((inside >> k) & 1u) && use_blk) bb[k] = load_f2_at(ATAC_GLOBAL(G.blk), boff[k]);
     }
     issue();
-    unsigned pa = 0;
-#pragma unroll
-    for (int k = 0; k < PPL; ++k)
-        if (((open.mask >> k) & 1u) && cell_value(lo[k], hi[k], open.fx[k], open.fy[k], open.fz[k], K.cth) < 0.0f)
-            pa |= 1u << k;
-    prev_act = pa;
-    prev_slow = pa & open.slow;
-    unsigned act = 0, opn = 0;
     if (all_active) {
-        act = valid;
-    } else {
-#pragma unroll
-        for (int k = 0; k < PPL; ++k) {
-            if (!((inside >> k) & 1u)) continue;
-            const int dec = block_decision(bb[k], K.cth);
-            if (dec == 1) act |= 1u << k;
-            else if (dec < 0) opn |= 1u << k;
-        }
+        open = 0u;
+        return valid;
     }
+    unsigned act = 0, opn = 0;
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
-        open.cell[k] = coff[k];
-        open.fx[k] = fr[0][k];
-        open.fy[k] = fr[1][k];
-        open.fz[k] = fr[2][k];
+        if (!((inside >> k) & 1u)) continue;
+        const int dec = block_decision(bb[k], cth);
+        if (dec == 1) act |= 1u << k;
+        else if (dec < 0) opn |= 1u << k;
     }
-    open.mask = opn;
-    open.slow = off_cell;
-    slow = act & off_cell;
+    open = opn;
+    return act | opn;
+}
+
+// A candidate of detect_candidates on a grid with the cell table, in two
+// halves so its cell read can share a wait: candidate_read computes the
+// coordinates and the FD-stencil test and issues the cell read of an open
+// candidate; candidate_finish decides it: active (settled, or open and
+// interpolated from its cell as detect_value does), with `slow` set if it is
+// active and its FD stencil leaves the cell (!fast_cell_test).
+struct CandRead {
+    float4 lo, hi;
+    float f[3];
+    bool open, off_cell;
+};
+
+ATAC_HD void candidate_read(const GridDev& G, const InstDev& in, float x, float y, float z, bool open, CandRead& r) {
+    double c[3];
+    grid_coords(in, x, y, z, c);
+    r.off_cell = !fast_cell_test(G, in, c);
+    r.open = open;
+    if (open) {   // an open candidate is inside the box; its cell (clamped at the last node) is in the table
+        uint32_t idx[3];
+        for (int a = 0; a < 3; ++a) {
+            int i = (int)c[a];
+            i = i < G.n[a] - 1 ? i : G.n[a] - 1;
+            idx[a] = (uint32_t)i;
+            r.f[a] = (float)(c[a] - (double)i);
+        }
+        const uint32_t off = (idx[0] + (uint32_t)G.n[0] * (idx[1] + (uint32_t)G.n[1] * idx[2])) * 32u;
+        r.lo = load_f4_at(ATAC_GLOBAL(G.cells), off);
+        r.hi = load_f4_at(ATAC_GLOBAL(G.cells), off + 16u);
+    }
+}
+
+ATAC_HD bool candidate_finish(const CandRead& r, float cth, bool& slow) {
+    const bool act = r.open ? cell_value(r.lo, r.hi, r.f[0], r.f[1], r.f[2], cth) < 0.0f : true;
+    slow = act && r.off_cell;
     return act;
 }
 
-// The open points of the last step (the same interpolation as detect_step).
-template <int PPL>
-ATAC_HD unsigned open_finish(const GridDev& G, const StepConsts& K, const OpenPoints<PPL>& open, unsigned& slow) {
-    unsigned pa = 0;
-#pragma unroll
-    for (int k = 0; k < PPL; ++k) {
-        if ((open.mask >> k) & 1u) {
-            const float4 lo = load_f4_at(ATAC_GLOBAL(G.cells), open.cell[k]);
-            const float4 hi = load_f4_at(ATAC_GLOBAL(G.cells), open.cell[k] + 16u);
-            if (cell_value(lo, hi, open.fx[k], open.fy[k], open.fz[k], K.cth) < 0.0f) pa |= 1u << k;
-        }
-    }
-    slow = pa & open.slow;
-    return pa;
+ATAC_HD bool candidate_active(const GridDev& G, const InstDev& in, float x, float y, float z, bool open,
+                              bool& slow) {
+    CandRead r;
+    candidate_read(G, in, x, y, z, open, r);
+    return candidate_finish(r, (float)in.col_th, slow);
 }
 
 // Cell and fractions of grid coordinates c known to pass fast_cell_c (the

@@ -53,7 +53,7 @@ typedef struct {
 
 typedef struct {
     int32_t tile_points;   /* points per tile (0: 4096); a multiple of 64, at most 32768 */
-    int32_t detect_block;  /* threads per detection workgroup: 128, 256, 512, 1024 (0: 256) */
+    int32_t detect_block;  /* threads per detection workgroup: 64, 128, 256, 512, 1024 (0: 256) */
     int32_t blocks_per_cu; /* detection: 0 = one workgroup per tile (default); n > 0 = n persistent
                               workgroups per CU (at most the occupancy limit) walking the tiles */
     int32_t order;         /* work queues: 0 one per XCD, items grouped by grid (default);

@@ -173,8 +173,9 @@ int hc_detect_decisions(const float* vals, const int32_t n[3], const double minb
     InstDev in;
     std::memset(&in, 0, sizeof in);
     compose_instance(*inst, G, px, py, pz, npts, in);
-    // the detection kernels: 4 points at a time through the block bounds, then
-    // the open points interpolated one by one (resolve_kernel), ragged tail included
+    // the detection kernel's passes: 4 points at a time through the block bounds
+    // (detect_candidates), then each candidate finished (candidate_active),
+    // ragged tail included
     for (int64_t o0 = 0; o0 < npts; o0 += 4) {
         float x[4], y[4], z[4];
         unsigned valid = 0;
@@ -183,16 +184,11 @@ int hc_detect_decisions(const float* vals, const int32_t n[3], const double minb
             x[k] = px[o]; y[k] = py[o]; z[k] = pz[o];
             if (o0 + k < npts) valid |= 1u << k;
         }
-        unsigned open, sl;
-        const unsigned act = detect_blocks<4>(G, in, x, y, z, valid, false, open, sl, [] {});
+        unsigned open;
+        const unsigned cand = detect_candidates<4>(G, in, x, y, z, valid, false, open, [] {}, [] {});
         for (int k = 0; k < 4 && o0 + k < npts; ++k) {
-            bool a = (act >> k) & 1u, s = (sl >> k) & 1u;
-            if ((open >> k) & 1u) {
-                double cc[3];
-                grid_coords(in, x[k], y[k], z[k], cc);
-                a = detect_interp(G, in, x[k], y[k], z[k], cc);
-                s = a && !fast_cell_test(G, in, cc);
-            }
+            bool a = false, s = false;
+            if ((cand >> k) & 1u) a = candidate_active(G, in, x[k], y[k], z[k], ((open >> k) & 1u) != 0u, s);
             batched[o0 + k] = a ? 1 : 0;
             slow[o0 + k] = s ? 1 : 0;
         }

@@ -16,7 +16,10 @@ def build():
            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
            "-I" + os.path.join(_REPO, "assemble-them-all_amd", "csrc"), "-I" + os.path.join(_REPO, "include"),
            os.path.join(_HERE, "hostcheck.hip"), "-o", _LIB]
-    subprocess.run(cmd, check=True, stderr=subprocess.DEVNULL)
+    r = subprocess.run(cmd, stderr=subprocess.PIPE, text=True)
+    if r.returncode != 0:
+        errs = [ln for ln in r.stderr.splitlines() if "packed-fp32-ops" not in ln]
+        raise RuntimeError("building libhostcheck.so failed:\n" + "\n".join(errs[-40:]))
 
 
 _lib = None
