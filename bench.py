@@ -188,11 +188,11 @@ def main():
                  + ACT_BYTES * n_active + CHUNK_BYTES * n_chunks_lb)
     det_ms = stage_ms["detect"]
     achieved = det_bytes / (det_ms * 1e-3) / 1e9
-    # the step: detection; the record pass reads the active list and points and
-    # writes one record per active point; the contact and adjoint passes each read
-    # the records and write one partial row per work item; the reductions write fm,
-    # read g_fm and write the 26 gradients per instance
-    step_bytes = (det_bytes + (ACT_BYTES + 12 + RECORD_BYTES) * n_active + 2 * RECORD_BYTES * n_active
+    # the step: detection; the contact pass reads the active list and points and
+    # writes one record per active point; the adjoint pass reads the records; both
+    # write one partial row per work item; the reductions write fm, read g_fm and
+    # write the 26 gradients per instance
+    step_bytes = (det_bytes + (ACT_BYTES + 12 + RECORD_BYTES) * n_active + RECORD_BYTES * n_active
                   + 8 * (12 + 26) * n_chunks_lb + 8 * 12 * b.n * 2 + 8 * 26 * b.n)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
@@ -232,8 +232,8 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": w.name, "instances": b.n, "queries": n_q, "grids": f"{len(w.grids)} x "
                    f"{'x'.join(str(v) for v in w.grids[0].n)} FP32", "active_fraction": round(n_active / n_q, 5),
-                   "step": "detect + records + slow-path samples + contact wrench + reduce + adjoint (VJP) "
-                           "+ reduce, HIP graph replay",
+                   "step": "detect + slow-path records + contact wrench (records built inline) + reduce "
+                           "+ adjoint (VJP) + reduce, HIP graph replay",
                    "tile_points": info["tile_points"], "detect_block": info["detect_block"],
                    "points_per_lane": info["points_per_lane"],
                    "persistent_blocks": info["persistent_blocks"], "n_cu": n_cu, "parallelism": "none"},
