@@ -29,6 +29,7 @@ EXPORTED = [
     "atac_batch_active_ids", "atac_batch_touched_values", "atac_eval_plain", "atac_batch_timing",
     "atac_sdf_distance", "atac_sdf_collision",
     "atac_batch_eval_host", "atac_batch_vjp_host", "atac_sdf_distance_host", "atac_sdf_collision_host",
+    "atac_sdf_from_mesh", "atac_sdf_mesh_grid",
 ]
 
 
@@ -106,6 +107,9 @@ def lib():
     L.atac_batch_vjp_host.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
     L.atac_sdf_distance_host.argtypes = [vp, dp, C.c_double, vp, i64, vp]
     L.atac_sdf_collision_host.argtypes = [vp, dp, dp, C.c_double, u32, vp, vp, i64, vp, vp]
+    fp = C.POINTER(C.c_float)
+    L.atac_sdf_from_mesh.argtypes = [vp, i64, vp, i64, fp, C.c_float, C.c_float, C.c_float, i32, i32, i32, i32, vp]
+    L.atac_sdf_mesh_grid.argtypes = [vp, i64, C.c_double, i32, fp, fp, dp, C.POINTER(i32)]
     for name in EXPORTED:
         if name not in ("atac_last_error",):
             getattr(L, name).restype = C.c_int

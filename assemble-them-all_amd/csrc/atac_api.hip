@@ -11,6 +11,7 @@
 
 #include "atac.h"
 #include "atac_compose.h"
+#include "sdf_build.h"
 #include "atac_kernels.h"
 #include "atac_types.h"
 
@@ -975,3 +976,88 @@ int atac_sdf_collision_host(atac_grid g, const double E2[16], const double phi2[
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ mesh -> SDF grid
+int atac_sdf_from_mesh(const float* x, int64_t nv, const uint32_t* tri, int64_t nf, const float origin[3], float dx,
+                       float dy, float dz, int32_t ni, int32_t nj, int32_t nk, int32_t exact_band, float* phi) {
+    if (!origin || !phi || nv < 0 || nf < 0 || (nv > 0 && !x) || (nf > 0 && !tri))
+        return fail(ATAC_ERR_INVALID, "atac_sdf_from_mesh: null argument");
+    if (ni < 1 || nj < 1 || nk < 1 || exact_band < 0 || !(dx > 0.0f) || !(dy > 0.0f) || !(dz > 0.0f))
+        return fail(ATAC_ERR_INVALID, "atac_sdf_from_mesh: bad grid");
+    const int64_t n = (int64_t)ni * nj * nk;
+    if (n >= ((int64_t)1 << 31) || nf >= ((int64_t)1 << 32))
+        return fail(ATAC_ERR_INVALID, "atac_sdf_from_mesh: grids hold fewer than 2^31 nodes and meshes fewer than 2^32 triangles");
+    for (int64_t e = 0; e < 3 * nf; ++e)
+        if ((int64_t)tri[e] >= nv) return fail(ATAC_ERR_INVALID, "atac_sdf_from_mesh: triangle index out of range");
+    LsArgs a{};
+    a.nf = nf;
+    a.o[0] = origin[0]; a.o[1] = origin[1]; a.o[2] = origin[2];
+    a.dx = dx; a.dy = dy; a.dz = dz;
+    a.ni = ni; a.nj = nj; a.nk = nk;
+    a.band = exact_band;
+    a.phi0 = (float)(ni + nj + nk) * (dx + dy + dz);   // upper bound on any distance
+    float* d_x = nullptr;
+    uint32_t* d_tri = nullptr;
+    unsigned long long* d_best = nullptr;
+    float* d_phi = nullptr;
+    int32_t *d_closest = nullptr, *d_count = nullptr;
+    int rc = dalloc(&d_x, (size_t)std::max<int64_t>(3 * nv, 1));
+    if (!rc) rc = dalloc(&d_tri, (size_t)std::max<int64_t>(3 * nf, 1));
+    if (!rc) rc = dalloc(&d_best, (size_t)n);
+    if (!rc) rc = dalloc(&d_phi, (size_t)n);
+    if (!rc) rc = dalloc(&d_closest, (size_t)n);
+    if (!rc) rc = dalloc(&d_count, (size_t)n);
+    hipError_t e = hipSuccess;
+    if (!rc && nv > 0) e = hipMemcpy(d_x, x, sizeof(float) * 3 * nv, hipMemcpyHostToDevice);
+    if (!rc && e == hipSuccess && nf > 0) e = hipMemcpy(d_tri, tri, sizeof(uint32_t) * 3 * nf, hipMemcpyHostToDevice);
+    if (!rc && e == hipSuccess) e = hipMemset(d_best, 0xff, sizeof(unsigned long long) * n);
+    if (!rc && e == hipSuccess) e = hipMemset(d_count, 0, sizeof(int32_t) * n);
+    if (!rc && e == hipSuccess) {
+        a.x = d_x; a.tri = d_tri; a.best = d_best; a.phi = d_phi; a.closest = d_closest; a.count = d_count;
+        e = ls_build(a, nullptr);
+        if (e == hipSuccess) e = hipMemcpy(phi, d_phi, sizeof(float) * n, hipMemcpyDeviceToHost);
+    }
+    dfree(d_x); dfree(d_tri); dfree(d_best); dfree(d_phi); dfree(d_closest); dfree(d_count);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(ATAC_ERR_HIP, hipGetErrorString(e));
+    return ATAC_OK;
+}
+
+int atac_sdf_mesh_grid(const double* V, int64_t nv, double dx, int32_t min_size, float min_box[3], float max_box[3],
+                       double d[3], int32_t n[3]) {
+    if (!V || nv < 1 || min_size < 1 || !(dx > 0.0) || !min_box || !max_box || !d || !n)
+        return fail(ATAC_ERR_INVALID, "atac_sdf_mesh_grid: bad arguments");
+    // FP32 bounds of the FP32 vertices (sdfgen::update_minmax), FP64 bounds for the spacing
+    float lo[3], hi[3];
+    double blo[3], bhi[3];
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = 3.402823466e+38f;
+        hi[a] = -3.402823466e+38f;
+        blo[a] = bhi[a] = V[a];
+    }
+    for (int64_t i = 0; i < nv; ++i)
+        for (int a = 0; a < 3; ++a) {
+            const double v = V[3 * i + a];
+            const float f = (float)v;
+            if (f < lo[a]) lo[a] = f;
+            if (f > hi[a]) hi[a] = f;
+            if (v < blo[a]) blo[a] = v;
+            if (v > bhi[a]) bhi[a] = v;
+        }
+    for (int a = 0; a < 3; ++a) {
+        const double s = (bhi[a] - blo[a]) / min_size;
+        d[a] = s < dx ? s : dx;
+    }
+    for (int a = 0; a < 3; ++a) {
+        const float unit = (float)d[a];
+        lo[a] -= 1.0f * unit;   // one cell of padding
+        hi[a] += 1.0f * unit;
+        const float ext = hi[a] - lo[a] + (float)(0.5 * d[a]);
+        const float cells = ext / (float)d[a];
+        if (!(cells >= 0.0f && cells < 2147483000.0f)) return fail(ATAC_ERR_INVALID, "atac_sdf_mesh_grid: bad extent");
+        n[a] = (int32_t)(uint32_t)cells + 1;
+        min_box[a] = lo[a];
+        max_box[a] = hi[a];
+    }
+    return ATAC_OK;
+}

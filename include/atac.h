@@ -190,6 +190,25 @@ int atac_batch_eval_host(atac_batch b, uint32_t flags, double* fm, double* Km, d
 int atac_batch_vjp_host(atac_batch b, const double* g_fm, double* g_q1, double* g_q2, double* g_phi1, double* g_phi2,
                         double* g_kn, double* g_kt);
 int atac_sdf_distance_host(atac_grid g, const double E2[16], double col_th, const double* xw, int64_t n, double* d);
+
+/* Signed distance grid of a closed triangle mesh: the level-set construction the
+ * reference's SDF bodies are built with (sdfgen::make_level_set3,
+ * simulation/externals/sdfgen/makelevelset3.cpp:163-230, called from
+ * BodySDFObj::precompute_SDF, Body/BodySDFObj.cpp:38-77, and SDFMesh::precompute_SDF,
+ * Mesh.cpp:54-95): exact distances within exact_band cells of each triangle, fast
+ * sweeping elsewhere, inside (negative) where the +x ray crossings so far are odd;
+ * the same values node for node. x: 3*nv FP32 vertex coordinates; tri: 3*nf vertex
+ * indices (< nv); origin: node (0,0,0); phi (host out): ni*nj*nk FP32, x fastest.
+ * Runs on the current device; synchronous. */
+int atac_sdf_from_mesh(const float* x, int64_t nv, const uint32_t* tri, int64_t nf, const float origin[3], float dx,
+                       float dy, float dz, int32_t ni, int32_t nj, int32_t nk, int32_t exact_band, float* phi);
+
+/* The grid precompute_SDF places around a mesh (BodySDFObj.cpp:38-77 with
+ * min_size = res; Mesh.cpp:54-95 with min_size = 20): V is 3*nv FP64 vertex
+ * coordinates; outputs the FP32 min_box (the grid origin) and max_box (the
+ * bounding box padded by one cell), the spacing d (FP64) and the node counts n. */
+int atac_sdf_mesh_grid(const double* V, int64_t nv, double dx, int32_t min_size, float min_box[3], float max_box[3],
+                       double d[3], int32_t n[3]);
 int atac_sdf_collision_host(atac_grid g, const double E2[16], const double phi2[6], double col_th, uint32_t flags,
                             const double* xw, const double* xw_dot, int64_t n, double* values, double* derivs);
 

@@ -5,7 +5,9 @@ bench.py's cpu_baseline leg, as the checker. The product path never imports it.
 
 Every function here forwards to the C restatement; the reference routine each one
 follows is cited in atac_oracle.cpp (BodySDFObj.cpp:123-377,
-CollisionDetection.cpp:124-147, ForceGeneralSDFContact.cpp:147-447 and the p6 lines).
+CollisionDetection.cpp:124-147, ForceGeneralSDFContact.cpp:147-447 and the p6 lines)
+and levelset_oracle.cpp (sdfgen makelevelset3.cpp:6-230, BodySDFObj.cpp:38-77,
+Mesh.cpp:54-95).
 """
 from __future__ import annotations
 
@@ -82,6 +84,11 @@ def lib():
                                                   C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                                   C.c_uint, dp, dp, dp, dp]
         L.oracle_se3_exp.argtypes = [dp, dp]
+        vp = C.c_void_p
+        fp = C.POINTER(C.c_float)
+        L.oracle_make_level_set3.argtypes = [vp, C.c_int64, vp, C.c_int64, fp, C.c_float, C.c_float, C.c_float,
+                                             C.c_int, C.c_int, C.c_int, C.c_int, vp, vp]
+        L.oracle_sdf_mesh_grid.argtypes = [dp, C.c_int64, C.c_double, C.c_int, fp, fp, dp, C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -247,3 +254,35 @@ def print_error_metric(a, b):
     err = float(np.linalg.norm(a - b))
     rel = err / max(min(float(np.linalg.norm(a)), float(np.linalg.norm(b))), 1e-4)
     return err, rel
+
+
+def make_level_set3(x, tri, origin, d, n, exact_band=1, with_closest=False):
+    """sdfgen::make_level_set3 restated: x (nv x 3 FP32 vertices), tri (nf x 3),
+    origin and spacing d (FP32), n = (ni, nj, nk). Returns phi (nk, nj, ni) FP32
+    (and the closest triangle per node)."""
+    xv = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 3)
+    tv = np.ascontiguousarray(tri, dtype=np.uint32).reshape(-1, 3)
+    ni, nj, nk = (int(a) for a in n)
+    phi = np.empty((nk, nj, ni), dtype=np.float32)
+    cl = np.empty((nk, nj, ni), dtype=np.int32)
+    o = (C.c_float * 3)(*[float(np.float32(a)) for a in origin])
+    rc = lib().oracle_make_level_set3(xv.ctypes.data_as(C.c_void_p), len(xv), tv.ctypes.data_as(C.c_void_p), len(tv), o,
+                                      float(np.float32(d[0])), float(np.float32(d[1])), float(np.float32(d[2])),
+                                      ni, nj, nk, int(exact_band), phi.ctypes.data_as(C.c_void_p),
+                                      cl.ctypes.data_as(C.c_void_p))
+    if rc:
+        raise ValueError("oracle_make_level_set3: bad arguments")
+    return (phi, cl) if with_closest else phi
+
+
+def sdf_mesh_grid(V, dx, min_size):
+    """precompute_SDF's grid placement: V (nv x 3 FP64). Returns (min_box FP32,
+    max_box FP32, spacing FP64, (ni, nj, nk))."""
+    v = np.ascontiguousarray(V, dtype=np.float64).reshape(-1, 3)
+    mn = (C.c_float * 3)()
+    mx = (C.c_float * 3)()
+    d = np.zeros(3)
+    n = (C.c_int * 3)()
+    if lib().oracle_sdf_mesh_grid(_dp(v), len(v), float(dx), int(min_size), mn, mx, _dp(d), n):
+        raise ValueError("oracle_sdf_mesh_grid: bad arguments")
+    return np.array(mn[:], dtype=np.float32), np.array(mx[:], dtype=np.float32), d, tuple(int(a) for a in n)
