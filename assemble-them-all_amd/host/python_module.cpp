@@ -124,6 +124,23 @@ PYBIND11_MODULE(_redmax, m) {
                         return new BodySDFObj(nullptr, name, g, mn, mx, dx, dy, dz, col_th, points_of(pts), index0);
                     },
                     py::arg("name"), py::arg("path"), py::arg("col_th"), py::arg("contact_points"), py::arg("index0") = 0)
+        .def_static("from_mesh", [](std::string name, const arr& V, const py::array_t<int64_t, py::array::c_style | py::array::forcecast>& F,
+                                    double dx, int res, double col_th, const arr& pts, int index0) {
+                        if (V.ndim() != 2 || V.shape(1) != 3 || F.ndim() != 2 || F.shape(1) != 3)
+                            throw std::runtime_error("from_mesh: V is n x 3, F is m x 3");
+                        const std::vector<Vector3> vs = points_of(V);
+                        std::vector<std::array<int, 3>> fs((size_t)F.shape(0));
+                        const int64_t* fp = F.data();
+                        for (size_t t = 0; t < fs.size(); ++t)
+                            for (int a = 0; a < 3; ++a) fs[t][a] = (int)fp[3 * t + a];
+                        sdfgen::Array3<dtype> g;
+                        Vector3 mn, mx;
+                        dtype dx_, dy_, dz_;
+                        BodySDFObj::precompute_SDF(vs, fs, dx, res, g, mn, mx, dx_, dy_, dz_);
+                        return new BodySDFObj(nullptr, name, g, mn, mx, dx_, dy_, dz_, col_th, points_of(pts), index0);
+                    },
+                    py::arg("name"), py::arg("V"), py::arg("F"), py::arg("dx"), py::arg("res"), py::arg("col_th"),
+                    py::arg("contact_points"), py::arg("index0") = 0)
         .def("save_sdf", &BodySDFObj::save_SDF_file)
         .def_readwrite("col_th", &BodySDFObj::_col_th)
         .def_property_readonly("min_box", [](const BodySDFObj& b) { return fixed_out(b._min_box); })

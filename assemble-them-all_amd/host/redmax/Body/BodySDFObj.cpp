@@ -16,6 +16,43 @@ BodySDFObj::BodySDFObj(Simulation* sim, std::string name, const sdfgen::Array3<d
 
 BodySDFObj::~BodySDFObj() { release_device_grid(); }
 
+void BodySDFObj::precompute_SDF(const std::vector<Vector3>& V, const std::vector<std::array<int, 3>>& F, dtype dx,
+                                int res, sdfgen::Array3<dtype>& sdf, Vector3& min_box, Vector3& max_box,
+                                dtype& dx_out, dtype& dy_out, dtype& dz_out) {
+    if (V.empty() || F.empty()) throw_error("BodySDFObj::precompute_SDF: empty mesh");
+    std::vector<double> v(3 * V.size());
+    std::vector<float> vf(3 * V.size());
+    for (size_t i = 0; i < V.size(); ++i)
+        for (int a = 0; a < 3; ++a) {
+            v[3 * i + a] = V[i](a);
+            vf[3 * i + a] = (float)V[i](a);   // sdfgen's Vec3f vertices
+        }
+    std::vector<uint32_t> f(3 * F.size());
+    for (size_t t = 0; t < F.size(); ++t)
+        for (int a = 0; a < 3; ++a) {
+            if (F[t][a] < 0 || (size_t)F[t][a] >= V.size()) throw_error("BodySDFObj::precompute_SDF: bad face index");
+            f[3 * t + a] = (uint32_t)F[t][a];
+        }
+    float mn[3], mx[3];
+    double d[3];
+    int32_t n[3];
+    if (atac_sdf_mesh_grid(v.data(), (int64_t)V.size(), dx, res, mn, mx, d, n) != ATAC_OK)
+        throw_error(std::string("BodySDFObj::precompute_SDF: ") + atac_last_error());
+    std::vector<float> phi((size_t)n[0] * n[1] * n[2]);
+    if (atac_sdf_from_mesh(vf.data(), (int64_t)V.size(), f.data(), (int64_t)F.size(), mn, (float)d[0], (float)d[1],
+                           (float)d[2], n[0], n[1], n[2], 1, phi.data()) != ATAC_OK)
+        throw_error(std::string("BodySDFObj::precompute_SDF: ") + atac_last_error());
+    sdf.resize(n[0], n[1], n[2]);
+    for (size_t e = 0; e < phi.size(); ++e) sdf.a[e] = (dtype)phi[e];
+    for (int a = 0; a < 3; ++a) {
+        min_box(a) = mn[a];
+        max_box(a) = mx[a];
+    }
+    dx_out = d[0];
+    dy_out = d[1];
+    dz_out = d[2];
+}
+
 bool BodySDFObj::load_SDF_file(const std::string& path, sdfgen::Array3<dtype>& sdf, Vector3& min_box, Vector3& max_box,
                                dtype& dx, dtype& dy, dtype& dz) {
     std::ifstream in(path.c_str());

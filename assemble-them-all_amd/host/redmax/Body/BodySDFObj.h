@@ -2,9 +2,11 @@
 // (simulation/redmax/Body/BodySDFObj.h:13-16: _min_box, _max_box, _dx/_dy/_dz,
 // _col_th, _SDF in sdfgen's Array3 layout a[i + ni*(j + nj*k)]) plus the query
 // methods distance / collision (BodySDFObj.h:35-76), evaluated on the GPU.
-// The grid is given directly or read from the reference's .sdf text format
-// (BodySDFObj.cpp:86-116); building it from a mesh is not part of this layer.
+// The grid is given directly, read from the reference's .sdf text format
+// (BodySDFObj.cpp:86-116), or built from a triangle mesh on the GPU
+// (precompute_SDF, BodySDFObj.cpp:38-77).
 #pragma once
+#include <array>
 #include <string>
 #include <vector>
 #include "Body/Body.h"
@@ -42,6 +44,14 @@ public:
     static bool load_SDF_file(const std::string& path, sdfgen::Array3<dtype>& sdf, Vector3& min_box, Vector3& max_box,
                               dtype& dx, dtype& dy, dtype& dz);
     void save_SDF_file(const std::string& path) const;
+
+    // the grid precompute_SDF builds from a closed triangle mesh (BodySDFObj.cpp:38-77):
+    // spacing min(dx, bounding-box extent / res) per axis, one cell of padding around
+    // the FP32 bounds, values from the level-set construction (sdfgen::make_level_set3)
+    // run on the GPU by atac_sdf_from_mesh; throws on a bad mesh
+    static void precompute_SDF(const std::vector<Vector3>& V, const std::vector<std::array<int, 3>>& F, dtype dx,
+                               int res, sdfgen::Array3<dtype>& sdf, Vector3& min_box, Vector3& max_box, dtype& dx_out,
+                               dtype& dy_out, dtype& dz_out);
 
     // query_SDF(R2^T (xw - p2)) + col_th, FLT_MAX + col_th outside the box (BodySDFObj.cpp:207-213)
     dtype distance(Vector3 xw);

@@ -84,3 +84,24 @@ def test_bad_mesh_inputs(gpu):
         level_set(V, bad, (-1.2, -1.2, -1.2), (0.1, 0.1, 0.1), (25, 25, 25))
     with pytest.raises(atac.AtacError):
         level_set(V, F, (-1.2, -1.2, -1.2), (0.1, 0.0, 0.1), (25, 25, 25))
+
+
+def test_body_sdf_from_mesh(gpu):
+    """BodySDFObj built from a mesh (BodySDFObj.cpp:38-77, spacing min(dx, extent / res))
+    holds the oracle's grid, and its distance query is the oracle's."""
+    from atac import _redmax as R
+    V, F = M.icosphere(0.8, 2, (0.1, 0.2, -0.1))
+    res = 30
+    body = R.BodySDFObj.from_mesh("sphere", V, F, 0.05, res, 0.01, np.zeros((1, 3)))
+    mn, mx, d, n = O.sdf_mesh_grid(V, 0.05, res)
+    ref = O.make_level_set3(V.astype(np.float32), F, mn, d.astype(np.float32), n)
+    assert tuple(body.shape) == n
+    assert np.array_equal(body.values(), ref.astype(np.float64))
+    assert np.array_equal(np.ravel(body.min_box), mn.astype(np.float64))
+    assert np.array_equal(np.ravel(body.max_box), mx.astype(np.float64))
+    assert np.array_equal(np.ravel(body.dxyz), d)
+    G = O.GridRef(ref, mn.astype(np.float64), mx.astype(np.float64), d)
+    rng = np.random.default_rng(4)
+    for x in rng.uniform(mn, mx, size=(50, 3)):
+        want = O.distance(G, np.eye(4), 0.01, x[None])[0]
+        assert abs(body.distance(x) - want) <= 1e-6 * np.abs(ref).max()
