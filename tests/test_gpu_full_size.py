@@ -1,0 +1,101 @@
+"""The 4,096-instance batched workload at full size (51.8M queries over 16 128^3
+grids), through properties that do not depend on the size: agreement with the FP64
+oracle on a sample of instances (active set outside the |d| < 1e-6 band, wrench to
+1e-5 of the per-contact scale), the total active count against the oracle's
+detection over every instance, bitwise determinism, graph replay equal to the eager
+launch, and tuning invariance of the active set."""
+import numpy as np
+import pytest
+import torch
+
+from atac import workloads as W
+from atac.batch import batch_from_workload
+from helpers import band_mask, oracle_workload, rel_err
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _np(out):
+    torch.cuda.synchronize()
+    return {k: v.cpu().numpy().copy() for k, v in out.items()}
+
+
+@pytest.fixture(scope="module")
+def full(gpu):
+    print("[case] batched 4096 instances, 51.8M queries", flush=True)
+    w = W.batched()
+    b = batch_from_workload(w)
+    return w, b
+
+
+def test_full_size_sample_parity(full):
+    w, b = full
+    out = _np(b.eval(p6=True))
+    sample = list(range(0, len(w.instances), 64))       # 64 instances across the batch
+    ref = oracle_workload(w, flags=O.FLAG_P6 | O.FLAG_DERIV, instances=[w.instances[i] for i in sample])
+    checked = 0
+    for k, i in enumerate(sample):
+        r = ref[k]
+        band = int(band_mask(w, w.instances[i]).sum())
+        assert abs(int(out["n_active"][i]) - r["n_active"]) <= band
+        if r["n_active"] and band == 0:
+            assert rel_err(out["fm"][i], r["fm"], r["scale_fm"]) <= TOL
+            assert rel_err(out["dfm_dp6"][i], r["dfm_dp6"], r["scale_fm"]) <= TOL
+            checked += 1
+    assert checked > 10
+
+
+def test_full_size_active_count(full):
+    """Every instance's active count against the oracle's detection (distance < 0)."""
+    w, b = full
+    out = _np(b.eval())
+    grids = [O.GridRef.from_grid(g) for g in w.grids]
+    for i, ins in enumerate(w.instances):
+        pts = W.round_points_f32(w.point_sets[ins.points])
+        xw = pts @ ins.E1[:3, :3].T + ins.E1[:3, 3]
+        d = O.distance(grids[ins.grid], ins.E2, ins.col_th, xw)
+        lo, hi = int((d < -1e-6).sum()), int((d < 1e-6).sum())
+        assert lo <= int(out["n_active"][i]) <= hi, f"instance {i}"
+
+
+def test_full_size_determinism_and_graph(full):
+    w, b = full
+    g = torch.tensor(np.random.default_rng(5).normal(size=(b.n, 12)), device="cuda")
+    a1 = _np(b.eval(checkpoint=True))
+    v1 = _np(b.vjp(g))
+    a2 = _np(b.eval(checkpoint=True))
+    v2 = _np(b.vjp(g))
+    for k in a1:
+        assert np.array_equal(a1[k], a2[k]), k
+    for k in v1:
+        assert np.array_equal(v1[k], v2[k]), k
+    out = b.alloc_outputs()
+    grads = b.alloc_gradients()
+    b.graph_capture(out, checkpoint=True, g_fm=g, grads=grads)
+    for _ in range(3):
+        b.graph_launch()
+        torch.cuda.synchronize()
+        go, gg = _np(out), _np(grads)
+        assert np.array_equal(go["fm"], a1["fm"]) and np.array_equal(go["n_active"], a1["n_active"])
+        for k in v1:
+            assert np.array_equal(gg[k], v1[k]), k
+
+
+def test_full_size_tuning_invariance(full):
+    w, b = full
+    base = _np(b.eval())
+    try:
+        for tu in ({"tile_points": 2048, "detect_block": 128, "points_per_lane": 2},
+                   {"tile_points": 8192, "detect_block": 512, "points_per_lane": 1}):
+            b.retune(tu)
+            o = _np(b.eval())
+            assert np.array_equal(o["n_active"], base["n_active"])
+            # the tiling regroups the FP32 lane sums: a coarse bound here (the per-contact
+            # scale of the sampled parity test above is the tight one)
+            scale = np.abs(base["fm"]).max(axis=1)
+            assert np.all(np.abs(o["fm"] - base["fm"]).max(axis=1) <= 1e-3 * scale + 1e-6)
+    finally:
+        b.retune({})
