@@ -115,6 +115,35 @@ def cpu_baseline(w, gpu_fm, budget_s, threads):
     }
 
 
+def sdf_build_measurement(cpu):
+    """Mesh -> SDF grid (the level-set construction): GPU time for a 5,120-triangle
+    sphere into 64^3 and 128^3 grids, and the CPU restatement at 64^3, with the
+    number of nodes whose values differ (0 expected)."""
+    import torch
+    from atac import workloads as W
+    from atac.sdfmesh import level_set
+    V, F = W.icosphere(1.0, 4)
+    res = {}
+    for n in (64, 128):
+        d = 2.6 / (n - 1)
+        origin, dd, nn = (-1.3, -1.3, -1.3), (d, d, d), (n, n, n)
+        level_set(V, F, origin, dd, nn)          # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        phi = level_set(V, F, origin, dd, nn)
+        gpu_ms = (time.perf_counter() - t0) * 1e3
+        e = {"grid": f"{n}^3", "triangles": int(len(F)), "gpu_ms": round(gpu_ms, 2)}
+        if cpu and n == 64:
+            from oracle import oracle as O
+            t0 = time.perf_counter()
+            ref = O.make_level_set3(V.astype(np.float32), F, origin, dd, nn)
+            e["cpu_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+            e["cpu_kind"] = "port, 1 thread"
+            e["nodes_differing"] = int((phi.view(np.uint32) != ref.view(np.uint32)).sum())
+        res[f"{n}"] = e
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -226,6 +255,9 @@ def main():
         cpu, err = cpu_baseline(w, out["fm"].cpu().numpy(), args.cpu_budget, threads)
         log(f"cpu baseline {cpu['value']:.3e} queries/s on {threads} threads; err {err}")
 
+    sdf_build = sdf_build_measurement(not args.no_cpu)
+    log(f"sdf build: {sdf_build}")
+
     line = {
         "metric": METRIC, "value": n_q / ms * 1e3, "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
@@ -243,6 +275,7 @@ def main():
         "plain_path": plain,
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "step_effective_GBps": round(step_bytes / (ms * 1e-3) / 1e9, 1),
+        "sdf_build": sdf_build,
     }
     print(json.dumps(line), flush=True)
 
