@@ -7,6 +7,7 @@
 
 #include "Body/Body.h"
 #include "Body/BodySDFObj.h"
+#include "Simulation/BatchedBDF1.h"
 #include "Force/ForceGeneralSDFContact.h"
 
 namespace py = pybind11;
@@ -220,4 +221,44 @@ PYBIND11_MODULE(_redmax, m) {
         .def("time_report_us", [](ForceGeneralSDFContact& f) {
             return py::make_tuple(f._time.get("collision detection"), f._time.get("compute force"));
         });
+
+    py::class_<BatchedBDF1>(m, "BatchedBDF1",
+                            "BDF1 steps of parts on translational joints against fixed parts, many candidates at once")
+        .def(py::init([](py::list parts, std::vector<std::pair<int, int>> contacts, double kn, double kt, double mu,
+                         double damping, double col_th, double h, std::vector<double> gravity, bool precise) {
+                 if (gravity.size() != 3) throw std::invalid_argument("gravity has 3 components");
+                 std::vector<BDF1Part> ps;
+                 for (auto item : parts) {
+                     auto t = item.cast<py::tuple>();
+                     if (t.size() != 3) throw std::invalid_argument("a part is (points, BodySDFObj or None, E0)");
+                     BDF1Part p;
+                     const arr pts = t[0].cast<arr>();
+                     if (pts.size() > 0) p.points = points_of(pts);
+                     if (!t[1].is_none()) p.grid = t[1].cast<BodySDFObj*>()->device_grid();
+                     const arr E0 = t[2].cast<arr>();
+                     if (E0.size() != 16) throw std::invalid_argument("E0 must be 4 x 4");
+                     std::copy(E0.data(), E0.data() + 16, p.E0.begin());
+                     ps.push_back(std::move(p));
+                 }
+                 BDF1ContactParams cp;
+                 cp.kn = kn; cp.kt = kt; cp.mu = mu; cp.damping = damping; cp.col_th = col_th;
+                 return new BatchedBDF1(std::move(ps), std::move(contacts), cp, h,
+                                        make_vector3(gravity[0], gravity[1], gravity[2]), precise);
+             }),
+             py::arg("parts"), py::arg("contacts"), py::arg("kn") = 1e6, py::arg("kt") = 1e3, py::arg("mu") = 0.0,
+             py::arg("damping") = 0.0, py::arg("col_th") = 0.01, py::arg("h") = 1e-3,
+             py::arg("gravity") = std::vector<double>{0.0, 0.0, 0.0}, py::arg("precise") = false,
+             py::keep_alive<1, 2>())
+        .def("add_candidate", [](BatchedBDF1& s, int move, double mass, const arr& q, const arr& qdot, const arr& u) {
+                 return s.add_candidate(move, mass, vec3(q), vec3(qdot), vec3(u));
+             }, py::arg("move"), py::arg("mass"), py::arg("q"), py::arg("qdot"), py::arg("u"))
+        .def("set_state", [](BatchedBDF1& s, int c, const arr& q, const arr& qdot) { s.set_state(c, vec3(q), vec3(qdot)); })
+        .def("set_force", [](BatchedBDF1& s, int c, const arr& u) { s.set_force(c, vec3(u)); })
+        .def("step", &BatchedBDF1::step, py::arg("n") = 1, py::call_guard<py::gil_scoped_release>())
+        .def("q", [](const BatchedBDF1& s, int c) { return fixed_out(s.q(c)); })
+        .def("qdot", [](const BatchedBDF1& s, int c) { return fixed_out(s.qdot(c)); })
+        .def("converged", &BatchedBDF1::converged)
+        .def("newton_iterations", &BatchedBDF1::newton_iterations)
+        .def_property_readonly("n_candidates", &BatchedBDF1::n_candidates)
+        .def_property_readonly("n_evaluations", &BatchedBDF1::n_evaluations);
 }

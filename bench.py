@@ -144,6 +144,44 @@ def sdf_build_measurement(cpu):
     return res
 
 
+def bdf1_measurement(cpu):
+    """Implicit (BDF1) stepping of a peg on a translational joint in the hole, pushed
+    along each of the 26 cube directions: 26 candidates stepped together, every Newton
+    iteration and line-search trial of all of them one batched contact evaluation
+    (FP32 contact arithmetic, FP64 Newton on the host)."""
+    from atac import _redmax as R
+    from atac import workloads as W
+    w = W.peg_in_hole(n_points=5000, n_poses=8)
+    g = w.grids[0]
+    hole = R.BodySDFObj("hole", g.values.astype(np.float64), g.min_box, g.max_box, g.dxyz, 0.01, np.zeros((1, 3)), 0)
+    E_peg = w.instances[3].E1.copy()
+    parts = [(w.point_sets[0], None, E_peg), (np.zeros((0, 3)), hole, np.eye(4))]
+    sim = R.BatchedBDF1(parts, [(0, 1)], kn=W.KN, kt=W.KT, mu=W.MU, damping=W.DAMPING, col_th=0.01, h=1e-3)
+    dirs = W.cube_directions()
+    for u in dirs:
+        sim.add_candidate(0, 1.5, np.zeros(3), np.zeros(3), 500.0 * np.asarray(u))
+    sim.step(1)                                  # builds the batch
+    e0 = sim.n_evaluations
+    n = 20
+    t0 = time.perf_counter()
+    sim.step(n)
+    ms = (time.perf_counter() - t0) * 1e3 / n
+    conv = sum(bool(sim.converged(c)) for c in range(len(dirs)))
+    res = {"candidates": len(dirs), "contact_points": int(len(w.point_sets[0])), "ms_per_step": round(ms, 3),
+           "batched_evaluations_per_step": round((sim.n_evaluations - e0) / n, 2),
+           "converged_last_step": conv}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from bdf1_reference import ReferenceBDF1
+        ref = ReferenceBDF1([(w.point_sets[0], None, E_peg), (np.zeros((0, 3)), g, np.eye(4))], [(0, 1)], W.KN, W.KT,
+                            W.MU, W.DAMPING, 0.01, 1e-3, np.zeros(3))
+        t0 = time.perf_counter()
+        ref.step(0, 1.5, np.zeros(3), np.zeros(3), 500.0 * np.asarray(dirs[0]), 3)
+        res["cpu_ms_per_step_per_candidate"] = round((time.perf_counter() - t0) * 1e3 / 3, 1)
+        res["cpu_kind"] = "port (FP64 oracle forces, Python Newton), 1 thread"
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -257,6 +295,8 @@ def main():
 
     sdf_build = sdf_build_measurement(not args.no_cpu)
     log(f"sdf build: {sdf_build}")
+    bdf1 = bdf1_measurement(not args.no_cpu)
+    log(f"bdf1: {bdf1}")
 
     line = {
         "metric": METRIC, "value": n_q / ms * 1e3, "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
@@ -276,6 +316,7 @@ def main():
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "step_effective_GBps": round(step_bytes / (ms * 1e-3) / 1e9, 1),
         "sdf_build": sdf_build,
+        "bdf1_translational": bdf1,
     }
     print(json.dumps(line), flush=True)
 
