@@ -204,8 +204,14 @@ int atac_grid_create(const float* values, int32_t ni, int32_t nj, int32_t nk, co
             delete g;
             return fail(ATAC_ERR_INVALID, "atac_grid_create: grids hold fewer than 2^27 nodes");
         }
+        // the detection forms table offsets with 24-bit products
+        if (ni >= (1 << 24) || nj >= (1 << 24) || nk >= (1 << 24) || (int64_t)G.nb[1] * G.nb[2] >= (1 << 24)) {
+            dfree(g->d_values);
+            delete g;
+            return fail(ATAC_ERR_INVALID, "atac_grid_create: n_y * n_z / 4 and each n stay below 2^24");
+        }
         rc = dalloc(&g->d_cells, (size_t)(2 * nc));
-        if (!rc) rc = dalloc(&g->d_blk, (size_t)nb);
+        if (!rc) rc = dalloc(&g->d_blk, (size_t)(nb + 1));   // + the entry of points outside the box
         if (!rc) {
             e = launch_grid_setup(G, g->d_cells, g->d_blk, nullptr);
             if (e == hipSuccess) e = hipDeviceSynchronize();

@@ -4,6 +4,7 @@
 #include <cstring>
 #include "atac.h"
 #include "atac_types.h"
+#include "contact_math.h"
 
 namespace atac {
 
@@ -56,6 +57,7 @@ inline void compose_instance(const atac_instance& s, const GridDev& G, const flo
     }
     d.fkn = (float)d.kn; d.fkt = (float)d.kt; d.fmu = (float)d.mu;
     d.fdamping = (float)d.damping; d.fcontact_scale = (float)d.contact_scale;
+    block_thresholds(d.col_th, d.blk_off, d.blk_on);
     for (float& f : d.fpad) f = 0.0f;
 }
 

@@ -549,8 +549,9 @@ __global__ void build_cells_kernel(GridDev G, float4* cells) {
 
 __global__ void build_blocks_kernel(GridDev G, float2* blk) {
     const int64_t nb0 = G.nb[0], nb1 = G.nb[1], nb = nb0 * nb1 * G.nb[2];
-    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += (int64_t)gridDim.x * blockDim.x)
-        blk[b] = block_bounds(G, (int)(b % nb0), (int)((b / nb0) % nb1), (int)(b / (nb0 * nb1)));
+    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b <= nb; b += (int64_t)gridDim.x * blockDim.x)
+        blk[b] = b < nb ? block_bounds(G, (int)(b % nb0), (int)((b / nb0) % nb1), (int)(b / (nb0 * nb1)))
+                        : make_float2(INFINITY, INFINITY);   // points outside the box
 }
 
 hipError_t launch_grid_setup(const GridDev& G, float4* cells, float2* blk, hipStream_t st) {

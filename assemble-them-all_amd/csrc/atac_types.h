@@ -19,9 +19,12 @@ struct GridDev {
     // interpolates (v000 v100 v010 v110 | v001 v101 v011 v111, offsets clamped at
     // the last node) in 2 x float4, node-index order
     const float4* cells;
-    // conservative bounds of the detection value per block of kBlk^3 cells:
+    // conservative bounds of the detection value per block of kBlk^3 cells: the
     // (min, max) of every node the block's cells read (with the detection's clamp
-    // at the last node), or (-inf, +inf) if one of them is not finite
+    // at the last node) widened by kBlkTol * max(|min|, |max|) and rounded
+    // outward, or (-inf, +inf) if one of them is not finite; one more entry after
+    // the nb[0]*nb[1]*nb[2] blocks, (+inf, +inf), is the bound of points outside
+    // the box
     const float2* blk;
     int32_t nb[3];      // blocks per axis: (n - 1) / kBlk + 1
     int32_t pad;
@@ -56,7 +59,10 @@ struct InstDev {
     // arithmetic (read as wave-uniform scalars)
     float fR1[9], fR2[9], fw1[3], fv1[3], fw2[3], fv2[3];
     float fkn, fkt, fmu, fdamping, fcontact_scale;
-    float fpad[13];
+    // thresholds of the widened block bounds (block_decision): a block with
+    // min > blk_off holds no active point, one with max < blk_on only active ones
+    float blk_off, blk_on;
+    float fpad[11];
 };
 static_assert(sizeof(InstDev) == 704, "InstDev layout");
 

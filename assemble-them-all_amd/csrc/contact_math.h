@@ -270,6 +270,22 @@ ATAC_HD void pack_cell(const GridDev& G, int i, int j, int k, float4 out[2]) {
     out[1] = make_float4(v[b + dk], v[b + dk + di], v[b + dk + dj], v[b + dk + dj + di]);
 }
 
+// FP32 roundings of an FP64 value toward -inf and toward +inf.
+ATAC_HD float f32_down(double x) {
+    float f = (float)x;
+    if ((double)f > x) f = nextafterf(f, -INFINITY);
+    return f;
+}
+ATAC_HD float f32_up(double x) {
+    float f = (float)x;
+    if ((double)f < x) f = nextafterf(f, INFINITY);
+    return f;
+}
+
+// Margin of the block bounds (relative to the magnitude of the values they
+// bound); block_decision gives the reason for its size.
+constexpr double kBlkTol = 4e-6;
+
 ATAC_HD float2 block_bounds(const GridDev& G, int bi, int bj, int bk) {
     // nodes read by the block's cells: [b*kBlk, min(b*kBlk + kBlk, n - 1)] per axis
     const int i0 = bi * kBlk, i1 = i0 + kBlk < G.n[0] - 1 ? i0 + kBlk : G.n[0] - 1;
@@ -285,7 +301,18 @@ ATAC_HD float2 block_bounds(const GridDev& G, int bi, int bj, int bk) {
                 lo = fminf(lo, v);
                 hi = fmaxf(hi, v);
             }
-    return finite ? make_float2(lo, hi) : make_float2(-INFINITY, INFINITY);
+    if (!finite) return make_float2(-INFINITY, INFINITY);
+    const double m = kBlkTol * fmax(fabs((double)lo), fabs((double)hi));
+    return make_float2(f32_down((double)lo - m), f32_up((double)hi + m));
+}
+
+// The thresholds block_decision compares an instance's block bounds with:
+// -col_th -+ kBlkTol * |col_th|, for col_th as the detection adds it (FP32),
+// rounded outward.
+ATAC_HD void block_thresholds(double col_th, float& off, float& on) {
+    const double c = (double)(float)col_th, m = kBlkTol * fabs(c);
+    off = f32_up(m - c);
+    on = f32_down(-c - m);
 }
 
 // FP64 trilinear value of a cell from its FP32 corners, in the reference's
@@ -326,10 +353,10 @@ ATAC_HD void sdf_cell32(const GridDev& G, const Corners& c, const float f[3], fl
 // trilinear value in FP32. Returns the value query_SDF(x) + col_th, or +inf when
 // x is outside [min_box, max_box] (the reference returns FLT_MAX + col_th).
 ATAC_HD void grid_coords(const InstDev& in, float px, float py, float pz, double c[3]) {
-    const double x = px, y = py, z = pz;
-    c[0] = in.A[0] * x + in.A[1] * y + in.A[2] * z + in.b[0];
-    c[1] = in.A[3] * x + in.A[4] * y + in.A[5] * z + in.b[1];
-    c[2] = in.A[6] * x + in.A[7] * y + in.A[8] * z + in.b[2];
+    const double x = px, y = py, z = pz;   // three fused FP64 steps per axis, host and device alike
+    c[0] = fma(in.A[2], z, fma(in.A[1], y, fma(in.A[0], x, in.b[0])));
+    c[1] = fma(in.A[5], z, fma(in.A[4], y, fma(in.A[3], x, in.b[1])));
+    c[2] = fma(in.A[8], z, fma(in.A[7], y, fma(in.A[6], x, in.b[2])));
 }
 
 ATAC_HD float detect_value(const GridDev& G, const InstDev& in, float px, float py, float pz) {
@@ -396,16 +423,41 @@ ATAC_HD bool detect_interp(const GridDev& G, const InstDev& in, float px, float 
 // The detection decision detect_value(...) < 0, bit for bit, with fewer grid
 // reads. The trilinear value is a convex combination of its 8 corners, so it lies
 // in the (min, max) of the block holding the cell, up to the FP32 rounding of the
-// 7 lerps and the col_th add: at most 13 units of 2^-24 of max|corner| + |col_th|
-// (each lerp adds <= 4 units of its largest input; the add 1 unit). A block whose
-// bounds clear the threshold by 4e-6 of that scale (5x the rounding bound) settles
-// the point; otherwise detect_interp decides it.
-ATAC_HD int block_decision(float2 b, float cth) {   // 1 active, 0 inactive, -1 open
-    const float tol = 4e-6f * (fmaxf(fabsf(b.x), fabsf(b.y)) + fabsf(cth));
-    if (b.x + cth > tol) return 0;
-    if (b.y + cth < -tol) return 1;
-    return -1;
+// 7 lerps and the col_th add: at most 13 units of 2^-24 of M + |col_th|, M =
+// max|node| of the block (each lerp adds <= 4 units of its largest input; the add
+// 1 unit). The table holds min - kBlkTol*M and max + kBlkTol*M, the instance
+// -col_th -+ kBlkTol*|col_th| (all rounded outward), so min' > off means
+// min + col_th > 4e-6 (M + |col_th|), 5x the rounding bound: every point of the
+// block is inactive (and max' < on: every one active). The FP64 set-up arithmetic
+// of these values stays far inside the margin. Otherwise (-1) detect_interp
+// decides the point. The two outcomes exclude each other (min' <= max',
+// on <= off), and the entry of points outside the box, (+inf, +inf), is inactive.
+ATAC_HD int block_decision(float2 b, const InstDev& in) {   // 1 active, 0 inactive, -1 open
+    return b.x > in.blk_off ? 0 : (b.y < in.blk_on ? 1 : -1);
 }
+
+// Saturating FP64 -> int32 truncation (NaN -> 0), the hardware conversion's rule.
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ int cvt_sat_i32(double c) {
+    int r;
+    asm("v_cvt_i32_f64 %0, %1" : "=v"(r) : "v"(c));
+    return r;
+}
+// a * b + c for a, b below 2^24 (the full-rate 24-bit multiply-add)
+__device__ __forceinline__ uint32_t mad24(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t r;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+#else
+inline int cvt_sat_i32(double c) {
+    if (c != c) return 0;
+    if (c >= 2147483647.0) return 2147483647;
+    if (c <= -2147483648.0) return -2147483647 - 1;
+    return (int)c;
+}
+inline uint32_t mad24(uint32_t a, uint32_t b, uint32_t c) { return (a & 0xffffffu) * (b & 0xffffffu) + c; }
+#endif
 
 ATAC_HD bool detect_active(const GridDev& G, const InstDev& in, float px, float py, float pz, double cc[3]) {
     grid_coords(in, px, py, pz, cc);
@@ -417,8 +469,7 @@ ATAC_HD bool detect_active(const GridDev& G, const InstDev& in, float px, float 
     i = i < G.n[0] - 1 ? i : G.n[0] - 1;
     j = j < G.n[1] - 1 ? j : G.n[1] - 1;
     k = k < G.n[2] - 1 ? k : G.n[2] - 1;
-    const int dec = block_decision(ATAC_GLOBAL(G.blk)[i / kBlk + G.nb[0] * (j / kBlk + G.nb[1] * (k / kBlk))],
-                                   (float)in.col_th);
+    const int dec = block_decision(ATAC_GLOBAL(G.blk)[i / kBlk + G.nb[0] * (j / kBlk + G.nb[1] * (k / kBlk))], in);
     if (dec >= 0) return dec == 1;
     return detect_interp(G, in, px, py, pz, cc);
 }
@@ -479,21 +530,23 @@ __device__ __forceinline__ float2 load_f2_at(global_ptr<T> base, uint32_t byte_o
 #endif
 
 // Stream pass of the detection of PPL points (detect_active without the cell
-// reads): once the points' coordinates are computed, `before()` runs (the caller
-// issues reads that can share this step's wait), the block-bound reads of the
-// inside points are issued, then `issue()` (the caller's next loads go out
-// behind them, so waiting for the bounds does not wait for them), then the
-// bounds are applied. Returns the candidates (bit
-// k): the points they settle as active and the ones they leave open (also in
-// `open`); candidate_active finishes both. Only what every point needs is done
-// here, so a wavefront's lanes stay busy: the per-candidate work runs later,
-// densely, one candidate per lane. With all_active every valid point is a
-// settled candidate and nothing is read.
+// reads): once the points' block-bound offsets are computed, `before()` runs (the
+// caller issues reads that can share this step's wait), the block-bound reads are
+// issued, then `issue()` (the caller's next loads go out behind them, so waiting
+// for the bounds does not wait for them), then the bounds are applied. Returns
+// the candidates (bit k): the points they settle as active and the ones they
+// leave open (also in `open`); candidate_active finishes both. Only what every
+// point needs is done here, without branches, so a wavefront's lanes stay busy:
+// a point outside the box (or not valid) reads the table's outside entry, which
+// settles it as inactive, and the per-candidate work runs later, densely, one
+// candidate per lane. With all_active every valid point is a settled candidate
+// and nothing is read; on a grid without the table every inside point is open.
 template <int PPL, typename F0, typename F>
 ATAC_HD unsigned detect_candidates(const GridDev& G, const InstDev& in, const float (&x)[PPL], const float (&y)[PPL],
                                    const float (&z)[PPL], unsigned valid, bool all_active, unsigned& open,
                                    F0&& before, F&& issue) {
-    const float cth = (float)in.col_th;
+    static_assert(kBlk == 2, "block index = cell index / 2");
+    const uint32_t outside = (uint32_t)G.nb[0] * (uint32_t)G.nb[1] * (uint32_t)G.nb[2] * 8u;
     unsigned inside = 0;
     uint32_t boff[PPL];
 #pragma unroll
@@ -501,39 +554,41 @@ ATAC_HD unsigned detect_candidates(const GridDev& G, const InstDev& in, const fl
         double c[3];
         grid_coords(in, x[k], y[k], z[k], c);
         bool ins = ((valid >> k) & 1u) != 0u;
-        uint32_t idx[3];
+        uint32_t bi[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            const bool in_a = c[a] >= 0.0 && c[a] <= in.cmax[a];
-            ins = ins && in_a;
-            const int i = (int)(in_a ? c[a] : 0.0);   // inside: floor
-            idx[a] = (uint32_t)(i < G.n[a] - 1 ? i : G.n[a] - 1);
+            ins = ins && c[a] >= 0.0 && c[a] <= in.cmax[a];
+            const int i = cvt_sat_i32(c[a]);   // inside: floor; else unused
+            bi[a] = (uint32_t)(i < G.n[a] - 1 ? i : G.n[a] - 1) >> 1;
         }
         if (ins) inside |= 1u << k;
-        boff[k] = ((idx[0] / kBlk) + (uint32_t)G.nb[0] * ((idx[1] / kBlk) + (uint32_t)G.nb[1] * (idx[2] / kBlk))) * 8u;
+        const uint32_t b = mad24(mad24(bi[2], (uint32_t)G.nb[1], bi[1]), (uint32_t)G.nb[0], bi[0]);
+        boff[k] = ins ? b * 8u : outside;
     }
-    pin(inside);
     sched_fence();   // the points are in: the reads below wait for nothing earlier
     before();
     float2 bb[PPL];
     const bool use_blk = G.fast && !all_active;
+    if (use_blk) {
 #pragma unroll
-    for (int k = 0; k < PPL; ++k) {
-        bb[k] = make_float2(-INFINITY, INFINITY);   // no table: the point stays open
-        if (((inside >> k) & 1u) && use_blk) bb[k] = load_f2_at(ATAC_GLOBAL(G.blk), boff[k]);
+        for (int k = 0; k < PPL; ++k) bb[k] = load_f2_at(ATAC_GLOBAL(G.blk), boff[k]);
     }
     issue();
     if (all_active) {
         open = 0u;
         return valid;
     }
+    if (!use_blk) {
+        open = inside;
+        return inside;
+    }
+    const float off = in.blk_off, on = in.blk_on;
     unsigned act = 0, opn = 0;
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
-        if (!((inside >> k) & 1u)) continue;
-        const int dec = block_decision(bb[k], cth);
-        if (dec == 1) act |= 1u << k;
-        else if (dec < 0) opn |= 1u << k;
+        const bool a_k = bb[k].y < on, o_k = !(bb[k].x > off) && !a_k;   // block_decision 1 / -1
+        act |= (a_k ? 1u : 0u) << k;
+        opn |= (o_k ? 1u : 0u) << k;
     }
     open = opn;
     return act | opn;
@@ -564,7 +619,7 @@ ATAC_HD void candidate_read(const GridDev& G, const InstDev& in, float x, float 
             idx[a] = (uint32_t)i;
             r.f[a] = (float)(c[a] - (double)i);
         }
-        const uint32_t off = (idx[0] + (uint32_t)G.n[0] * (idx[1] + (uint32_t)G.n[1] * idx[2])) * 32u;
+        const uint32_t off = (idx[0] + (uint32_t)G.n[0] * mad24(idx[2], (uint32_t)G.n[1], idx[1])) * 32u;
         r.lo = load_f4_at(ATAC_GLOBAL(G.cells), off);
         r.hi = load_f4_at(ATAC_GLOBAL(G.cells), off + 16u);
     }

@@ -25,10 +25,19 @@ def build():
 _lib = None
 
 
+def _stale():
+    if not os.path.exists(_LIB):
+        return True
+    csrc = os.path.join(_REPO, "assemble-them-all_amd", "csrc")
+    srcs = [os.path.join(_HERE, "hostcheck.hip"), os.path.join(_REPO, "include", "atac.h")]
+    srcs += [os.path.join(csrc, f) for f in os.listdir(csrc) if f.endswith(".h")]
+    return max(os.path.getmtime(f) for f in srcs) > os.path.getmtime(_LIB)
+
+
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(_LIB):
+        if _stale():
             build()
         _lib = C.CDLL(_LIB)
     return _lib
