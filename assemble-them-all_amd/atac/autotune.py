@@ -20,7 +20,7 @@ import torch
 DEFAULT_CANDIDATES = [
     {"tile_points": tp, "detect_block": blk, "points_per_lane": ppl}
     for tp in (1024, 2048, 4096, 8192)
-    for blk in (256, 512)
+    for blk in (128, 256, 512)
     for ppl in (2, 4)
     if tp >= blk * ppl // 2
 ]

@@ -194,11 +194,12 @@ int atac_grid_create(const float* values, int32_t ni, int32_t nj, int32_t nk, co
         G.inv_d[a] = 1.0 / dxyz[a];
         G.nb[a] = (G.n[a] - 1) / kBlk + 1;
     }
+    set_fd_reach(G);
     {
         // corner-packed cells, one per node (the last layer clamped), addressed
         // by 32-bit byte offsets in the detection kernel
         const int64_t nc = (int64_t)ni * nj * nk;
-        const int64_t nb = (int64_t)G.nb[0] * G.nb[1] * G.nb[2];
+        const int64_t nb = blk_outside(G);   // entries of the block table before the outside entry
         if (nc * 32 >= ((int64_t)1 << 32)) {
             dfree(g->d_values);
             delete g;
@@ -406,8 +407,8 @@ static int wave_span_of(int tile_points, int block, int ppl) {
 
 static int plan(atac_batch b, const atac_tuning* t) {
     atac_tuning tu = t ? *t : atac_tuning{};
-    if (tu.tile_points <= 0) tu.tile_points = 4096;
-    if (tu.detect_block <= 0) tu.detect_block = 256;
+    if (tu.tile_points <= 0) tu.tile_points = 2048;
+    if (tu.detect_block <= 0) tu.detect_block = 128;
     if (tu.points_per_lane <= 0) tu.points_per_lane = 4;
     if (tu.tile_points > 32768 || tu.tile_points % 64 != 0)
         return fail(ATAC_ERR_INVALID, "tile_points must be a multiple of 64 and <= 32768");

@@ -24,8 +24,6 @@
 
 namespace atac {
 
-typedef float f4v __attribute__((ext_vector_type(4)));
-typedef float f2v __attribute__((ext_vector_type(2)));
 
 // ------------------------------------------------------------------ wave primitives
 template <int CTRL>
@@ -125,49 +123,42 @@ __device__ __forceinline__ void mark_slow_index(Rec32& r, int slow) { r.c[3] = _
 __device__ __forceinline__ void mark_slow_index(Rec64&, int) {}
 
 // ------------------------------------------------------------------ batched detection
-// PPL consecutive points from offset o (nontemporal vector loads). Past the end
-// (o >= end) the tile's first PPL points are read instead, for the caller to
-// mask out: the loads are unconditional and their values are used as loaded, so
-// they stay in flight until the points are tested (a tile holds at least one
-// point; the point arrays are padded to a multiple of 4 and o is a multiple of
-// PPL).
+// Points o0 + 64 k + lane, k < PPL, of a step into (x, y, z)[k]: each load
+// instruction reads 64 consecutive points (a contiguous 256-byte span), so the
+// points a wavefront handles together are neighbours in the Morton order and
+// their block-bound reads share cache lines. Points at or past `end` read point 0
+// (in bounds; their results are masked by the caller).
 template <int PPL, typename P>
-__device__ __forceinline__ void load_points(P px, P py, P pz, int o, int end, float (&x)[PPL], float (&y)[PPL],
-                                            float (&z)[PPL]) {
-    const uint32_t oo = (uint32_t)(o < end ? o : 0) * 4u;   // byte offset
-    if constexpr (PPL == 4) {
-        const f4v vx = nt_load_at<f4v>(px, oo), vy = nt_load_at<f4v>(py, oo), vz = nt_load_at<f4v>(pz, oo);
+__device__ __forceinline__ void load_points(P px, P py, P pz, int o0, int lane, int end, float (&x)[PPL],
+                                            float (&y)[PPL], float (&z)[PPL]) {
 #pragma unroll
-        for (int k = 0; k < PPL; ++k) { x[k] = vx[k]; y[k] = vy[k]; z[k] = vz[k]; }
-    } else if constexpr (PPL == 2) {
-        const f2v vx = nt_load_at<f2v>(px, oo), vy = nt_load_at<f2v>(py, oo), vz = nt_load_at<f2v>(pz, oo);
-#pragma unroll
-        for (int k = 0; k < PPL; ++k) { x[k] = vx[k]; y[k] = vy[k]; z[k] = vz[k]; }
-    } else {
-        x[0] = nt_load_at<float>(px, oo);
-        y[0] = nt_load_at<float>(py, oo);
-        z[0] = nt_load_at<float>(pz, oo);
+    for (int k = 0; k < PPL; ++k) {
+        const int o = o0 + 64 * k + lane;
+        const uint32_t oo = (uint32_t)(o < end ? o : 0) * 4u;   // byte offset
+        x[k] = nt_load_at<float>(px, oo);
+        y[k] = nt_load_at<float>(py, oo);
+        z[k] = nt_load_at<float>(pz, oo);
     }
 }
 
-// Points of one tile per work item. The tile is streamed: PPL consecutive
-// points per lane (vector loads, each wavefront a contiguous span of the tile;
-// the next points are loaded behind the block-bound reads of the current ones,
-// so a step waits once), and only the block bounds are read (detect_candidates).
-// The candidates a step finds (settled active or open; a few per cent, near the
-// surface) are appended, with their coordinates, to the wavefront's LDS buffer in
-// ascending order (a wave-wide exclusive prefix of the per-lane counts built from
-// ballots of their bits); whenever 64 are waiting, the wavefront finishes them
-// one per lane (candidate_active: cell read and interpolation of the open ones,
-// FD-stencil test) instead of every stream step branching into that work for a
-// few lanes. The active offsets (bit 15: slow path; tiles hold at most 32768
-// points) go to a per-wave LDS segment, then to the tile's slots of the active
-// list, and the tile pushes its work items for the record, contact and adjoint
-// passes.
+// Points of one tile per work item. The tile is streamed: a step takes 64 * PPL
+// consecutive points of the wavefront's contiguous span, lane l the points
+// 64 k + l (each load and block-bound read instruction covers 64 neighbouring
+// points; the next points are loaded behind the block-bound reads of the current
+// ones, so a step waits once), and only the block bounds are read
+// (detect_candidates). The candidates a step finds (settled active or open; a
+// few per cent, near the surface) are appended, with their coordinates, to the
+// wavefront's LDS ring in ascending order (ballots per point slot k); whenever 64
+// are waiting, the next step finishes them one per lane (candidate_locate and
+// candidate_finish: cell read and interpolation of the open ones, FD-stencil
+// test), overlapped with its own work, instead of every stream step branching
+// into that work for a few lanes. The active offsets (bit 15: slow path; tiles
+// hold at most 32768 points) go to a per-wave LDS segment, then to the tile's
+// slots of the active list, and the tile pushes its work items for the record,
+// contact and adjoint passes.
 template <int BLOCK, int PPL>
 __global__ __launch_bounds__(BLOCK) void detect_kernel(DetectArgs a) {
     constexpr int NW = BLOCK / 64;
-    constexpr int NB = PPL == 1 ? 1 : (PPL == 2 ? 2 : 3);   // bits of a per-lane count <= PPL
     constexpr int CAP = detect_ring_cap(PPL);
     extern __shared__ __align__(16) unsigned char s_lds[];
     __shared__ int s_wcount[NW];
@@ -189,70 +180,91 @@ __global__ __launch_bounds__(BLOCK) void detect_kernel(DetectArgs a) {
         const auto pz = as_global(in.pz) + t.begin;
         uint16_t* q = s_queue + w * span;
         const int wb = w * span, we = min(t.count, wb + span);
-        int wcount = 0, ncand = 0;   // wave-uniform
-        // finishes candidates [b, b + n), n <= 64, one per lane, into the output segment
-        auto finish = [&](int b, int n) {
-            bool act = false, slow = false;
-            uint32_t tag = 0;
+        // candidates wait in a ring of CAP entries: [rd, rd + occ) (mod CAP)
+        int wcount = 0, rd = 0, occ = 0, npend = 0;   // wave-uniform
+        auto ring = [](int i) { return i >= CAP ? i - CAP : i; };   // i < 2 CAP
+        const float cth = (float)in.col_th;
+        // A batch: the n <= 64 oldest candidates, one per lane. start() reads
+        // them and issues their cell reads; complete() decides them into the
+        // output segment. A step starts one at its top, before its own work, and
+        // completes it once its own reads are out, so the cell reads' latency
+        // overlaps the step. Every step starts one, empty or not, and every lane
+        // issues the two reads (lanes without an open candidate read the first
+        // cell), so the reads in flight are the same on every path.
+        CandRead pend;
+        uint32_t ptag = 0;
+        const auto cells = ATAC_GLOBAL(G.cells);
+        auto start = [&](int n) {
+            uint32_t off = 0;
             if (lane < n) {
-                const float4 e = cand[b + lane];
-                tag = __float_as_uint(e.w);
-                act = candidate_active(G, in, e.x, e.y, e.z, (tag >> 15) != 0u, slow);
+                const float4 e = cand[ring(rd + lane)];
+                ptag = __float_as_uint(e.w);
+                off = candidate_locate(G, in, e.x, e.y, e.z, (ptag >> 15) != 0u, pend);
             }
+            pend.lo = load_f4_at(cells, off);
+            pend.hi = load_f4_at(cells, off + 16u);
+            rd = ring(rd + n);
+            occ -= n;
+            npend = n;
+        };
+        auto complete = [&]() {
+            bool act = false, slow = false;
+            if (lane < npend) act = candidate_finish(pend, cth, slow);
             const uint64_t bal = __ballot(act);
-            if (act) q[wcount + __popcll(bal & lt_mask)] = (uint16_t)((tag & 0x7fffu) | (slow ? 0x8000u : 0u));
+            if (act) q[wcount + __popcll(bal & lt_mask)] = (uint16_t)((ptag & 0x7fffu) | (slow ? 0x8000u : 0u));
             wcount += __popcll(bal);
+            npend = 0;
         };
         // one step: points (cx, cy, cz) at o0 are tested while the next ones load
         // into (nx, ny, nz); the loop alternates the two buffers, so no register
         // copy waits for the loads in flight
         auto step = [&](int o0, const float (&cx)[PPL], const float (&cy)[PPL], const float (&cz)[PPL],
                         float (&nx)[PPL], float (&ny)[PPL], float (&nz)[PPL]) {
-            const int o = o0 + lane * PPL;
+            start(occ >= 64 ? 64 : 0);
             unsigned valid = 0;
 #pragma unroll
             for (int k = 0; k < PPL; ++k)
-                if (o + k < we) valid |= 1u << k;
+                if (o0 + 64 * k + lane < we) valid |= 1u << k;
             unsigned op;
             const unsigned m = detect_candidates<PPL>(G, in, cx, cy, cz, valid, all_active, op, [] {}, [&]() {
                 __builtin_amdgcn_sched_barrier(0);
-                load_points<PPL>(px, py, pz, o + 64 * PPL, we, nx, ny, nz);
+                load_points<PPL>(px, py, pz, o0 + 64 * PPL, lane, we, nx, ny, nz);
                 __builtin_amdgcn_sched_barrier(0);
+                complete();   // its cell reads went out before this step's reads
             });
-            const int c = __popc(m);
-            int pre = 0, tot = 0;
+            // appended in point order: point k of every lane (lane order) before point k + 1
+            int pos = rd + occ;   // stays < 2 CAP: occ + (this step's candidates) <= CAP
 #pragma unroll
-            for (int bit = 0; bit < NB; ++bit) {
-                const uint64_t bb = __ballot((c >> bit) & 1);
-                pre += __popcll(bb & lt_mask) << bit;
-                tot += __popcll(bb) << bit;
+            for (int k = 0; k < PPL; ++k) {
+                const bool is_c = ((m >> k) & 1u) != 0u;
+                const uint64_t bal = __ballot(is_c);
+                if (is_c)
+                    cand[ring(pos + __popcll(bal & lt_mask))] = make_float4(
+                        cx[k], cy[k], cz[k], __uint_as_float((uint32_t)(o0 + 64 * k + lane) | (((op >> k) & 1u) << 15)));
+                pos += __popcll(bal);
             }
-            int pos = ncand + pre;
-#pragma unroll
-            for (int k = 0; k < PPL; ++k)
-                if ((m >> k) & 1u)
-                    cand[pos++] =
-                        make_float4(cx[k], cy[k], cz[k], __uint_as_float((uint32_t)(o + k) | (((op >> k) & 1u) << 15)));
-            ncand += tot;
-            if (ncand >= 64) {
-                int b = 0;
-                for (; ncand - b >= 64; b += 64) finish(b, 64);
-                // the < 64 left move to the front, one per lane (b >= 64 > left: no overlap)
-                const int left = ncand - b;
-                float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                if (lane < left) e = cand[b + lane];
-                if (lane < left) cand[lane] = e;
-                ncand = left;
+            occ = pos - rd;
+            // the next step takes at most 64 * PPL and starts by taking 64 if it
+            // has them: room for it while occ <= 128 (always with one point per
+            // lane); beyond (rare), batches finish here
+            if constexpr (PPL > 1) {
+                while (occ > 128) {
+                    start(64);
+                    complete();
+                }
             }
         };
         float x0[PPL], y0[PPL], z0[PPL], x1[PPL], y1[PPL], z1[PPL];
-        load_points<PPL>(px, py, pz, wb + lane * PPL, we, x0, y0, z0);
+        load_points<PPL>(px, py, pz, wb, lane, we, x0, y0, z0);
         for (int o0 = wb; o0 < we; o0 += 2 * 64 * PPL) {
             step(o0, x0, y0, z0, x1, y1, z1);
             if (o0 + 64 * PPL >= we) break;
             step(o0 + 64 * PPL, x1, y1, z1, x0, y0, z0);
         }
-        if (ncand > 0) finish(0, ncand);
+        while (occ > 0) {
+            start(occ < 64 ? occ : 64);
+            complete();
+        }
         if (lane == 0) s_wcount[w] = wcount;
         __syncthreads();
         int cnt = 0;
@@ -548,10 +560,9 @@ __global__ void build_cells_kernel(GridDev G, float4* cells) {
 }
 
 __global__ void build_blocks_kernel(GridDev G, float2* blk) {
-    const int64_t nb0 = G.nb[0], nb1 = G.nb[1], nb = nb0 * nb1 * G.nb[2];
-    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b <= nb; b += (int64_t)gridDim.x * blockDim.x)
-        blk[b] = b < nb ? block_bounds(G, (int)(b % nb0), (int)((b / nb0) % nb1), (int)(b / (nb0 * nb1)))
-                        : make_float2(INFINITY, INFINITY);   // points outside the box
+    const int64_t ne = (int64_t)blk_outside(G);   // + the outside entry
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e <= ne; e += (int64_t)gridDim.x * blockDim.x)
+        blk[e] = blk_fill(G, (uint32_t)e);
 }
 
 hipError_t launch_grid_setup(const GridDev& G, float4* cells, float2* blk, hipStream_t st) {

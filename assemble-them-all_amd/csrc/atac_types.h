@@ -22,12 +22,15 @@ struct GridDev {
     // conservative bounds of the detection value per block of kBlk^3 cells: the
     // (min, max) of every node the block's cells read (with the detection's clamp
     // at the last node) widened by kBlkTol * max(|min|, |max|) and rounded
-    // outward, or (-inf, +inf) if one of them is not finite; one more entry after
-    // the nb[0]*nb[1]*nb[2] blocks, (+inf, +inf), is the bound of points outside
-    // the box
+    // outward, or (-inf, +inf) if one of them is not finite; blocks grouped in
+    // bricks of 2x2x2, then one more entry, (+inf, +inf), the bound of points
+    // outside the box (blk_entry / blk_outside)
     const float2* blk;
     int32_t nb[3];      // blocks per axis: (n - 1) / kBlk + 1
     int32_t pad;
+    // the FD stencil's reach in cell units, 1e-5 * inv_d + 1e-9 (fd_m), and 1 - fd_m
+    double fd_m[3];
+    double fd_1m[3];
 };
 
 constexpr int kBlk = 2;

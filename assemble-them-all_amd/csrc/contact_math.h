@@ -270,6 +270,29 @@ ATAC_HD void pack_cell(const GridDev& G, int i, int j, int k, float4 out[2]) {
     out[1] = make_float4(v[b + dk], v[b + dk + di], v[b + dk + dj], v[b + dk + dj + di]);
 }
 
+// Saturating FP64 -> int32 truncation (NaN -> 0), the hardware conversion's rule.
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ int cvt_sat_i32(double c) {
+    int r;
+    asm("v_cvt_i32_f64 %0, %1" : "=v"(r) : "v"(c));
+    return r;
+}
+// a * b + c for a, b below 2^24 (the full-rate 24-bit multiply-add)
+__device__ __forceinline__ uint32_t mad24(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t r;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+#else
+inline int cvt_sat_i32(double c) {
+    if (c != c) return 0;
+    if (c >= 2147483647.0) return 2147483647;
+    if (c <= -2147483648.0) return -2147483647 - 1;
+    return (int)c;
+}
+inline uint32_t mad24(uint32_t a, uint32_t b, uint32_t c) { return (a & 0xffffffu) * (b & 0xffffffu) + c; }
+#endif
+
 // FP32 roundings of an FP64 value toward -inf and toward +inf.
 ATAC_HD float f32_down(double x) {
     float f = (float)x;
@@ -304,6 +327,26 @@ ATAC_HD float2 block_bounds(const GridDev& G, int bi, int bj, int bk) {
     if (!finite) return make_float2(-INFINITY, INFINITY);
     const double m = kBlkTol * fmax(fabs((double)lo), fabs((double)hi));
     return make_float2(f32_down((double)lo - m), f32_up((double)hi + m));
+}
+
+// Block table layout: blocks in bricks of 2x2x2 (8 entries, one 64-byte line),
+// bricks x-fastest, so the blocks of nearby points share lines. After the bricks
+// comes the entry of points outside the box.
+ATAC_HD uint32_t blk_bricks(const GridDev& G, int a) { return ((uint32_t)G.nb[a] + 1u) >> 1; }
+ATAC_HD uint32_t blk_outside(const GridDev& G) { return blk_bricks(G, 0) * blk_bricks(G, 1) * blk_bricks(G, 2) * 8u; }
+ATAC_HD uint32_t blk_entry(const GridDev& G, uint32_t bx, uint32_t by, uint32_t bz) {
+    const uint32_t brick = mad24(mad24(bz >> 1, blk_bricks(G, 1), by >> 1), blk_bricks(G, 0), bx >> 1);
+    return brick * 8u + ((bz & 1u) << 2 | (by & 1u) << 1 | (bx & 1u));
+}
+// Entry e (<= blk_outside) of the table: the block's bounds, (+inf, +inf) for
+// the outside entry and for the padding of the bricks past the last block.
+ATAC_HD float2 blk_fill(const GridDev& G, uint32_t e) {
+    if (e >= blk_outside(G)) return make_float2(INFINITY, INFINITY);
+    const uint32_t brick = e >> 3, t0 = blk_bricks(G, 0), t1 = blk_bricks(G, 1);
+    const uint32_t bx = 2u * (brick % t0) + (e & 1u), by = 2u * ((brick / t0) % t1) + ((e >> 1) & 1u),
+                   bz = 2u * (brick / (t0 * t1)) + ((e >> 2) & 1u);
+    if (bx >= (uint32_t)G.nb[0] || by >= (uint32_t)G.nb[1] || bz >= (uint32_t)G.nb[2]) return make_float2(INFINITY, INFINITY);
+    return block_bounds(G, (int)bx, (int)by, (int)bz);
 }
 
 // The thresholds block_decision compares an instance's block bounds with:
@@ -436,28 +479,6 @@ ATAC_HD int block_decision(float2 b, const InstDev& in) {   // 1 active, 0 inact
     return b.x > in.blk_off ? 0 : (b.y < in.blk_on ? 1 : -1);
 }
 
-// Saturating FP64 -> int32 truncation (NaN -> 0), the hardware conversion's rule.
-#if defined(__HIP_DEVICE_COMPILE__)
-__device__ __forceinline__ int cvt_sat_i32(double c) {
-    int r;
-    asm("v_cvt_i32_f64 %0, %1" : "=v"(r) : "v"(c));
-    return r;
-}
-// a * b + c for a, b below 2^24 (the full-rate 24-bit multiply-add)
-__device__ __forceinline__ uint32_t mad24(uint32_t a, uint32_t b, uint32_t c) {
-    uint32_t r;
-    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
-    return r;
-}
-#else
-inline int cvt_sat_i32(double c) {
-    if (c != c) return 0;
-    if (c >= 2147483647.0) return 2147483647;
-    if (c <= -2147483648.0) return -2147483647 - 1;
-    return (int)c;
-}
-inline uint32_t mad24(uint32_t a, uint32_t b, uint32_t c) { return (a & 0xffffffu) * (b & 0xffffffu) + c; }
-#endif
 
 ATAC_HD bool detect_active(const GridDev& G, const InstDev& in, float px, float py, float pz, double cc[3]) {
     grid_coords(in, px, py, pz, cc);
@@ -469,7 +490,7 @@ ATAC_HD bool detect_active(const GridDev& G, const InstDev& in, float px, float 
     i = i < G.n[0] - 1 ? i : G.n[0] - 1;
     j = j < G.n[1] - 1 ? j : G.n[1] - 1;
     k = k < G.n[2] - 1 ? k : G.n[2] - 1;
-    const int dec = block_decision(ATAC_GLOBAL(G.blk)[i / kBlk + G.nb[0] * (j / kBlk + G.nb[1] * (k / kBlk))], in);
+    const int dec = block_decision(ATAC_GLOBAL(G.blk)[blk_entry(G, i / kBlk, j / kBlk, k / kBlk)], in);
     if (dec >= 0) return dec == 1;
     return detect_interp(G, in, px, py, pz, cc);
 }
@@ -482,20 +503,29 @@ ATAC_HD bool detect_active(const GridDev& G, const InstDev& in, float px, float 
 // fast_cell_c without the outputs: does the FD stencil of c stay in its cell?
 ATAC_HD bool fast_cell_test(const GridDev& G, const InstDev& in, const double c[3]);
 
+// The FD stencil's reach in cell units (GridDev::fd_m).
+ATAC_HD void set_fd_reach(GridDev& G) {
+    for (int a = 0; a < 3; ++a) {
+        G.fd_m[a] = kFdEps * G.inv_d[a] + kCellMargin;
+        G.fd_1m[a] = 1.0 - G.fd_m[a];
+    }
+}
+
 // fast_cell in grid-index coordinates c = (x - min_box) / d (the detection map):
 // the cell is interior, and the FD stencil (+-1e-5 along each axis) stays inside
 // both the cell and [min_box, max_box] with kCellMargin cell units to spare.
+// Straight-line (no early exit): for c >= 0 the truncation is the floor, and a
+// saturated or NaN conversion fails the tests as the floor would.
 ATAC_HD bool fast_cell_c(const GridDev& G, const InstDev& in, const double c[3], int idx[3], double fd[3]) {
+    bool ok = true;
     for (int a = 0; a < 3; ++a) {
-        const double fl = floor(c[a]);
-        if (!(fl >= 0.0 && fl <= (double)(G.n[a] - 2))) return false;
-        const double fr = c[a] - fl;
-        const double m = kFdEps * G.inv_d[a] + kCellMargin;
-        if (fr < m || fr > 1.0 - m || c[a] + m > in.cmax[a]) return false;
-        idx[a] = (int)fl;
+        const int i = cvt_sat_i32(c[a]);
+        const double fr = c[a] - (double)i, m = G.fd_m[a];
+        ok = ok && c[a] >= 0.0 && i <= G.n[a] - 2 && fr >= m && fr <= G.fd_1m[a] && c[a] + m <= in.cmax[a];
+        idx[a] = i;
         fd[a] = fr;
     }
-    return true;
+    return ok;
 }
 
 ATAC_HD bool fast_cell_test(const GridDev& G, const InstDev& in, const double c[3]) {
@@ -546,7 +576,7 @@ ATAC_HD unsigned detect_candidates(const GridDev& G, const InstDev& in, const fl
                                    const float (&z)[PPL], unsigned valid, bool all_active, unsigned& open,
                                    F0&& before, F&& issue) {
     static_assert(kBlk == 2, "block index = cell index / 2");
-    const uint32_t outside = (uint32_t)G.nb[0] * (uint32_t)G.nb[1] * (uint32_t)G.nb[2] * 8u;
+    const uint32_t outside = blk_outside(G) * 8u;
     unsigned inside = 0;
     uint32_t boff[PPL];
 #pragma unroll
@@ -562,8 +592,7 @@ ATAC_HD unsigned detect_candidates(const GridDev& G, const InstDev& in, const fl
             bi[a] = (uint32_t)(i < G.n[a] - 1 ? i : G.n[a] - 1) >> 1;
         }
         if (ins) inside |= 1u << k;
-        const uint32_t b = mad24(mad24(bi[2], (uint32_t)G.nb[1], bi[1]), (uint32_t)G.nb[0], bi[0]);
-        boff[k] = ins ? b * 8u : outside;
+        boff[k] = ins ? blk_entry(G, bi[0], bi[1], bi[2]) * 8u : outside;
     }
     sched_fence();   // the points are in: the reads below wait for nothing earlier
     before();
@@ -606,20 +635,34 @@ struct CandRead {
     bool open, off_cell;
 };
 
-ATAC_HD void candidate_read(const GridDev& G, const InstDev& in, float x, float y, float z, bool open, CandRead& r) {
+// candidate_read without the cell read: returns the byte offset of the open
+// candidate's cell in the table (0 for a settled one, whose cell is not read).
+ATAC_HD uint32_t candidate_locate(const GridDev& G, const InstDev& in, float x, float y, float z, bool open,
+                                  CandRead& r) {
+    // a candidate is inside the box (c >= 0): the truncation is the floor. One
+    // truncation per axis serves the cell (clamped at the last node, as
+    // detect_value clamps), its fractions and fast_cell_c's test: the fraction in
+    // the clamped cell is fast_cell_c's whenever that test can pass (i <= n - 2).
     double c[3];
     grid_coords(in, x, y, z, c);
-    r.off_cell = !fast_cell_test(G, in, c);
+    bool ok = true;
+    uint32_t idx[3];
+    for (int a = 0; a < 3; ++a) {
+        const int i = cvt_sat_i32(c[a]);
+        const int ic = i < G.n[a] - 1 ? i : G.n[a] - 1;
+        const double fr = c[a] - (double)ic, m = G.fd_m[a];
+        ok = ok && c[a] >= 0.0 && i <= G.n[a] - 2 && fr >= m && fr <= G.fd_1m[a] && c[a] + m <= in.cmax[a];
+        idx[a] = (uint32_t)ic;
+        r.f[a] = (float)fr;
+    }
+    r.off_cell = !ok;
     r.open = open;
-    if (open) {   // an open candidate is inside the box; its cell (clamped at the last node) is in the table
-        uint32_t idx[3];
-        for (int a = 0; a < 3; ++a) {
-            int i = (int)c[a];
-            i = i < G.n[a] - 1 ? i : G.n[a] - 1;
-            idx[a] = (uint32_t)i;
-            r.f[a] = (float)(c[a] - (double)i);
-        }
-        const uint32_t off = (idx[0] + (uint32_t)G.n[0] * mad24(idx[2], (uint32_t)G.n[1], idx[1])) * 32u;
+    return open ? (idx[0] + (uint32_t)G.n[0] * mad24(idx[2], (uint32_t)G.n[1], idx[1])) * 32u : 0u;
+}
+
+ATAC_HD void candidate_read(const GridDev& G, const InstDev& in, float x, float y, float z, bool open, CandRead& r) {
+    const uint32_t off = candidate_locate(G, in, x, y, z, open, r);
+    if (open) {
         r.lo = load_f4_at(ATAC_GLOBAL(G.cells), off);
         r.hi = load_f4_at(ATAC_GLOBAL(G.cells), off + 16u);
     }

@@ -191,6 +191,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU oracle work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the mesh -> SDF build and BDF1 stepping measurements (profiling runs)")
     ap.add_argument("--tile-points", type=int, default=0)
     ap.add_argument("--detect-block", type=int, default=0)
     ap.add_argument("--order", type=int, default=0)
@@ -293,10 +295,12 @@ def main():
         cpu, err = cpu_baseline(w, out["fm"].cpu().numpy(), args.cpu_budget, threads)
         log(f"cpu baseline {cpu['value']:.3e} queries/s on {threads} threads; err {err}")
 
-    sdf_build = sdf_build_measurement(not args.no_cpu)
-    log(f"sdf build: {sdf_build}")
-    bdf1 = bdf1_measurement(not args.no_cpu)
-    log(f"bdf1: {bdf1}")
+    sdf_build = bdf1 = None
+    if not args.no_extras:
+        sdf_build = sdf_build_measurement(not args.no_cpu)
+        log(f"sdf build: {sdf_build}")
+        bdf1 = bdf1_measurement(not args.no_cpu)
+        log(f"bdf1: {bdf1}")
 
     line = {
         "metric": METRIC, "value": n_q / ms * 1e3, "unit": "queries/s", "n_gpus": 1, "steps": args.steps,

@@ -32,17 +32,15 @@ static void host_grid(const float* vals, const int32_t n[3], const double minb[3
         G.inv_d[a] = 1.0 / d[a];
         G.nb[a] = (n[a] - 1) / kBlk + 1;
     }
+    set_fd_reach(G);
     {
         const int64_t ci = n[0], cj = n[1], ck = n[2];
         hg.cells.resize(2 * ci * cj * ck);
         for (int64_t c = 0; c < ci * cj * ck; ++c)
             pack_cell(G, (int)(c % ci), (int)((c / ci) % cj), (int)(c / (ci * cj)), &hg.cells[2 * c]);
-        const int64_t nb = (int64_t)G.nb[0] * G.nb[1] * G.nb[2];
-        hg.blk.resize(nb + 1);
-        for (int64_t b = 0; b < nb; ++b)
-            hg.blk[b] = block_bounds(G, (int)(b % G.nb[0]), (int)((b / G.nb[0]) % G.nb[1]),
-                                     (int)(b / ((int64_t)G.nb[0] * G.nb[1])));
-        hg.blk[nb] = make_float2(INFINITY, INFINITY);
+        const int64_t ne = blk_outside(G);
+        hg.blk.resize(ne + 1);
+        for (int64_t e = 0; e <= ne; ++e) hg.blk[e] = blk_fill(G, (uint32_t)e);
         G.cells = hg.cells.data();
         G.blk = hg.blk.data();
         G.fast = 1;
@@ -213,7 +211,7 @@ int hc_detect_decisions(const float* vals, const int32_t n[3], const double minb
         i = i < G.n[0] - 1 ? i : G.n[0] - 1;
         j = j < G.n[1] - 1 ? j : G.n[1] - 1;
         k = k < G.n[2] - 1 ? k : G.n[2] - 1;
-        settled[o] = block_decision(G.blk[i / kBlk + G.nb[0] * (j / kBlk + G.nb[1] * (k / kBlk))], in) >= 0 ? 1 : 0;
+        settled[o] = block_decision(G.blk[blk_entry(G, i / kBlk, j / kBlk, k / kBlk)], in) >= 0 ? 1 : 0;
     }
     return 0;
 }
