@@ -24,7 +24,6 @@
 
 namespace atac {
 
-
 // ------------------------------------------------------------------ wave primitives
 template <int CTRL>
 __device__ __forceinline__ int dpp_i(int v) {
@@ -506,10 +505,12 @@ __global__ __launch_bounds__(256) void vjp_kernel(VjpArgs a) {
 // One wavefront per instance. The lanes first load the instance's tile entries
 // (active count, first work item) 64 tiles at a time; then, lanes over
 // components, the work items' partial rows are summed in tile order and each
-// tile's items in order, the tile entries broadcast from their lanes, so every
-// row load's address is known up front. The order depends only on the tiling:
-// the same result on every run. Every instance's outputs are written (zeros when
-// it has no active contact).
+// tile's items in order, the tile entries broadcast from their lanes, the rows
+// listed kRedGroup at a time so that their reads are in flight together. The
+// order depends only on the tiling: the same result on every run. Every
+// instance's outputs are written (zeros when it has no active contact).
+constexpr int kRedGroup = 16;   // partial rows whose reads a reduce wavefront issues together
+
 __global__ __launch_bounds__(256) void reduce_kernel(ReduceArgs a) {
     const int lane = threadIdx.x & 63;
     const int i = (int)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -523,20 +524,42 @@ __global__ __launch_bounds__(256) void reduce_kernel(ReduceArgs a) {
         const bool mine = comp < a.nacc;
         double sum = 0.0;
         for (int g0 = 0; g0 < nt; g0 += 64) {
-            int cnt = 0, c0 = 0;
-            if (g0 + lane < nt) {
+            int cnt = 0, c0 = 0, nch = 0;
+            if (g0 + lane < nt) {   // both entries read at once (tile_chunk0 is only meaningful when cnt > 0)
                 cnt = a.tile_active[tb + g0 + lane];
-                if (cnt > 0) c0 = a.tile_chunk0[tb + g0 + lane];
+                c0 = a.tile_chunk0[tb + g0 + lane];
+                nch = (cnt + kChunk - 1) / kChunk;
             }
             if (comp0 == 0) nact += cnt;
             const int ng = min(64, nt - g0);
-            for (int t = 0; t < ng; ++t) {
-                const int tc = __builtin_amdgcn_readlane(cnt, t);
-                if (tc <= 0) continue;
-                const int tc0 = __builtin_amdgcn_readlane(c0, t);
-                const int nch = (tc + kChunk - 1) / kChunk;
-                for (int k = 0; k < nch; ++k)
-                    if (mine) sum += part[(int64_t)(tc0 + k) * a.nacc + comp];
+            // the items of tiles [g0, g0 + ng) in order, kRedGroup at a time: a
+            // wave-uniform cursor (tile t, item k) lists their rows, the rows' reads
+            // go out together, and the values are added in list order
+            int t = 0, k = 0;
+            for (;;) {
+                int rows[kRedGroup];
+                int nr = 0;
+#pragma unroll
+                for (int j = 0; j < kRedGroup; ++j) {
+                    while (t < ng && k >= __builtin_amdgcn_readlane(nch, t)) {
+                        ++t;
+                        k = 0;
+                    }
+                    rows[j] = -1;
+                    if (t < ng) {
+                        rows[j] = __builtin_amdgcn_readlane(c0, t) + k;
+                        ++k;
+                        nr = j + 1;
+                    }
+                }
+                double v[kRedGroup];
+#pragma unroll
+                for (int j = 0; j < kRedGroup; ++j)
+                    v[j] = (rows[j] >= 0 && mine) ? part[(int64_t)rows[j] * a.nacc + comp] : 0.0;
+#pragma unroll
+                for (int j = 0; j < kRedGroup; ++j)
+                    if (rows[j] >= 0) sum += v[j];
+                if (nr < kRedGroup) break;
             }
         }
         if (mine) {
