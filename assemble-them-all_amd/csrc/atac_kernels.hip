@@ -58,22 +58,14 @@ __device__ __forceinline__ T wave_sum(T v) {
     return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
 }
 
-// Per-lane wrench accumulator: Neumaier-compensated FP32 for the FP32 contact
-// arithmetic, plain FP64 for the FP64 one.
+// Per-lane wrench accumulator: FP64 for both arithmetics (an FP32 term is
+// converted exactly and added in FP64: two full-rate instructions, and the
+// lane's sum carries no FP32 rounding).
 template <typename R>
-struct LaneAcc;
-template <>
-struct LaneAcc<float> {
-    Comp32 c;
-    __device__ void zero() { c.zero(); }
-    __device__ void add(float v) { c.add(v); }
-    __device__ double value() const { return c.value(); }
-};
-template <>
-struct LaneAcc<double> {
+struct LaneAcc {
     double s;
     __device__ void zero() { s = 0.0; }
-    __device__ void add(double v) { s += v; }
+    __device__ void add(R v) { s += (double)v; }
     __device__ double value() const { return s; }
 };
 
@@ -347,15 +339,33 @@ __global__ __launch_bounds__(64) void slow_kernel(SlowArgs a) {
     }
 }
 
-// Writes n (<= 64) wave-uniform values to dst[0..n) with one store per lane.
+// Column sums of a wavefront's 64 x N lane values through LDS (s: this wave's
+// 64 * N doubles), in a fixed order: lane k < N adds column k as four interleaved
+// sums of rows r = 4i + q, combined as (q0 + q1) + (q2 + q3). Returns lane k's
+// column sum (0 on lanes >= N). One pass for all N values: no per-value wave
+// reduction and no wave-uniform results to hold.
 template <int N>
-__device__ __forceinline__ void store_row(double* dst, const double (&v)[N]) {
-    static_assert(N <= 64, "one value per lane");
+__device__ __forceinline__ double lds_column_sums(double* s, const double (&v)[N]) {
+    static_assert(N <= 64, "one column per lane");
     const int lane = threadIdx.x & 63;
-    double mine = 0.0;
 #pragma unroll
-    for (int k = 0; k < N; ++k) mine = (lane == k) ? v[k] : mine;
-    if (lane < N) dst[lane] = mine;
+    for (int k = 0; k < N; ++k) s[lane * N + k] = v[k];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    double r = 0.0;
+    if (lane < N) {
+        double q[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+        for (int i = 0; i < 64; i += 4)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) q[j] += s[(i + j) * N + lane];
+        r = (q[0] + q[1]) + (q[2] + q[3]);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();   // the buffer is free for the wave's next use
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    return r;
 }
 
 // ------------------------------------------------------------------ batched contact pass
@@ -371,6 +381,7 @@ __global__ __launch_bounds__(256) void contact_kernel(ContactArgs a) {
     constexpr int NACC = NF + (DERIV ? 288 : 0);
     constexpr int U = kChunk / 64;
     __shared__ double s_km[DERIV ? 4 : 1][DERIV ? 288 : 1];
+    __shared__ double s_red[4][64 * NF];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int total = a.counters[CNT_CHUNKS];
     for (int c = wave_id(); c < total; c += wave_count()) {
@@ -447,10 +458,11 @@ __global__ __launch_bounds__(256) void contact_kernel(ContactArgs a) {
             }
         }
         double* dst = a.partial + (int64_t)c * NACC;
-        double row[NF];
+        double lv[NF];
 #pragma unroll
-        for (int k = 0; k < NF; ++k) row[k] = wave_sum(acc[k].value());
-        store_row<NF>(dst, row);
+        for (int k = 0; k < NF; ++k) lv[k] = acc[k].value();
+        const double col = lds_column_sums<NF>(&s_red[w][0], lv);
+        if (lane < NF) dst[lane] = col;
         if (DERIV) {
             __builtin_amdgcn_wave_barrier();
             for (int k = lane; k < 288; k += 64) dst[NF + k] = s_km[DERIV ? w : 0][DERIV ? k : 0];
@@ -465,7 +477,8 @@ __global__ __launch_bounds__(256) void contact_kernel(ContactArgs a) {
 template <typename R>
 __global__ __launch_bounds__(256) void vjp_kernel(VjpArgs a) {
     using Rec = typename RecOf<R>::type;
-    const int lane = threadIdx.x & 63;
+    __shared__ double s_red[4][64 * kVjpOut];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int total = a.counters[CNT_CHUNKS];
     for (int c = wave_id(); c < total; c += wave_count()) {
         const ChunkDev ch = as_const(a.chunks)[c];
@@ -492,12 +505,12 @@ __global__ __launch_bounds__(256) void vjp_kernel(VjpArgs a) {
             cur = nxt;
         }
         double* dst = a.partial + (int64_t)c * kVjpOut;
-        double row[kVjpOut];
+        double lv[kVjpOut];
 #pragma unroll
-        for (int k = 0; k < kVjpOut; ++k) row[k] = wave_sum((double)acc[k]);
-        row[24] *= in.kn_chain;   // d/d kn = kn_chain * d/d kn_eff (ForceGeneralSDFContact.cpp:155-156)
-        row[25] *= in.kn_chain;
-        store_row<kVjpOut>(dst, row);
+        for (int k = 0; k < kVjpOut; ++k) lv[k] = (double)acc[k];
+        double col = lds_column_sums<kVjpOut>(&s_red[w][0], lv);
+        if (lane == 24 || lane == 25) col *= in.kn_chain;   // d/d kn = kn_chain * d/d kn_eff (ForceGeneralSDFContact.cpp:155-156)
+        if (lane < kVjpOut) dst[lane] = col;
     }
 }
 

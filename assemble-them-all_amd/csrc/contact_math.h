@@ -1150,18 +1150,4 @@ ATAC_HD void contact_vjp(const InstR<R>& I, const ContactFwd<R>& s, const R gb[1
     }
 }
 
-// Neumaier-compensated FP32 accumulator
-struct Comp32 {
-    float s, c;
-    ATAC_HD void zero() { s = 0.0f; c = 0.0f; }
-    ATAC_HD void add(float v) {
-        float t = s + v;
-        float big = fabsf(s) >= fabsf(v) ? s : v;
-        float small = fabsf(s) >= fabsf(v) ? v : s;
-        c += (big - t) + small;
-        s = t;
-    }
-    ATAC_HD double value() const { return (double)s + (double)c; }
-};
-
 }  // namespace atac
