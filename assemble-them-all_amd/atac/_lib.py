@@ -30,6 +30,8 @@ EXPORTED = [
     "atac_sdf_distance", "atac_sdf_collision",
     "atac_batch_eval_host", "atac_batch_vjp_host", "atac_sdf_distance_host", "atac_sdf_collision_host",
     "atac_sdf_from_mesh", "atac_sdf_mesh_grid",
+    "atac_mesh_create", "atac_mesh_destroy", "atac_mesh_distance_normal", "atac_mesh_distance_normal_host",
+    "atac_bvh_force_host",
 ]
 
 
@@ -110,6 +112,11 @@ def lib():
     fp = C.POINTER(C.c_float)
     L.atac_sdf_from_mesh.argtypes = [vp, i64, vp, i64, fp, C.c_float, C.c_float, C.c_float, i32, i32, i32, i32, vp]
     L.atac_sdf_mesh_grid.argtypes = [vp, i64, C.c_double, i32, fp, fp, dp, C.POINTER(i32)]
+    L.atac_mesh_create.argtypes = [vp, i64, vp, i64, C.POINTER(vp)]
+    L.atac_mesh_destroy.argtypes = [vp]
+    L.atac_mesh_distance_normal.argtypes = [vp, dp, C.c_double, vp, i64, vp, vp, vp, vp]
+    L.atac_mesh_distance_normal_host.argtypes = [vp, dp, C.c_double, vp, i64, vp, vp, vp]
+    L.atac_bvh_force_host.argtypes = [vp, C.POINTER(Instance), vp, i64, u32, vp, vp, vp, vp, C.POINTER(i64)]
     for name in EXPORTED:
         if name not in ("atac_last_error",):
             getattr(L, name).restype = C.c_int

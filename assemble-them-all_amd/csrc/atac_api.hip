@@ -12,6 +12,7 @@
 #include "atac.h"
 #include "atac_compose.h"
 #include "sdf_build.h"
+#include "bvh_host.h"
 #include "atac_kernels.h"
 #include "atac_types.h"
 
@@ -1066,5 +1067,147 @@ int atac_sdf_mesh_grid(const double* V, int64_t nv, double dx, int32_t min_size,
         min_box[a] = lo[a];
         max_box[a] = hi[a];
     }
+    return ATAC_OK;
+}
+
+// ------------------------------------------------------------------ BVH contact variant
+struct atac_mesh_s {
+    MeshDev dev;
+    float4* d_nodes = nullptr;
+    int32_t* d_tri = nullptr;
+    int32_t* d_F = nullptr;
+    int32_t* d_FE = nullptr;
+    double* d_V = nullptr;
+    double* d_FN = nullptr;
+    double* d_EN = nullptr;
+    double* d_VN = nullptr;
+    void release() {
+        dfree(d_nodes); dfree(d_tri); dfree(d_F); dfree(d_FE);
+        dfree(d_V); dfree(d_FN); dfree(d_EN); dfree(d_VN);
+    }
+};
+
+namespace {
+template <typename T>
+int upload(T** d, const std::vector<T>& h) {
+    int rc = dalloc(d, h.size());
+    if (rc) return rc;
+    if (!h.empty()) ATAC_HIP(hipMemcpy(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+    return ATAC_OK;
+}
+}  // namespace
+
+int atac_mesh_create(const double* V, int64_t nv, const int64_t* F, int64_t nf, atac_mesh* out) {
+    if (!out) return fail(ATAC_ERR_INVALID, "atac_mesh_create: out is NULL");
+    *out = nullptr;
+    MeshHost h;
+    const std::string why = mesh_build(V, nv, F, nf, h);
+    if (!why.empty()) return fail(ATAC_ERR_INVALID, "atac_mesh_create: " + why);
+    auto* m = new atac_mesh_s();
+    int rc = upload(&m->d_nodes, h.nodes);
+    if (!rc) rc = upload(&m->d_tri, h.tri);
+    if (!rc) rc = upload(&m->d_F, h.F);
+    if (!rc) rc = upload(&m->d_FE, h.FE);
+    if (!rc) rc = upload(&m->d_V, h.V);
+    if (!rc) rc = upload(&m->d_FN, h.FN);
+    if (!rc) rc = upload(&m->d_EN, h.EN);
+    if (!rc) rc = upload(&m->d_VN, h.VN);
+    if (rc) {
+        m->release();
+        delete m;
+        return rc;
+    }
+    MeshDev& D = m->dev;
+    D.nodes = m->d_nodes; D.tri = m->d_tri; D.V = m->d_V; D.F = m->d_F;
+    D.FN = m->d_FN; D.FE = m->d_FE; D.EN = m->d_EN; D.VN = m->d_VN;
+    for (int a = 0; a < 3; ++a) { D.bmin[a] = h.bmin[a]; D.bmax[a] = h.bmax[a]; }
+    D.n_nodes = (int32_t)(h.nodes.size() / 2);
+    D.nf = (int32_t)nf;
+    *out = m;
+    return ATAC_OK;
+}
+
+int atac_mesh_destroy(atac_mesh m) {
+    if (!m) return ATAC_OK;
+    m->release();
+    delete m;
+    return ATAC_OK;
+}
+
+int atac_mesh_distance_normal(atac_mesh m, const double E2[16], double col_th, const double* xw, int64_t n, double* d,
+                              double* normal, int32_t* face, void* stream) {
+    if (!m || !E2 || n < 0 || (n > 0 && (!xw || !d)))
+        return fail(ATAC_ERR_INVALID, "atac_mesh_distance_normal: bad arguments");
+    MeshQueryArgs a{};
+    a.M = m->dev;
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) a.R2[3 * r + c] = E2[4 * r + c];
+        a.p2[r] = E2[4 * r + 3];
+    }
+    a.col_th = col_th;
+    a.xw = xw; a.n = n; a.d = d; a.normal = normal; a.face = face;
+    ATAC_HIP(launch_mesh_distance(a, (hipStream_t)stream));
+    return ATAC_OK;
+}
+
+int atac_mesh_distance_normal_host(atac_mesh m, const double E2[16], double col_th, const double* xw, int64_t n,
+                                   double* d, double* normal, int32_t* face) {
+    if (!m || !E2 || n < 0 || (n > 0 && (!xw || !d)))
+        return fail(ATAC_ERR_INVALID, "atac_mesh_distance_normal_host: bad arguments");
+    if (n == 0) return ATAC_OK;
+    double *dx = nullptr, *dd = nullptr, *dn = nullptr;
+    int32_t* df = nullptr;
+    int rc = dalloc(&dx, (size_t)(3 * n));
+    if (!rc) rc = dalloc(&dd, (size_t)n);
+    if (!rc && normal) rc = dalloc(&dn, (size_t)(3 * n));
+    if (!rc && face) rc = dalloc(&df, (size_t)n);
+    hipError_t e = hipSuccess;
+    if (!rc) e = hipMemcpy(dx, xw, sizeof(double) * 3 * n, hipMemcpyHostToDevice);
+    if (!rc && e == hipSuccess) rc = atac_mesh_distance_normal(m, E2, col_th, dx, n, dd, dn, df, nullptr);
+    if (!rc && e == hipSuccess) e = hipMemcpy(d, dd, sizeof(double) * n, hipMemcpyDeviceToHost);
+    if (!rc && e == hipSuccess && normal) e = hipMemcpy(normal, dn, sizeof(double) * 3 * n, hipMemcpyDeviceToHost);
+    if (!rc && e == hipSuccess && face) e = hipMemcpy(face, df, sizeof(int32_t) * n, hipMemcpyDeviceToHost);
+    if (!rc && e != hipSuccess) rc = fail(ATAC_ERR_HIP, hipGetErrorString(e));
+    dfree(dx); dfree(dd); dfree(dn); dfree(df);
+    return rc;
+}
+
+int atac_bvh_force_host(atac_mesh m, const atac_instance* inst, const double* xi, int64_t n, uint32_t flags,
+                        double* fm, double* Km, double* Dm, double* dfm_dp6, int64_t* n_active) {
+    if (!m || !inst || !fm || n < 0 || (n > 0 && !xi)) return fail(ATAC_ERR_INVALID, "atac_bvh_force_host: bad arguments");
+    if ((flags & ATAC_EVAL_DERIV) && (!Km || !Dm)) return fail(ATAC_ERR_INVALID, "atac_bvh_force_host: Km / Dm needed");
+    if ((flags & ATAC_EVAL_P6) && !dfm_dp6) return fail(ATAC_ERR_INVALID, "atac_bvh_force_host: dfm_dp6 needed");
+    // the detection map of compose_instance over a unit grid at the origin: body 2's frame
+    GridDev unit{};
+    for (int a = 0; a < 3; ++a) { unit.d[a] = 1.0; unit.inv_d[a] = 1.0; }
+    BvhForceArgs a{};
+    compose_instance(*inst, unit, nullptr, nullptr, nullptr, n, a.in);
+    a.M = m->dev;
+    a.n = n;
+    a.flags = flags;
+    double *dx = nullptr, *dout = nullptr;
+    int64_t* dna = nullptr;
+    int rc = dalloc(&dx, (size_t)std::max<int64_t>(3 * n, 1));
+    if (!rc) rc = dalloc(&dout, (size_t)kBvhOut);
+    if (!rc) rc = dalloc(&dna, 1);
+    hipError_t e = hipSuccess;
+    std::vector<double> h(kBvhOut);
+    int64_t na = 0;
+    if (!rc && n > 0) e = hipMemcpy(dx, xi, sizeof(double) * 3 * n, hipMemcpyHostToDevice);
+    if (!rc && e == hipSuccess) {
+        a.xi = dx; a.out = dout; a.n_active = dna;
+        e = launch_bvh_force(a, nullptr);
+    }
+    if (!rc && e == hipSuccess) e = hipMemcpy(h.data(), dout, sizeof(double) * kBvhOut, hipMemcpyDeviceToHost);
+    if (!rc && e == hipSuccess) e = hipMemcpy(&na, dna, sizeof(int64_t), hipMemcpyDeviceToHost);
+    if (!rc && e != hipSuccess) rc = fail(ATAC_ERR_HIP, hipGetErrorString(e));
+    dfree(dx); dfree(dout); dfree(dna);
+    if (rc) return rc;
+    for (int k = 0; k < 12; ++k) fm[k] = h[k];
+    if (flags & ATAC_EVAL_P6)
+        for (int k = 0; k < 12; ++k) dfm_dp6[k] = h[12 + k];
+    if (flags & ATAC_EVAL_DERIV)
+        for (int k = 0; k < 144; ++k) { Km[k] = h[24 + k]; Dm[k] = h[168 + k]; }
+    if (n_active) *n_active = na;
     return ATAC_OK;
 }

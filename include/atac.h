@@ -31,6 +31,7 @@ extern "C" {
 typedef struct atac_grid_s* atac_grid;     /* FP32 SDF grid resident in HBM */
 typedef struct atac_points_s* atac_points; /* Morton-sorted SoA FP32 contact points */
 typedef struct atac_batch_s* atac_batch;   /* instance table, tile plan, workspace */
+typedef struct atac_mesh_s* atac_mesh;     /* triangle mesh + BVH + pseudonormals (BVH contact variant) */
 
 /* One force object: ForceGeneralSDFContact(contact_body, SDF_body, kn, kt, mu, damping)
  * with its _scale, the contact body's _contact_scale, the SDF body's _col_th and the
@@ -211,6 +212,39 @@ int atac_sdf_mesh_grid(const double* V, int64_t nv, double dx, int32_t min_size,
                        double d[3], int32_t n[3]);
 int atac_sdf_collision_host(atac_grid g, const double E2[16], const double phi2[6], double col_th, uint32_t flags,
                             const double* xw, const double* xw_dot, int64_t n, double* values, double* derivs);
+
+/* ---- BVH contact variant ----
+ * The triangle mesh of a BVH body in its body frame, prepared for exact signed
+ * distance queries: a BVH, face normals, uniform edge normals and angle-weighted
+ * vertex normals, as BodyBVHObj sets up libigl's AABB tree
+ * (Body/BVH/BVHEngine.h:33-55, Body/BVH/AABBTree.h:56-60; BodyMeshObj.cpp:179-182 for
+ * the bounding box). V: 3*nv FP64 coordinates; F: 3*nf vertex indices (< nv). */
+int atac_mesh_create(const double* V, int64_t nv, const int64_t* F, int64_t nf, atac_mesh* out);
+int atac_mesh_destroy(atac_mesh m);
+
+/* BodyBVHObj::distance_normal_parallel (Body/BodyBVHObj.cpp:322-356): for each world
+ * point xw[i] (3*n FP64) with the body at E2, d = signed distance + col_th inside the
+ * vertices' bounding box (libigl signed_distance_pseudonormal: closest point over
+ * all triangles, sign from the pseudonormal of the closest feature) and 10 outside;
+ * normal (3*n) = that pseudonormal (0 outside); face (n) = the closest triangle
+ * (-1 outside). normal and face may be NULL. Device pointers, asynchronous on stream. */
+int atac_mesh_distance_normal(atac_mesh m, const double E2[16], double col_th, const double* xw, int64_t n, double* d,
+                              double* normal, int32_t* face, void* stream);
+/* The same with host pointers (synchronous). */
+int atac_mesh_distance_normal_host(atac_mesh m, const double E2[16], double col_th, const double* xw, int64_t n,
+                                   double* d, double* normal, int32_t* face);
+
+/* ForceGeneralBVHContact::computeForce / computeForceWithDerivative
+ * (Force/ForceGeneralBVHContact.cpp:46-219 and its derivative overloads): the
+ * contacts of the contact body's points xi (3*n FP64, body frame) against the mesh
+ * (collision_detection_general_BVH, CollisionDetection/CollisionDetection.cpp:150-185:
+ * inside the bounding box and d = signed distance + col_th < 0), the wrench fm[12]
+ * (contact body then BVH body, each in its own frame), with ATAC_EVAL_P6 the
+ * contact-scale column dfm_dp6[12], with ATAC_EVAL_DERIV Km[144] and Dm[144]
+ * (row-major, w.r.t. [q1; q2] and [phi1; phi2]); n_active (may be NULL) = contacts.
+ * inst->grid is ignored. FP64 arithmetic, a fixed summation order. Host pointers. */
+int atac_bvh_force_host(atac_mesh m, const atac_instance* inst, const double* xi, int64_t n, uint32_t flags,
+                        double* fm, double* Km, double* Dm, double* dfm_dp6, int64_t* n_active);
 
 #ifdef __cplusplus
 }

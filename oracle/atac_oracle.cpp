@@ -33,6 +33,8 @@
 #include <cstring>
 #include <limits>
 #include <vector>
+#include <map>
+#include <array>
 #include <algorithm>
 #ifdef _OPENMP
 #include <omp.h>
@@ -289,14 +291,16 @@ struct SDFBody {
     int mode;    // 0 reference FD, 1 analytic
     M3 R() const { return E.block<3, 3>(0, 0); }
     V3 p() const { return E.block<3, 1>(0, 3); }
+    double value(const V3& x) const { return query_SDF(G, x, false); }
     V3 dSDF(const V3& x) const { return mode == 0 ? query_dSDF_fd(G, x) : query_dSDF_analytic(G, x); }
     M3 ddSDF(const V3& x) const { return mode == 0 ? query_ddSDF_fd(G, x) : query_ddSDF_analytic(G, x); }
 };
 
 // BodySDFObj::distance, BodySDFObj.cpp:207-213
-double distance(const SDFBody& B, const V3& xw) {
+template <class Body>
+double distance(const Body& B, const V3& xw) {
     V3 x = B.R().T() * (xw - B.p());
-    return query_SDF(B.G, x, false) + B.col_th;
+    return B.value(x) + B.col_th;
 }
 
 struct CollisionOut {
@@ -310,14 +314,15 @@ struct CollisionOut {
 };
 
 // BodySDFObj::collision (values only), BodySDFObj.cpp:239-274
-void collision(const SDFBody& B, const V3& xw, const V3& xw_dot, CollisionOut& o) {
+template <class Body>
+void collision(const Body& B, const V3& xw, const V3& xw_dot, CollisionOut& o) {
     M3 I = M3::identity();
     M3 R2 = B.R();
     V3 p2 = B.p();
-    V3 w2_dot = B.phi.block<3, 1>(0, 0);
-    V3 v2_dot = B.phi.block<3, 1>(3, 0);
+    V3 w2_dot = B.phi.template block<3, 1>(0, 0);
+    V3 v2_dot = B.phi.template block<3, 1>(3, 0);
     V3 x = R2.T() * (xw - p2);
-    o.d = query_SDF(B.G, x, false) + B.col_th;
+    o.d = B.value(x) + B.col_th;
     V3 dd_dx = B.dSDF(x);
     V3 e = dd_dx / dd_dx.norm();
     o.n = R2 * e;
@@ -330,15 +335,16 @@ void collision(const SDFBody& B, const V3& xw, const V3& xw_dot, CollisionOut& o
 }
 
 // BodySDFObj::collision (values and derivatives), BodySDFObj.cpp:278-377
-void collision_deriv(const SDFBody& B, const V3& xw, const V3& xw_dot, CollisionOut& o) {
+template <class Body>
+void collision_deriv(const Body& B, const V3& xw, const V3& xw_dot, CollisionOut& o) {
     M3 I = M3::identity();
     M3 R2 = B.R();
     V3 p2 = B.p();
-    V3 w2_dot = B.phi.block<3, 1>(0, 0);
-    V3 v2_dot = B.phi.block<3, 1>(3, 0);
+    V3 w2_dot = B.phi.template block<3, 1>(0, 0);
+    V3 v2_dot = B.phi.template block<3, 1>(3, 0);
     V3 x = R2.T() * (xw - p2);
 
-    o.d = query_SDF(B.G, x, false) + B.col_th;
+    o.d = B.value(x) + B.col_th;
     V3 dd_dx = B.dSDF(x);
     V3 e = dd_dx / dd_dx.norm();
     o.n = R2 * e;
@@ -441,7 +447,8 @@ void detect(const M4& E1, const SDFBody& B, const double* pts, int64_t n, std::v
 }
 
 // ForceGeneralSDFContact::computeForce(contacts, fm), ForceGeneralSDFContact.cpp:147-219
-void compute_force(const std::vector<Contact>& contacts, const M4& E1, const V6& phi1, const SDFBody& B,
+template <class Body>
+void compute_force(const std::vector<Contact>& contacts, const M4& E1, const V6& phi1, const Body& B,
                    const ForceParams& P, V12& fm, double* scale = nullptr) {
     const double eps = 1e-8;  // constants::eps
     M3 R1 = E1.block<3, 3>(0, 0);
@@ -502,7 +509,8 @@ static Mat<6, C> dGammaT_times(const Mat<3, C>& dxi2, const V3& y) {
 // ForceGeneralSDFContact::computeForceWithDerivative(contacts, fm, Km, Dm[, dfm_dp]),
 // ForceGeneralSDFContact.cpp:221-447; the p6 (contact scale) column follows :693-699 and the
 // friction branches' p6 lines (:761-765, :815-819, :869-873, :953-957, :1017-1021).
-void compute_force_deriv(const std::vector<Contact>& contacts, const M4& E1, const V6& phi1, const SDFBody& B,
+template <class Body>
+void compute_force_deriv(const std::vector<Contact>& contacts, const M4& E1, const V6& phi1, const Body& B,
                          const ForceParams& P, V12& fm, M12& Km, M12& Dm, V12* dfm_dp6, double* scale = nullptr) {
     const double eps = 1e-8;
     fm = V12::zero(); Km = M12::zero(); Dm = M12::zero();
@@ -637,6 +645,159 @@ void compute_force_deriv(const std::vector<Contact>& contacts, const M4& E1, con
             scale[1] += (Km - Km0).norm();
             scale[2] += (Dm - Dm0).norm();
         }
+    }
+}
+
+// ---------------------------------------------------------------- BVH contact variant
+// BodyBVHObj's sampler (Body/BodyBVHObj.cpp:285-356 over Body/BVH/AABBTree.h:74-82):
+// libigl's signed_distance_pseudonormal. libigl is not vendored in the reference;
+// its published algorithm is restated here: the closest point over all triangles
+// (brute force; equal distances resolve to the smallest face index), barycentric
+// coordinates of that point recomputed from it (libigl barycentric_coordinates),
+// the feature they name (<= 1e-12: vertex, edge or face; libigl
+// pseudonormal_test) and that feature's normal: face normal, edge normal = the
+// normalised sum of the adjacent face normals (per_edge_normals, uniform), vertex
+// normal = the normalised angle-weighted sum (per_vertex_normals, angle; angles by
+// the law of cosines as libigl's internal_angles). Sign = +1 when (p - c) . n >= 0.
+struct Mesh {
+    std::vector<V3> V;
+    std::vector<std::array<int64_t, 3>> F;
+    std::vector<V3> FN, EN, VN;
+    std::vector<std::array<int64_t, 3>> FE;   // edge opposite corner c
+    V3 bmin, bmax;
+};
+
+static V3 normalized(const V3& v) {
+    const double r = std::sqrt(dot(v, v));
+    return r > 0.0 ? v / r : v;
+}
+
+void mesh_prepare(Mesh& m) {
+    const size_t nf = m.F.size(), nv = m.V.size();
+    for (int a = 0; a < 3; ++a) { m.bmin[a] = INFINITY; m.bmax[a] = -INFINITY; }
+    for (const V3& v : m.V)
+        for (int a = 0; a < 3; ++a) { m.bmin[a] = std::min(m.bmin[a], v[a]); m.bmax[a] = std::max(m.bmax[a], v[a]); }
+    m.FN.assign(nf, V3::zero());
+    for (size_t f = 0; f < nf; ++f) {
+        const V3 &a = m.V[m.F[f][0]], &b = m.V[m.F[f][1]], &c = m.V[m.F[f][2]];
+        m.FN[f] = normalized(cross(b - a, c - a));
+    }
+    std::map<std::pair<int64_t, int64_t>, int64_t> eid;
+    m.FE.assign(nf, {0, 0, 0});
+    for (size_t f = 0; f < nf; ++f)
+        for (int c = 0; c < 3; ++c) {
+            int64_t u = m.F[f][(c + 1) % 3], w = m.F[f][(c + 2) % 3];
+            if (u > w) std::swap(u, w);
+            m.FE[f][c] = eid.emplace(std::make_pair(u, w), (int64_t)eid.size()).first->second;
+        }
+    m.EN.assign(eid.size(), V3::zero());
+    for (size_t f = 0; f < nf; ++f)
+        for (int c = 0; c < 3; ++c) m.EN[m.FE[f][c]] += m.FN[f];
+    for (V3& e : m.EN) e = normalized(e);
+    m.VN.assign(nv, V3::zero());
+    for (size_t f = 0; f < nf; ++f) {
+        double l2[3];   // squared length of the edge opposite corner c
+        for (int c = 0; c < 3; ++c) {
+            const V3 e = m.V[m.F[f][(c + 1) % 3]] - m.V[m.F[f][(c + 2) % 3]];
+            l2[c] = dot(e, e);
+        }
+        for (int c = 0; c < 3; ++c) {
+            const double a2 = l2[(c + 1) % 3], b2 = l2[(c + 2) % 3];
+            double cs = (a2 > 0.0 && b2 > 0.0) ? (a2 + b2 - l2[c]) / (2.0 * std::sqrt(a2) * std::sqrt(b2)) : 1.0;
+            cs = std::min(1.0, std::max(-1.0, cs));
+            m.VN[m.F[f][c]] += std::acos(cs) * m.FN[f];
+        }
+    }
+    for (V3& v : m.VN) v = normalized(v);
+}
+
+// closest point of triangle (a, b, c) to p by its Voronoi regions (Ericson 5.1.5)
+static V3 closest_point_triangle(const V3& p, const V3& a, const V3& b, const V3& c) {
+    const V3 ab = b - a, ac = c - a, ap = p - a;
+    const double d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0.0 && d2 <= 0.0) return a;
+    const V3 bp = p - b;
+    const double d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0.0 && d4 <= d3) return b;
+    const double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) return a + (d1 / (d1 - d3)) * ab;
+    const V3 cp = p - c;
+    const double d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0.0 && d5 <= d6) return c;
+    const double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) return a + (d2 / (d2 - d6)) * ac;
+    const double va = d3 * d6 - d5 * d4;
+    if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0)
+        return b + ((d4 - d3) / ((d4 - d3) + (d5 - d6))) * (c - b);
+    const double den = 1.0 / (va + vb + vc);
+    return a + (vb * den) * ab + (vc * den) * ac;
+}
+
+// signed distance (without col_th), pseudonormal and face of p (body frame)
+double mesh_query(const Mesh& m, const V3& p, V3& n, int64_t& face) {
+    double best = INFINITY;
+    int64_t bf = -1;
+    V3 bc = p;
+    for (size_t f = 0; f < m.F.size(); ++f) {
+        const V3 c = closest_point_triangle(p, m.V[m.F[f][0]], m.V[m.F[f][1]], m.V[m.F[f][2]]);
+        const V3 pc = p - c;
+        const double d2 = dot(pc, pc);
+        if (d2 < best) { best = d2; bf = (int64_t)f; bc = c; }
+    }
+    face = bf;
+    const V3 &A = m.V[m.F[bf][0]], &B = m.V[m.F[bf][1]], &Cc = m.V[m.F[bf][2]];
+    const V3 v0 = B - A, v1 = Cc - A, v2 = bc - A;
+    const double d00 = dot(v0, v0), d01 = dot(v0, v1), d11 = dot(v1, v1), d20 = dot(v2, v0), d21 = dot(v2, v1);
+    const double den = d00 * d11 - d01 * d01;
+    const double bv = (d11 * d20 - d01 * d21) / den, bw = (d00 * d21 - d01 * d20) / den;
+    const double b[3] = {1.0 - bv - bw, bv, bw};
+    const double eps = 1e-12;
+    const int nz = (b[0] <= eps) + (b[1] <= eps) + (b[2] <= eps);
+    n = m.FN[bf];
+    if (nz >= 2) {
+        for (int c = 0; c < 3; ++c)
+            if (b[c] > eps) { n = m.VN[m.F[bf][c]]; break; }
+    } else if (nz == 1) {
+        for (int c = 0; c < 3; ++c)
+            if (b[c] <= eps) { n = m.EN[m.FE[bf][c]]; break; }
+    }
+    const double s = dot(p - bc, n) >= 0.0 ? 1.0 : -1.0;
+    return s * std::sqrt(best);
+}
+
+// BodyBVHObj seen through the collision / force routines above: value = signed
+// distance, gradient = pseudonormal, Hessian = 0 (BodyBVHObj.cpp:453-566 with
+// _use_sdf_gradient false)
+struct MeshBody {
+    const Mesh* mesh;
+    M4 E;
+    V6 phi;
+    double col_th;
+    M3 R() const { return E.block<3, 3>(0, 0); }
+    V3 p() const { return E.block<3, 1>(0, 3); }
+    double value(const V3& x) const { V3 n; int64_t f; return mesh_query(*mesh, x, n, f); }
+    V3 dSDF(const V3& x) const { V3 n; int64_t f; mesh_query(*mesh, x, n, f); return n; }
+    M3 ddSDF(const V3&) const { return M3::zero(); }
+    bool in_box(const V3& x) const {
+        for (int a = 0; a < 3; ++a)
+            if (x[a] < mesh->bmin[a] || x[a] > mesh->bmax[a]) return false;
+        return true;
+    }
+};
+
+// collision_detection_general_BVH (CollisionDetection.cpp:150-185) with
+// distance_normal_parallel's box filter (BodyBVHObj.cpp:322-356)
+void detect_bvh(const M4& E1, const MeshBody& B, const double* pts, int64_t n, std::vector<Contact>& contacts,
+                uint8_t* active) {
+    M3 R1 = E1.block<3, 3>(0, 0);
+    V3 p1 = E1.block<3, 1>(0, 3);
+    for (int64_t i = 0; i < n; ++i) {
+        V3 xi = v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+        V3 x = B.R().T() * ((R1 * xi + p1) - B.p());
+        bool hit = false;
+        if (B.in_box(x)) hit = B.value(x) + B.col_th < 0.;
+        if (active) active[i] = hit ? 1 : 0;
+        if (hit) contacts.push_back(Contact{xi, (int)i});
     }
 }
 
@@ -802,6 +963,81 @@ int oracle_force_fixed_contacts(const oracle_grid* g, const double* xi, int64_t 
 int oracle_se3_exp(const double* q, double* E) {
     M4 m = se3_exp(load_V6(q));
     std::memcpy(E, m.a, sizeof(double) * 16);
+    return 0;
+}
+
+// ---- BVH contact variant (brute-force signed distance; test infrastructure only)
+static Mesh make_mesh(const double* V, int64_t nv, const int64_t* F, int64_t nf) {
+    Mesh m;
+    for (int64_t i = 0; i < nv; ++i) m.V.push_back(v3(V[3 * i], V[3 * i + 1], V[3 * i + 2]));
+    for (int64_t f = 0; f < nf; ++f) m.F.push_back({F[3 * f], F[3 * f + 1], F[3 * f + 2]});
+    mesh_prepare(m);
+    return m;
+}
+
+// face normals (nf x 3), edge-opposite-corner indices (nf x 3), edge normals
+// (ne x 3, ne returned) and vertex normals (nv x 3); any output may be NULL
+int oracle_mesh_normals(const double* V, int64_t nv, const int64_t* F, int64_t nf, double* FN, int64_t* FE,
+                        double* EN, double* VN, int64_t* ne) {
+    Mesh m = make_mesh(V, nv, F, nf);
+    for (int64_t f = 0; f < nf; ++f)
+        for (int a = 0; a < 3; ++a) {
+            if (FN) FN[3 * f + a] = m.FN[f][a];
+            if (FE) FE[3 * f + a] = m.FE[f][a];
+        }
+    if (EN)
+        for (size_t e = 0; e < m.EN.size(); ++e)
+            for (int a = 0; a < 3; ++a) EN[3 * e + a] = m.EN[e][a];
+    if (VN)
+        for (int64_t v = 0; v < nv; ++v)
+            for (int a = 0; a < 3; ++a) VN[3 * v + a] = m.VN[v][a];
+    if (ne) *ne = (int64_t)m.EN.size();
+    return 0;
+}
+
+// BodyBVHObj::distance_normal_parallel for n world points (d = 10, normal 0 and
+// face -1 outside the vertices' bounding box)
+int oracle_mesh_distance_normal(const double* V, int64_t nv, const int64_t* F, int64_t nf, const double* E2,
+                                double col_th, const double* xw, int64_t n, double* d, double* normal, int64_t* face) {
+    Mesh m = make_mesh(V, nv, F, nf);
+    MeshBody B{&m, load_M4(E2), V6::zero(), col_th};
+    for (int64_t i = 0; i < n; ++i) {
+        const V3 x = B.R().T() * (v3(xw[3 * i], xw[3 * i + 1], xw[3 * i + 2]) - B.p());
+        V3 nn = V3::zero();
+        int64_t f = -1;
+        double dd = 10.0;
+        if (B.in_box(x)) dd = mesh_query(m, x, nn, f) + col_th;
+        d[i] = dd;
+        if (normal)
+            for (int a = 0; a < 3; ++a) normal[3 * i + a] = nn[a];
+        if (face) face[i] = f;
+    }
+    return 0;
+}
+
+// One ForceGeneralBVHContact evaluation (detection, then the force over the active
+// set); flags as oracle_eval_instance. in->grid and in->mode are not used.
+int oracle_bvh_eval(const double* V, int64_t nv, const int64_t* F, int64_t nf, const oracle_instance* in, unsigned flags,
+                    double* fm, double* Km, double* Dm, double* dfm_dp6, uint8_t* active, int64_t* n_active) {
+    Mesh m = make_mesh(V, nv, F, nf);
+    MeshBody B{&m, load_M4(in->E2), load_V6(in->phi2), in->col_th};
+    M4 E1 = load_M4(in->E1);
+    V6 phi1 = load_V6(in->phi1);
+    std::vector<Contact> contacts;
+    detect_bvh(E1, B, in->points, in->n_points, contacts, active);
+    ForceParams P{in->kn, in->kt, in->mu, in->damping, in->scale, in->contact_scale};
+    V12 f;
+    if (flags & 1u) {
+        M12 K, D; V12 p6;
+        compute_force_deriv(contacts, E1, phi1, B, P, f, K, D, (flags & 2u) ? &p6 : nullptr);
+        if (Km) std::memcpy(Km, K.a, sizeof(double) * 144);
+        if (Dm) std::memcpy(Dm, D.a, sizeof(double) * 144);
+        if ((flags & 2u) && dfm_dp6) std::memcpy(dfm_dp6, p6.a, sizeof(double) * 12);
+    } else {
+        compute_force(contacts, E1, phi1, B, P, f);
+    }
+    std::memcpy(fm, f.a, sizeof(double) * 12);
+    if (n_active) *n_active = (int64_t)contacts.size();
     return 0;
 }
 

@@ -89,6 +89,10 @@ def lib():
         L.oracle_make_level_set3.argtypes = [vp, C.c_int64, vp, C.c_int64, fp, C.c_float, C.c_float, C.c_float,
                                              C.c_int, C.c_int, C.c_int, C.c_int, vp, vp]
         L.oracle_sdf_mesh_grid.argtypes = [dp, C.c_int64, C.c_double, C.c_int, fp, fp, dp, C.POINTER(C.c_int)]
+        L.oracle_mesh_normals.argtypes = [vp, C.c_int64, vp, C.c_int64, vp, vp, vp, vp, C.POINTER(C.c_int64)]
+        L.oracle_mesh_distance_normal.argtypes = [vp, C.c_int64, vp, C.c_int64, dp, C.c_double, vp, C.c_int64, vp, vp, vp]
+        L.oracle_bvh_eval.argtypes = [vp, C.c_int64, vp, C.c_int64, C.POINTER(OracleInstance), C.c_uint, dp, dp, dp, dp,
+                                      C.POINTER(C.c_uint8), C.POINTER(C.c_int64)]
         _lib = L
     return _lib
 
@@ -286,3 +290,56 @@ def sdf_mesh_grid(V, dx, min_size):
     if lib().oracle_sdf_mesh_grid(_dp(v), len(v), float(dx), int(min_size), mn, mx, _dp(d), n):
         raise ValueError("oracle_sdf_mesh_grid: bad arguments")
     return np.array(mn[:], dtype=np.float32), np.array(mx[:], dtype=np.float32), d, tuple(int(a) for a in n)
+
+
+# ---------------------------------------------------------------- BVH contact variant
+def _mesh_args(V, F):
+    v = np.ascontiguousarray(V, dtype=np.float64).reshape(-1, 3)
+    f = np.ascontiguousarray(F, dtype=np.int64).reshape(-1, 3)
+    return v, f
+
+
+def mesh_normals(V, F):
+    """(face normals nf x 3, edge index of the edge opposite each corner nf x 3,
+    edge normals ne x 3, angle-weighted vertex normals nv x 3)."""
+    v, f = _mesh_args(V, F)
+    FN = np.zeros((len(f), 3))
+    FE = np.zeros((len(f), 3), dtype=np.int64)
+    VN = np.zeros((len(v), 3))
+    ne = C.c_int64(0)
+    EN = np.zeros((3 * len(f), 3))
+    lib().oracle_mesh_normals(v.ctypes.data_as(C.c_void_p), len(v), f.ctypes.data_as(C.c_void_p), len(f),
+                              FN.ctypes.data_as(C.c_void_p), FE.ctypes.data_as(C.c_void_p),
+                              EN.ctypes.data_as(C.c_void_p), VN.ctypes.data_as(C.c_void_p), C.byref(ne))
+    return FN, FE, EN[:ne.value], VN
+
+
+def mesh_distance_normal(V, F, xw, E2=None, col_th=0.0):
+    """BodyBVHObj::distance_normal_parallel by brute force: (d, normal, face)."""
+    v, f = _mesh_args(V, F)
+    x = np.ascontiguousarray(xw, dtype=np.float64).reshape(-1, 3)
+    E = _f64(np.eye(4) if E2 is None else E2).reshape(4, 4)
+    n = len(x)
+    d = np.zeros(n)
+    nrm = np.zeros((n, 3))
+    face = np.zeros(n, dtype=np.int64)
+    lib().oracle_mesh_distance_normal(v.ctypes.data_as(C.c_void_p), len(v), f.ctypes.data_as(C.c_void_p), len(f), _dp(E),
+                                      float(col_th), x.ctypes.data_as(C.c_void_p), n, d.ctypes.data_as(C.c_void_p),
+                                      nrm.ctypes.data_as(C.c_void_p), face.ctypes.data_as(C.c_void_p))
+    return d, nrm, face
+
+
+def bvh_eval(V, F, inst: OracleInstance, flags=0):
+    """One ForceGeneralBVHContact evaluation (flags: bit0 Km/Dm, bit1 p6 column)."""
+    v, f = _mesh_args(V, F)
+    fm = np.zeros(12)
+    Km = np.zeros(144)
+    Dm = np.zeros(144)
+    p6 = np.zeros(12)
+    act = np.zeros(inst.n_points, dtype=np.uint8)
+    na = C.c_int64(0)
+    lib().oracle_bvh_eval(v.ctypes.data_as(C.c_void_p), len(v), f.ctypes.data_as(C.c_void_p), len(f), C.byref(inst), flags,
+                          _dp(fm), _dp(Km), _dp(Dm), _dp(p6), act.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(na))
+    return {"fm": fm, "Km": Km.reshape(12, 12), "Dm": Dm.reshape(12, 12), "dfm_dp6": p6, "active": act.astype(bool),
+            "n_active": int(na.value)}
+
