@@ -1,13 +1,16 @@
-// pybind11 module atac._redmax: the C++ SDF-contact classes (Body, BodySDFObj,
-// ForceGeneralSDFContact) for Python, as the reference's redmax_py exposes its
+// pybind11 module atac._redmax: the C++ contact classes (Body, BodySDFObj,
+// ForceGeneralSDFContact, BodyBVHObj, ForceGeneralBVHContact, BatchedBDF1) for
+// Python, as the reference's redmax_py exposes its
 // simulator (simulation/redmax/python_interface.cpp:34-303). Arrays are numpy FP64.
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
 #include "Body/Body.h"
+#include "Body/BodyBVHObj.h"
 #include "Body/BodySDFObj.h"
 #include "Simulation/BatchedBDF1.h"
+#include "Force/ForceGeneralBVHContact.h"
 #include "Force/ForceGeneralSDFContact.h"
 
 namespace py = pybind11;
@@ -82,7 +85,7 @@ sdfgen::Array3<dtype> grid_of(const arr& values) {
 }  // namespace
 
 PYBIND11_MODULE(_redmax, m) {
-    m.doc() = "GPU-backed SDF contact force with the reference simulator's C++ interface";
+    m.doc() = "GPU-backed SDF and BVH contact forces with the reference simulator's C++ interface";
 
     py::class_<Body>(m, "Body")
         .def(py::init([](std::string name, const arr& pts, int index0) {
@@ -221,6 +224,70 @@ PYBIND11_MODULE(_redmax, m) {
         .def("time_report_us", [](ForceGeneralSDFContact& f) {
             return py::make_tuple(f._time.get("collision detection"), f._time.get("compute force"));
         });
+
+    py::class_<BodyBVHObj, Body>(m, "BodyBVHObj")
+        .def(py::init([](std::string name, const arr& V, const py::array_t<int64_t, py::array::c_style | py::array::forcecast>& F,
+                         double col_th, const arr& pts, int index0) {
+                 if (F.ndim() != 2 || F.shape(1) != 3) throw std::invalid_argument("triangles must have shape (m, 3)");
+                 std::vector<std::array<int, 3>> fs((size_t)F.shape(0));
+                 auto r = F.unchecked<2>();
+                 for (py::ssize_t i = 0; i < F.shape(0); ++i) fs[(size_t)i] = {(int)r(i, 0), (int)r(i, 1), (int)r(i, 2)};
+                 return new BodyBVHObj(nullptr, name, points_of(V), fs, col_th, points_of(pts), index0);
+             }),
+             py::arg("name"), py::arg("V"), py::arg("F"), py::arg("col_th"), py::arg("contact_points"),
+             py::arg("index0") = 0)
+        .def_readwrite("col_th", &BodyBVHObj::_col_th)
+        .def("get_AABB", [](const BodyBVHObj& b) {
+            auto bb = b.get_AABB();
+            return py::make_tuple(fixed_out(bb.first), fixed_out(bb.second));
+        })
+        .def("distance", [](BodyBVHObj& b, const arr& xw) { dtype d; b.distance(vec3(xw), d); return d; })
+        .def("distance_normal", [](BodyBVHObj& b, const arr& xw) {
+            dtype d;
+            Vector3 g;
+            b.distance_normal(vec3(xw), d, g);
+            return py::make_tuple(d, fixed_out(g));
+        })
+        .def("distance_normal_parallel", [](BodyBVHObj& b, const arr& xw) {
+            std::vector<dtype> d;
+            std::vector<Vector3> g;
+            b.distance_normal_parallel(points_of(xw), d, g);
+            py::array_t<double> gd({(py::ssize_t)g.size(), (py::ssize_t)3});
+            auto w = gd.mutable_unchecked<2>();
+            for (size_t i = 0; i < g.size(); ++i)
+                for (int k = 0; k < 3; ++k) w(i, k) = g[i](k);
+            return py::make_tuple(py::array_t<double>(d.size(), d.data()), gd);
+        });
+
+    py::class_<ForceGeneralBVHContact>(m, "ForceGeneralBVHContact")
+        .def(py::init([](Body* a, Body* b, double kn, double kt, double mu, double damping) {
+                 return new ForceGeneralBVHContact(nullptr, a, b, kn, kt, mu, damping);
+             }),
+             py::arg("contact_body"), py::arg("BVH_body"), py::arg("kn") = 1.0, py::arg("kt") = 0.0,
+             py::arg("mu") = 0.0, py::arg("damping") = 0.0, py::keep_alive<1, 2>(), py::keep_alive<1, 3>())
+        .def("set_stiffness", &ForceGeneralBVHContact::set_stiffness)
+        .def("set_friction", &ForceGeneralBVHContact::set_friction)
+        .def("set_damping", &ForceGeneralBVHContact::set_damping)
+        .def("set_scale", &ForceGeneralBVHContact::set_scale)
+        .def("computeForce", [](ForceGeneralBVHContact& f, int ndof_m) {
+            VectorX fm = VectorX::Zero(ndof_m), fr;
+            f.computeForce(fm, fr);
+            return vec_out(fm);
+        }, py::arg("ndof_m"))
+        .def("computeForceWithDerivative", [](ForceGeneralBVHContact& f, int ndof_m) {
+            VectorX fm = VectorX::Zero(ndof_m), fr;
+            MatrixX Km = MatrixX::Zero(ndof_m, ndof_m), Dm = MatrixX::Zero(ndof_m, ndof_m), Kr, Dr;
+            f.computeForceWithDerivative(fm, fr, Km, Dm, Kr, Dr);
+            return py::make_tuple(vec_out(fm), mat_out(Km), mat_out(Dm));
+        }, py::arg("ndof_m"))
+        .def("computeForceWithDesignDerivative", [](ForceGeneralBVHContact& f, int ndof_m, int ndof_p) {
+            VectorX fm = VectorX::Zero(ndof_m), fr;
+            MatrixX Km = MatrixX::Zero(ndof_m, ndof_m), Dm = MatrixX::Zero(ndof_m, ndof_m), Kr, Dr;
+            MatrixX dfm_dp = MatrixX::Zero(ndof_m, ndof_p), dfr_dp;
+            f.computeForceWithDerivative(fm, fr, Km, Dm, Kr, Dr, dfm_dp, dfr_dp);
+            return py::make_tuple(vec_out(fm), mat_out(Km), mat_out(Dm), mat_out(dfm_dp));
+        }, py::arg("ndof_m"), py::arg("ndof_p"))
+        .def("last_active_count", &ForceGeneralBVHContact::last_active_count);
 
     py::class_<BatchedBDF1>(m, "BatchedBDF1",
                             "BDF1 steps of parts on translational joints against fixed parts, many candidates at once")

@@ -105,3 +105,35 @@ def test_no_contact():
     got = m.force(xi, E1, np.zeros(6), np.eye(4), np.zeros(6), 0.0, 1.0, 1.0, 0.5, 0.1, deriv=True, p6=True)
     assert got["n_active"] == 0
     assert np.all(got["fm"] == 0) and np.all(got["Km"] == 0) and np.all(got["Dm"] == 0)
+
+
+def test_host_classes_scatter():
+    """BodyBVHObj / ForceGeneralBVHContact (pybind): the global fm, Km, Dm and the
+    contact-scale column hold the per-object result at each body's _index."""
+    from atac import _redmax as R
+    V, F, xi, E1, phi1, E2, phi2 = _scene("torus", 5)
+    args = dict(col_th=0.004, kn=2e4, kt=5e2, mu=0.4, damping=3.0)
+    a = R.Body("ball", xi, 6)
+    b = R.BodyBVHObj("torus", V, F, args["col_th"], np.zeros((0, 3)), 18)
+    a.E_0i, a.phi, b.E_0i, b.phi = E1, phi1, E2, phi2
+    a.contact_scale = 0.8
+    a.p6_column = 2
+    f = R.ForceGeneralBVHContact(a, b, args["kn"], args["kt"], args["mu"], args["damping"])
+    fm, Km, Dm, dp = f.computeForceWithDesignDerivative(30, 4)
+    inst = O.make_instance(0, xi, E1, phi1, E2, phi2, args["col_th"], args["kn"], args["kt"], args["mu"],
+                           args["damping"], 1.0, 0.8)
+    ref = O.bvh_eval(V, F, inst, flags=3)
+    assert f.last_active_count() == ref["n_active"] > 0
+    idx = np.r_[6:12, 18:24]
+    s = np.abs(ref["fm"]).max()
+    np.testing.assert_allclose(fm[idx], ref["fm"], atol=1e-9 * s)
+    np.testing.assert_allclose(Km[np.ix_(idx, idx)], ref["Km"], atol=1e-9 * np.abs(ref["Km"]).max())
+    np.testing.assert_allclose(Dm[np.ix_(idx, idx)], ref["Dm"], atol=1e-9 * np.abs(ref["Dm"]).max())
+    np.testing.assert_allclose(dp[idx, 2], ref["dfm_dp6"], atol=1e-9 * np.abs(ref["dfm_dp6"]).max())
+    rest = np.setdiff1d(np.arange(30), idx)
+    assert np.all(fm[rest] == 0)
+    # distance queries through the body
+    xb = np.random.default_rng(1).uniform(V.min(0), V.max(0), size=(50, 3))
+    d, n = b.distance_normal_parallel(xb @ E2[:3, :3].T + E2[:3, 3])
+    od, on, of = O.mesh_distance_normal(V, F, xb @ E2[:3, :3].T + E2[:3, 3], E2=E2, col_th=args["col_th"])
+    np.testing.assert_allclose(d, od, atol=1e-12)
