@@ -1185,23 +1185,27 @@ int atac_bvh_force_host(atac_mesh m, const atac_instance* inst, const double* xi
     a.M = m->dev;
     a.n = n;
     a.flags = flags;
-    double *dx = nullptr, *dout = nullptr;
-    int64_t* dna = nullptr;
+    double *dx = nullptr, *dout = nullptr, *dpart = nullptr;
+    int64_t *dna = nullptr, *dcnt = nullptr;
+    const int64_t nb = bvh_force_blocks(n);
+    if (nb >= ((int64_t)1 << 31)) return fail(ATAC_ERR_INVALID, "atac_bvh_force_host: too many points");
     int rc = dalloc(&dx, (size_t)std::max<int64_t>(3 * n, 1));
     if (!rc) rc = dalloc(&dout, (size_t)kBvhOut);
     if (!rc) rc = dalloc(&dna, 1);
+    if (!rc) rc = dalloc(&dpart, (size_t)(nb * kBvhOut));
+    if (!rc) rc = dalloc(&dcnt, (size_t)nb);
     hipError_t e = hipSuccess;
     std::vector<double> h(kBvhOut);
     int64_t na = 0;
     if (!rc && n > 0) e = hipMemcpy(dx, xi, sizeof(double) * 3 * n, hipMemcpyHostToDevice);
     if (!rc && e == hipSuccess) {
-        a.xi = dx; a.out = dout; a.n_active = dna;
+        a.xi = dx; a.out = dout; a.n_active = dna; a.part = dpart; a.part_cnt = dcnt;
         e = launch_bvh_force(a, nullptr);
     }
     if (!rc && e == hipSuccess) e = hipMemcpy(h.data(), dout, sizeof(double) * kBvhOut, hipMemcpyDeviceToHost);
     if (!rc && e == hipSuccess) e = hipMemcpy(&na, dna, sizeof(int64_t), hipMemcpyDeviceToHost);
     if (!rc && e != hipSuccess) rc = fail(ATAC_ERR_HIP, hipGetErrorString(e));
-    dfree(dx); dfree(dout); dfree(dna);
+    dfree(dx); dfree(dout); dfree(dna); dfree(dpart); dfree(dcnt);
     if (rc) return rc;
     for (int k = 0; k < 12; ++k) fm[k] = h[k];
     if (flags & ATAC_EVAL_P6)

@@ -185,12 +185,13 @@ __global__ __launch_bounds__(256) void mesh_distance_kernel(MeshQueryArgs a) {
     if (a.face) a.face[i] = f;
 }
 
-// One force object in one workgroup of 256 (4 wavefronts): wave w takes points
-// w*64 + lane, + 256, ...; a point inside the mesh's box with d = signed distance
-// + col_th < 0 is a contact (collision_detection_general_BVH,
-// CollisionDetection.cpp:150-185) evaluated with the FP64 contact math. Lane sums
-// in FP64, then column sums per wave through LDS and the four waves added in
-// order: the same result on every run.
+// One force object over workgroups of 256 (4 wavefronts), kBvhBlockPoints points
+// each: wave w of workgroup b takes points b*kBvhBlockPoints + w*64 + lane, + 256,
+// ...; a point inside the mesh's box with d = signed distance + col_th < 0 is a
+// contact (collision_detection_general_BVH, CollisionDetection.cpp:150-185)
+// evaluated with the FP64 contact math. Lane sums in FP64, column sums per wave
+// through LDS, the four waves added in order into the workgroup's partial row;
+// bvh_reduce_kernel adds the rows in workgroup order: the same result on every run.
 __global__ __launch_bounds__(256) void bvh_force_kernel(BvhForceArgs a) {
     constexpr int NF = 24;
     __shared__ double s_red[4][64 * NF];
@@ -205,11 +206,13 @@ __global__ __launch_bounds__(256) void bvh_force_kernel(BvhForceArgs a) {
     for (int k = 0; k < NF; ++k) acc[k] = 0.0;
     for (int k = lane; k < 288; k += 64) s_km[w][k] = 0.0;
     int cnt = 0;
-    for (int64_t base = (int64_t)w * 64; base < a.n; base += 256) {
+    const int64_t b0 = (int64_t)blockIdx.x * kBvhBlockPoints;
+    const int64_t b1 = b0 + kBvhBlockPoints < a.n ? b0 + kBvhBlockPoints : a.n;
+    for (int64_t base = b0 + (int64_t)w * 64; base < b1; base += 256) {
         const int64_t i = base + lane;
         ContactFwd<double> s;
         bool have = false;
-        if (i < a.n) {
+        if (i < b1) {
             double x[3];
             body2_point(a.in, a.xi[3 * i], a.xi[3 * i + 1], a.xi[3 * i + 2], x);
             const D3 xb = d3(x[0], x[1], x[2]);
@@ -271,13 +274,29 @@ __global__ __launch_bounds__(256) void bvh_force_kernel(BvhForceArgs a) {
     }
     if (lane == 0) s_cnt[w] = wc;
     __syncthreads();
+    double* row = a.part + (int64_t)blockIdx.x * kBvhOut;
     for (int k = threadIdx.x; k < kBvhOut; k += 256) {
         double v;
         if (k < NF) v = ((s_row[0][k] + s_row[1][k]) + s_row[2][k]) + s_row[3][k];
         else v = ((s_km[0][k - NF] + s_km[1][k - NF]) + s_km[2][k - NF]) + s_km[3][k - NF];
+        row[k] = v;
+    }
+    if (threadIdx.x == 0) a.part_cnt[blockIdx.x] = (int64_t)s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+}
+
+// The workgroups' partial rows added in workgroup order.
+__global__ __launch_bounds__(320) void bvh_reduce_kernel(BvhForceArgs a, int nb) {
+    const int k = threadIdx.x;
+    if (k < kBvhOut) {
+        double v = 0.0;
+        for (int b = 0; b < nb; ++b) v += a.part[(int64_t)b * kBvhOut + k];
         a.out[k] = v;
     }
-    if (threadIdx.x == 0 && a.n_active) *a.n_active = (int64_t)s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+    if (k == 0 && a.n_active) {
+        int64_t c = 0;
+        for (int b = 0; b < nb; ++b) c += a.part_cnt[b];
+        *a.n_active = c;
+    }
 }
 
 hipError_t launch_mesh_distance(const MeshQueryArgs& a, hipStream_t st) {
@@ -287,7 +306,11 @@ hipError_t launch_mesh_distance(const MeshQueryArgs& a, hipStream_t st) {
 }
 
 hipError_t launch_bvh_force(const BvhForceArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(bvh_force_kernel, dim3(1), dim3(256), 0, st, a);
+    const int nb = (int)bvh_force_blocks(a.n);
+    hipLaunchKernelGGL(bvh_force_kernel, dim3(nb), dim3(256), 0, st, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(bvh_reduce_kernel, dim3(1), dim3(320), 0, st, a, nb);
     return hipGetLastError();
 }
 

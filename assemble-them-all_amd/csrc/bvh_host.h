@@ -42,8 +42,12 @@ struct BvhForceArgs {
     uint32_t flags;       // ATAC_EVAL_* (DERIV, P6)
     double* out;          // 12 fm, 12 p6, 144 Km, 144 Dm
     int64_t* n_active;
+    double* part;         // per workgroup: kBvhOut partial sums
+    int64_t* part_cnt;    // per workgroup: contacts
 };
 constexpr int kBvhOut = 12 + 12 + 288;
+constexpr int kBvhBlockPoints = 2048;   // points per force workgroup
+inline int64_t bvh_force_blocks(int64_t n) { return n > 0 ? (n + kBvhBlockPoints - 1) / kBvhBlockPoints : 1; }
 
 hipError_t launch_mesh_distance(const MeshQueryArgs& a, hipStream_t st);
 hipError_t launch_bvh_force(const BvhForceArgs& a, hipStream_t st);

@@ -186,7 +186,7 @@ def bvh_measurement():
     """BVH contact variant on a 4,096-triangle torus: exact signed distance +
     pseudonormal queries per second (100k points around the mesh, host arrays in and
     out, so the PCIe copies are included) and one force evaluation with Km / Dm and
-    the p6 column for a 10,242-point ball pressed into the surface (one workgroup)."""
+    the p6 column for a 10,242-point ball pressed into the surface."""
     import torch
     from atac import _lib as L
     from atac.bvh import BVHMesh
