@@ -444,6 +444,8 @@ static int plan(atac_batch b, const atac_tuning* t) {
             td.count = (int32_t)std::min<int64_t>(T, d.pt_count - o);
             td.begin = o;
             td.slot = slot + o;
+            td.grid = d.grid;
+            td.pad = 0;
             b->tiles.push_back(td);
             chunks += (td.count + kChunk - 1) / kChunk;
         }

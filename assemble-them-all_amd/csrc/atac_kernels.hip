@@ -165,7 +165,7 @@ __global__ __launch_bounds__(BLOCK) void detect_kernel(DetectArgs a) {
         const int tile = order[kk];
         const TileDev t = as_const(a.tiles)[tile];
         const InstDev in = as_const(a.inst)[t.inst];
-        const GridDev G = as_const(a.grids)[in.grid];
+        const GridDev G = as_const(a.grids)[t.grid];
         const auto px = as_global(in.px) + t.begin;
         const auto py = as_global(in.py) + t.begin;
         const auto pz = as_global(in.pz) + t.begin;

@@ -74,6 +74,8 @@ struct TileDev {
     int32_t count;      // points in the tile
     int64_t begin;      // first point, index into the instance's arrays
     int64_t slot;       // first slot of the tile in the batch-wide active list
+    int32_t grid;       // the instance's grid (read with the instance, not after it)
+    int32_t pad;
 };
 
 // Active-contact record, written by the detection pass for every point with

@@ -34,7 +34,7 @@ METRIC = "contact-point SDF queries/s (force+adjoint) at 1 GPU; % HBM roofline; 
 # Algorithmic bytes of one detection launch (see roofline in main):
 DETECT_INST_BYTES = 72 + 24 + 24 + 8 + 24 + 8 + 4   # instance fields detection reads: A, b, cmax, col_th,
                                                     # point pointers, point count, grid index
-TILE_BYTES = 24 + 4 + 4 + 4                         # TileDev + queue entry + tile_active + tile_chunk0 writes
+TILE_BYTES = 32 + 4 + 4 + 4                         # TileDev + queue entry + tile_active + tile_chunk0 writes
 ACT_BYTES = 4                                       # active-list entry (point offset) per active point
 CHUNK_BYTES = 16                                    # work-item entry, one per kChunk active points
 RECORD_BYTES = 48                                   # FP32 active-contact record
