@@ -149,6 +149,11 @@ class ContactBatch:
                                       C.byref(tu) if tu is not None else None, C.byref(h)))
         self.handle = h
 
+    @property
+    def instances(self):
+        """The current Instance records (a ctypes array; set_instances accepts it back)."""
+        return self._inst
+
     @staticmethod
     def _tuning(tuning):
         if tuning is None:
@@ -190,7 +195,8 @@ class ContactBatch:
         check(lib().atac_batch_retune(self.handle, C.byref(tu)))
 
     def set_instances(self, instances):
-        arr = (L.Instance * self.n)(*instances)
+        """Instances as a sequence of Instance records (or a ctypes array of them)."""
+        arr = instances if isinstance(instances, C.Array) else (L.Instance * self.n)(*instances)
         self._inst = arr
         check(lib().atac_batch_set_instances(self.handle, arr, self.n, _stream()))
 

@@ -291,6 +291,24 @@ def main():
     stage_ms = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in st.items()}
     log(f"stage ms: {stage_ms}")
 
+    # end to end from host memory: the instances' poses uploaded (atac_batch_set_instances
+    # composes the detection maps on the host), evaluation and adjoint with d(loss)/d(wrench)
+    # copied in, the wrench and the gradients copied back, every step
+    inst_arr = b.instances
+    g_host = g.cpu()
+    e2e_steps = max(1, args.steps // 2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(e2e_steps):
+        b.set_instances(inst_arr)
+        b.eval(checkpoint=True, out=out)
+        g.copy_(g_host)
+        b.vjp(g, out=grads)
+        host_out = [out["fm"].cpu()] + [v.cpu() for v in grads.values()]
+    torch.cuda.synchronize()
+    e2e_ms = (time.perf_counter() - t0) * 1e3 / e2e_steps
+    log(f"end to end from host memory: {e2e_ms:.3f} ms per step")
+
     touched = b.touched_values()
     # detection: every point read once (12 B), every grid value its stencils touch
     # once (4 B), the instance fields and tile entries read, and the active list
@@ -363,6 +381,9 @@ def main():
         "err_vs_fp64": err,
         "plain_path": plain,
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "end_to_end_host": {"ms_per_step": round(e2e_ms, 4), "queries_per_s": n_q / e2e_ms * 1e3,
+                            "what": "instances uploaded from host memory (detection maps composed on the host), "
+                                    "evaluation + adjoint, wrench and gradients copied back, eager launches"},
         "step_effective_GBps": round(step_bytes / (ms * 1e-3) / 1e9, 1),
         "sdf_build": sdf_build,
         "bdf1_translational": bdf1,
