@@ -25,7 +25,7 @@ namespace atac {
 // leaf (w0 < 0): triangles tri[-w0 - 1 .. -w0 - 1 + w1) of the leaf order.
 struct MeshDev {
     const float4* nodes;    // 2 per node: (lo, w0) (hi, w1), w as int bits
-    const int32_t* tri;     // leaf order -> face index
+    const int32_t* tri;     // leaf order -> face index f, stored as ~f for a degenerate face
     const double* V;        // nv x 3
     const int32_t* F;       // nf x 3
     const double* FN;       // nf x 3 face normals (zero for a degenerate face)
@@ -49,11 +49,37 @@ ATAC_HD D3 scl(D3 a, double s) { return d3(a.x * s, a.y * s, a.z * s); }
 ATAC_HD double dt(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 ATAC_HD D3 row3(const double* p, int64_t i) { return d3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
 
+// Closest point of segment (u, v) to p (u itself for a zero-length segment).
+ATAC_HD D3 closest_on_segment(D3 p, D3 u, D3 v) {
+    const D3 uv = sub(v, u);
+    const double l2 = dt(uv, uv);
+    const double t = l2 > 0.0 ? fmin(fmax(dt(sub(p, u), uv) / l2, 0.0), 1.0) : 0.0;
+    return add(u, scl(uv, t));
+}
+
 // Closest point of triangle (a, b, c) to p and its barycentric coordinates
 // (Ericson, Real-Time Collision Detection, 5.1.5): the Voronoi region of p is
 // found from the same dot products; vertex regions give exact (1,0,0)-type
-// coordinates and edge regions an exact zero.
-ATAC_HD D3 closest_on_triangle(D3 p, D3 a, D3 b, D3 c, double bary[3]) {
+// coordinates and edge regions an exact zero. A degenerate triangle (zero area,
+// flagged when the mesh is prepared) is taken as its three edges, the first
+// nearest in the order ab, bc, ca: the region tests would compare rounding
+// noise there. Its coordinates are returned as (1/3, 1/3, 1/3), which selects
+// its face normal, zero.
+ATAC_HD D3 closest_on_triangle(D3 p, D3 a, D3 b, D3 c, double bary[3], bool degenerate) {
+    if (degenerate) {
+        bary[0] = bary[1] = bary[2] = 1.0 / 3.0;
+        D3 best = closest_on_segment(p, a, b);
+        D3 q = sub(p, best);
+        double bd = dt(q, q);
+        const D3 c1 = closest_on_segment(p, b, c);
+        q = sub(p, c1);
+        const double d1 = dt(q, q);
+        if (d1 < bd) { bd = d1; best = c1; }
+        const D3 c2 = closest_on_segment(p, c, a);
+        q = sub(p, c2);
+        if (dt(q, q) < bd) best = c2;
+        return best;
+    }
     const D3 ab = sub(b, a), ac = sub(c, a), ap = sub(p, a);
     const double d1 = dt(ab, ap), d2 = dt(ac, ap);
     if (d1 <= 0.0 && d2 <= 0.0) { bary[0] = 1.0; bary[1] = 0.0; bary[2] = 0.0; return a; }
@@ -137,10 +163,11 @@ ATAC_HD double mesh_signed_distance(const MeshDev& M, D3 p, D3& normal, int& fac
             if (w0 < 0) {   // leaf
                 const int first = -w0 - 1, cnt = node_w(hi.w);
                 for (int k = 0; k < cnt; ++k) {
-                    const int f = M.tri[first + k];
+                    const int tf = M.tri[first + k];
+                    const int f = tf < 0 ? ~tf : tf;
                     double b[3];
                     const D3 c = closest_on_triangle(p, row3(M.V, M.F[3 * f]), row3(M.V, M.F[3 * f + 1]),
-                                                     row3(M.V, M.F[3 * f + 2]), b);
+                                                     row3(M.V, M.F[3 * f + 2]), b, tf < 0);
                     const D3 pc = sub(p, c);
                     const double d2 = dt(pc, pc);
                     if (d2 < best || (d2 == best && f < bf)) {

@@ -110,11 +110,15 @@ std::string mesh_build(const double* V, int64_t nv, const int64_t* F, int64_t nf
             m.bmin[a] = std::min(m.bmin[a], V[3 * v + a]);
             m.bmax[a] = std::max(m.bmax[a], V[3 * v + a]);
         }
-    // face normals: (v1 - v0) x (v2 - v0), normalised; zero for a degenerate face
+    // face normals: (v1 - v0) x (v2 - v0), normalised; a face is degenerate when
+    // |ab x ac|^2 <= 1e-20 |ab|^2 |ac|^2 (zero area to rounding), and its normal is zero
     m.FN.assign(3 * nf, 0.0);
+    std::vector<char> degenerate(nf, 0);
     for (int64_t f = 0; f < nf; ++f) {
         const V3d a = vrow(m.V, m.F[3 * f]), b = vrow(m.V, m.F[3 * f + 1]), c = vrow(m.V, m.F[3 * f + 2]);
-        add_row(m.FN, f, vcross(vsub(b, a), vsub(c, a)), 1.0);
+        const V3d ab = vsub(b, a), ac = vsub(c, a), n = vcross(ab, ac);
+        degenerate[f] = vdot(n, n) <= 1e-20 * vdot(ab, ab) * vdot(ac, ac);
+        if (!degenerate[f]) add_row(m.FN, f, n, 1.0);
     }
     normalize_rows(m.FN);
     // edges: the edge opposite corner c of face f is (F[f][c+1], F[f][c+2]); its
@@ -158,6 +162,8 @@ std::string mesh_build(const double* V, int64_t nv, const int64_t* F, int64_t nf
     B.build(0, 0, (int32_t)nf, 0);
     m.nodes = std::move(B.nodes);
     m.tri = std::move(B.idx);
+    for (int32_t& f : m.tri)
+        if (degenerate[f]) f = ~f;
     return "";
 }
 

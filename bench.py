@@ -291,6 +291,17 @@ def main():
     stage_ms = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in st.items()}
     log(f"stage ms: {stage_ms}")
 
+    # the same step with FP64 contact arithmetic and FP64 records (ATAC_EVAL_PRECISE), eager launches
+    b.timing(True)
+    for _ in range(args.steps):
+        b.eval(checkpoint=True, precise=True, out=out)
+        b.vjp(g, out=grads)
+    st64 = b.timing(False)
+    torch.cuda.synchronize()
+    stage64 = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in st64.items()}
+    ms64 = sum(stage64.values())
+    log(f"FP64 contact arithmetic: stage ms {stage64}")
+
     # end to end from host memory: the instances' poses uploaded (atac_batch_set_instances
     # composes the detection maps on the host), evaluation and adjoint with d(loss)/d(wrench)
     # copied in, the wrench and the gradients copied back, every step
@@ -381,6 +392,9 @@ def main():
         "err_vs_fp64": err,
         "plain_path": plain,
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "fp64_contact": {"kernel_ms_per_step": round(ms64, 4), "queries_per_s": n_q / ms64 * 1e3,
+                         "stage_ms": {k: round(v, 4) for k, v in stage64.items()},
+                         "what": "detection unchanged; records, wrench and adjoint in FP64 (sum of the stage times)"},
         "end_to_end_host": {"ms_per_step": round(e2e_ms, 4), "queries_per_s": n_q / e2e_ms * 1e3,
                             "what": "instances uploaded from host memory (detection maps composed on the host), "
                                     "evaluation + adjoint, wrench and gradients copied back, eager launches"},

@@ -664,6 +664,7 @@ struct Mesh {
     std::vector<std::array<int64_t, 3>> F;
     std::vector<V3> FN, EN, VN;
     std::vector<std::array<int64_t, 3>> FE;   // edge opposite corner c
+    std::vector<char> degenerate;              // zero-area faces
     V3 bmin, bmax;
 };
 
@@ -677,10 +678,17 @@ void mesh_prepare(Mesh& m) {
     for (int a = 0; a < 3; ++a) { m.bmin[a] = INFINITY; m.bmax[a] = -INFINITY; }
     for (const V3& v : m.V)
         for (int a = 0; a < 3; ++a) { m.bmin[a] = std::min(m.bmin[a], v[a]); m.bmax[a] = std::max(m.bmax[a], v[a]); }
+    // a face is degenerate when |ab x ac|^2 <= 1e-20 |ab|^2 |ac|^2 (zero area to
+    // rounding); its normal is zero. libigl leaves this case to rounding (its
+    // region tests and barycentric recomputation divide zero by zero), so the
+    // rule here is a definition, shared with the GPU path.
     m.FN.assign(nf, V3::zero());
+    m.degenerate.assign(nf, 0);
     for (size_t f = 0; f < nf; ++f) {
         const V3 &a = m.V[m.F[f][0]], &b = m.V[m.F[f][1]], &c = m.V[m.F[f][2]];
-        m.FN[f] = normalized(cross(b - a, c - a));
+        const V3 ab = b - a, ac = c - a, n = cross(ab, ac);
+        m.degenerate[f] = dot(n, n) <= 1e-20 * dot(ab, ab) * dot(ac, ac);
+        if (!m.degenerate[f]) m.FN[f] = normalized(n);
     }
     std::map<std::pair<int64_t, int64_t>, int64_t> eid;
     m.FE.assign(nf, {0, 0, 0});
@@ -712,7 +720,25 @@ void mesh_prepare(Mesh& m) {
 }
 
 // closest point of triangle (a, b, c) to p by its Voronoi regions (Ericson 5.1.5)
-static V3 closest_point_triangle(const V3& p, const V3& a, const V3& b, const V3& c) {
+static V3 closest_point_segment(const V3& p, const V3& u, const V3& v) {
+    const V3 uv = v - u;
+    const double l2 = dot(uv, uv);
+    const double t = l2 > 0.0 ? std::min(std::max(dot(p - u, uv) / l2, 0.0), 1.0) : 0.0;
+    return u + t * uv;
+}
+
+// Ericson's regions; a degenerate triangle is taken as its three edges (the first
+// nearest of ab, bc, ca)
+static V3 closest_point_triangle(const V3& p, const V3& a, const V3& b, const V3& c, bool degenerate) {
+    if (degenerate) {
+        V3 best = closest_point_segment(p, a, b);
+        double bd = dot(p - best, p - best);
+        const V3 c1 = closest_point_segment(p, b, c);
+        if (dot(p - c1, p - c1) < bd) { bd = dot(p - c1, p - c1); best = c1; }
+        const V3 c2 = closest_point_segment(p, c, a);
+        if (dot(p - c2, p - c2) < bd) best = c2;
+        return best;
+    }
     const V3 ab = b - a, ac = c - a, ap = p - a;
     const double d1 = dot(ab, ap), d2 = dot(ac, ap);
     if (d1 <= 0.0 && d2 <= 0.0) return a;
@@ -739,7 +765,7 @@ double mesh_query(const Mesh& m, const V3& p, V3& n, int64_t& face) {
     int64_t bf = -1;
     V3 bc = p;
     for (size_t f = 0; f < m.F.size(); ++f) {
-        const V3 c = closest_point_triangle(p, m.V[m.F[f][0]], m.V[m.F[f][1]], m.V[m.F[f][2]]);
+        const V3 c = closest_point_triangle(p, m.V[m.F[f][0]], m.V[m.F[f][1]], m.V[m.F[f][2]], m.degenerate[f]);
         const V3 pc = p - c;
         const double d2 = dot(pc, pc);
         if (d2 < best) { best = d2; bf = (int64_t)f; bc = c; }
@@ -754,7 +780,9 @@ double mesh_query(const Mesh& m, const V3& p, V3& n, int64_t& face) {
     const double eps = 1e-12;
     const int nz = (b[0] <= eps) + (b[1] <= eps) + (b[2] <= eps);
     n = m.FN[bf];
-    if (nz >= 2) {
+    if (m.degenerate[bf]) {
+        // the face normal (zero)
+    } else if (nz >= 2) {
         for (int c = 0; c < 3; ++c)
             if (b[c] > eps) { n = m.VN[m.F[bf][c]]; break; }
     } else if (nz == 1) {

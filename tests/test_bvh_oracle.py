@@ -136,3 +136,26 @@ def test_pose_and_col_th():
     ins = d_body != 10.0
     np.testing.assert_allclose(d_w[ins], d_body[ins] + 0.02, atol=1e-13)
     np.testing.assert_array_equal(f_w, f_body)
+
+
+def test_degenerate_triangle_is_its_edges():
+    """A zero-area triangle (three collinear vertices, or a repeated vertex) is
+    taken as its edges: the distance to a collinear sliver is the distance to its
+    segment, and its normals are zero, so that distance is positive."""
+    V, F = box_mesh(0.5)
+    V = np.vstack([V, [[0.6, -0.2, 0.1], [0.6, 0.0, 0.1], [0.6, 0.2, 0.1]]])
+    nv = len(V)
+    F = np.vstack([F, [[nv - 3, nv - 2, nv - 1], [nv - 3, nv - 3, nv - 1]]])
+    FN, FE, EN, VN = O.mesh_normals(V, F)
+    assert np.all(FN[12:] == 0) and np.all(VN[nv - 3:] == 0)
+    rng = np.random.default_rng(6)
+    p = rng.uniform([0.5, -0.3, 0.0], [0.6, 0.3, 0.2], size=(2000, 3))
+    d, n, f = O.mesh_distance_normal(V, F, p)
+    on = f == 12
+    assert on.sum() > 100
+    t = np.clip((p[on, 1] + 0.2) / 0.4, 0.0, 1.0)
+    q = np.stack([np.full(on.sum(), 0.6), -0.2 + 0.4 * t, np.full(on.sum(), 0.1)], axis=1)
+    np.testing.assert_allclose(d[on], np.linalg.norm(p[on] - q, axis=1), rtol=1e-13, atol=1e-15)
+    assert np.all(n[on] == 0)
+    # the others are the cube's: closed form of the box
+    np.testing.assert_allclose(d[~on], box_sdf(p[~on], 0.5), atol=1e-13)

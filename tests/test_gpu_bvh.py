@@ -137,3 +137,40 @@ def test_host_classes_scatter():
     d, n = b.distance_normal_parallel(xb @ E2[:3, :3].T + E2[:3, 3])
     od, on, of = O.mesh_distance_normal(V, F, xb @ E2[:3, :3].T + E2[:3, 3], E2=E2, col_th=args["col_th"])
     np.testing.assert_allclose(d, od, atol=1e-12)
+
+
+def test_empty_inputs():
+    """No query points: empty distance arrays; a force evaluation over no points is zero."""
+    V, F = MESHES["cube"]()
+    m = BVHMesh(V, F)
+    d, n, f = m.distance_normal(np.zeros((0, 3)))
+    assert d.shape == (0,) and n.shape == (0, 3) and f.shape == (0,)
+    got = m.force(np.zeros((0, 3)), np.eye(4), np.zeros(6), np.eye(4), np.zeros(6), 0.01, 1e4, 1e3, 0.5, 1.0,
+                  deriv=True, p6=True)
+    assert got["n_active"] == 0
+    for key in ("fm", "dfm_dp6", "Km", "Dm"):
+        assert np.all(np.asarray(got[key]) == 0), key
+
+
+def test_degenerate_triangles():
+    """A mesh holding zero-area triangles (a collinear sliver floating beside a
+    cube face, and a triangle with a repeated vertex) gives the oracle's
+    distances, normals and faces: their face, edge and vertex normals are zero,
+    so the points nearest to them get a positive sign."""
+    V, F = box_mesh(0.5)
+    V = np.vstack([V, [[0.6, -0.2, 0.1], [0.6, 0.0, 0.1], [0.6, 0.2, 0.1]]])
+    nv = len(V)
+    F = np.vstack([F, [[nv - 3, nv - 2, nv - 1], [nv - 3, nv - 3, nv - 1]]])
+    rng = np.random.default_rng(5)
+    xw = rng.uniform(-0.5, 0.5, size=(3000, 3))
+    xw[:1000] = rng.uniform([0.5, -0.3, 0.0], [0.6, 0.3, 0.2], size=(1000, 3))   # around the slivers
+    m = BVHMesh(V, F)
+    d, n, f = m.distance_normal(xw, col_th=0.003)
+    od, on, of = O.mesh_distance_normal(V, F, xw, col_th=0.003)
+    np.testing.assert_allclose(d, od, rtol=0, atol=1e-12)
+    same = f == of
+    agree = np.all(np.abs(n - on) <= 1e-12, axis=1)
+    print("faces equal", same.mean(), "normals equal", agree.mean(), "on slivers", int((of >= 12).sum()), flush=True)
+    assert (of >= 12).sum() > 20
+    assert agree[same].all()
+    assert agree.mean() > 0.999
