@@ -289,3 +289,36 @@ def test_collision_api(gpu):
         scale = max(1.0, np.abs(bref).max())
         assert np.max(np.abs(a - bref)) <= 1e-4 * scale, name
         off += cnt
+
+
+def test_nonfinite_nodes(gpu):
+    """Grid nodes holding NaN or +inf (outside the part, where the field is
+    positive) leave their blocks open: the points there are interpolated, stay
+    inactive as in the oracle, and the wrench of the active points is unchanged."""
+    import dataclasses
+    w = W.peg_in_hole(n_points=4000, n_poses=8)
+    g0 = w.grids[0]
+    vals = np.array(g0.values, dtype=np.float32, copy=True)
+    flat = vals.reshape(-1)
+    rng = np.random.default_rng(21)
+    far = np.flatnonzero(flat > 0.1)
+    pick = rng.choice(far, size=600, replace=False)
+    flat[pick[:300]] = np.nan
+    flat[pick[300:]] = np.inf
+    wp = dataclasses.replace(w, grids=[dataclasses.replace(g0, values=vals)])
+    print(f"[case] peg_in_hole 8 poses x 4000 points, {len(pick)} non-finite nodes", flush=True)
+    ref = oracle_workload(wp, flags=O.FLAG_DERIV)
+    b = batch_from_workload(wp)
+    out = _np(b.eval(deriv=True, checkpoint=True))
+    n_checked = 0
+    for i, r in enumerate(ref):
+        band = band_mask(wp, wp.instances[i])
+        ids = b.active_ids(i)
+        gpu_set = np.zeros(len(wp.point_sets[0]), dtype=bool)
+        gpu_set[ids] = True
+        assert not np.any((gpu_set != r["active"]) & ~band), i
+        if r["n_active"] and not band.any():
+            assert np.all(np.isfinite(out["fm"][i]))
+            assert rel_err(out["fm"][i], r["fm"], r["scale_fm"]) <= TOL
+            n_checked += 1
+    assert n_checked > 0
