@@ -16,8 +16,18 @@ def _same_bits(a, b):
     return np.array_equal(np.asarray(a, np.float32).view(np.uint32), np.asarray(b, np.float32).view(np.uint32))
 
 
+def _box_with_slivers():
+    """A box plus two zero-area triangles beside it: three collinear vertices, and
+    a triangle with a repeated vertex."""
+    V, F = M.box(1.0)
+    V = np.vstack([V, [[1.2, -0.4, 0.05], [1.2, 0.1, 0.05], [1.2, 0.4, 0.05]]])
+    n = len(V)
+    return V, np.vstack([F, [[n - 3, n - 2, n - 1], [n - 3, n - 3, n - 1]]])
+
+
 CASES = [
     ("box", M.box(1.0), (-1.53, -1.47, -1.51), (0.1, 0.1, 0.1), (31, 30, 32), 1),
+    ("box_slivers", _box_with_slivers(), (-1.53, -1.47, -1.51), (0.1, 0.1, 0.1), (31, 30, 32), 1),
     ("box_band0", M.box(0.8, (0.1, -0.05, 0.02)), (-1.0, -1.0, -1.0), (0.07, 0.09, 0.11), (29, 23, 19), 0),
     ("sphere", M.icosphere(1.0, 3), (-1.31, -1.29, -1.3), (0.05, 0.05, 0.05), (53, 52, 53), 1),
     ("torus_band2", M.torus(1.0, 0.35, 32, 16), (-1.5, -1.5, -0.5), (0.04, 0.045, 0.05), (76, 68, 21), 2),
