@@ -174,3 +174,26 @@ def test_degenerate_triangles():
     assert (of >= 12).sum() > 20
     assert agree[same].all()
     assert agree.mean() > 0.999
+
+
+def test_force_many_workgroups():
+    """A force evaluation over several 2,048-point workgroups (their partial rows
+    added in workgroup order) equals the oracle."""
+    V, F = MESHES["torus"]()
+    rng = np.random.default_rng(8)
+    P, _ = icosphere(0.35, levels=4)
+    xi = np.concatenate([P, P * 0.95, P * 0.9])           # 7,686 points: 4 workgroups
+    E1 = O.se3_exp(np.concatenate([rng.normal(size=3) * 0.3, np.zeros(3)]))
+    E1[:3, 3] = V[rng.integers(len(V))] * 1.02
+    E2 = np.eye(4)
+    phi1, phi2 = rng.normal(size=6) * 0.2, rng.normal(size=6) * 0.1
+    args = dict(col_th=0.005, kn=3e4, kt=1e3, mu=0.6, damping=5.0, scale=1.1, contact_scale=0.9)
+    inst = O.make_instance(0, xi, E1, phi1, E2, phi2, args["col_th"], args["kn"], args["kt"], args["mu"],
+                           args["damping"], args["scale"], args["contact_scale"])
+    ref = O.bvh_eval(V, F, inst, flags=3)
+    got = BVHMesh(V, F).force(xi, E1, phi1, E2, phi2, deriv=True, p6=True, **args)
+    print("points", len(xi), "contacts", got["n_active"], flush=True)
+    assert got["n_active"] == ref["n_active"] > 50
+    for key in ("fm", "dfm_dp6", "Km", "Dm"):
+        a, b = np.asarray(got[key]), np.asarray(ref[key])
+        assert np.abs(a - b).max() <= 1e-9 * max(np.abs(b).max(), 1e-12), key

@@ -322,3 +322,35 @@ def test_nonfinite_nodes(gpu):
             assert rel_err(out["fm"][i], r["fm"], r["scale_fm"]) <= TOL
             n_checked += 1
     assert n_checked > 0
+
+
+def test_ragged_segment_sizes(gpu):
+    """Point sets of sizes on both sides of the wavefront (64), step (256 with four
+    points per lane) and tile (2048) boundaries, each the prefix of one contact
+    point set, in one batch: active counts and wrench equal the oracle's per set."""
+    w = W.peg_in_hole(n_points=5000, n_poses=2)
+    pts = W.round_points_f32(w.point_sets[0])
+    sizes = [1, 63, 64, 65, 255, 256, 257, 2047, 2048, 2049, 4095, 4097]
+    allp = np.concatenate([pts[:n] for n in sizes])
+    seg = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    P = DevicePoints(allp, seg)
+    g = DeviceGrid.from_grid(w.grids[0])
+    I = w.instances[1]
+    print(f"[case] ragged segments {sizes}", flush=True)
+    inst = [make_instance(0, 0, s, I.E1, I.phi1, I.E2, I.phi2, I.col_th, I.kn, I.kt, I.mu, I.damping)
+            for s in range(len(sizes))]
+    b = ContactBatch([g], [P], inst, tuning={"tile_points": 2048, "points_per_lane": 4})
+    out = _np(b.eval(deriv=True))
+    G = O.GridRef.from_grid(w.grids[0])
+    n_contact = 0
+    for s, n in enumerate(sizes):
+        r = O.eval_instance([G], O.make_instance(0, pts[:n], I.E1, I.phi1, I.E2, I.phi2, I.col_th, I.kn, I.kt,
+                                                 I.mu, I.damping), O.FLAG_DERIV)
+        assert int(out["n_active"][s]) == r["n_active"], (n, int(out["n_active"][s]), r["n_active"])
+        if r["n_active"]:
+            n_contact += 1
+            assert rel_err(out["fm"][s], r["fm"], r["scale_fm"]) <= TOL, n
+            assert rel_err(out["Km"][s], r["Km"], r["scale_Km"]) <= TOL, n
+        else:
+            assert np.all(out["fm"][s] == 0.0)
+    assert n_contact >= 6
