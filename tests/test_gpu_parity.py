@@ -6,7 +6,7 @@ Tolerances (relative to the FP64 reference on identical FP32-rounded inputs):
   active set          identical outside the |d| < 1e-6 band
 The per-contact normalisation is used because peg-in-hole wrenches cancel (net
 lateral force ~ 0 while each contact carries ~ kn*d); the error against ||fm_ref||
-itself is reported by bench.py and DESIGN.md alongside.
+itself is reported by bench.py alongside.
 """
 import numpy as np
 import pytest
