@@ -714,6 +714,18 @@ ATAC_HD void sdf_cell64(const GridDev& G, const Corners& c, const double f[3], d
 }
 
 // ------------------------------------------------------------------ contact math
+// Quotient a / b of the contact math: FP64 divides; FP32 on the GPU multiplies by
+// the hardware reciprocal (v_rcp_f32, 1 ulp) instead of the ten-instruction IEEE
+// division sequence.
+ATAC_HD inline double qdiv(double a, double b) { return a / b; }
+ATAC_HD inline float qdiv(float a, float b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return a * __builtin_amdgcn_rcpf(b);
+#else
+    return a / b;
+#endif
+}
+
 template <typename R>
 struct InstR {
     m3<R> R1, R2;
@@ -899,7 +911,7 @@ ATAC_HD void record_decode(const Rec64& r, const float*, const InstDev& in, Cont
 template <typename R>
 ATAC_HD void collision_forward(const InstR<R>& I, ContactFwd<R>& s) {
     s.gn = norm(s.g);
-    s.e = s.g * ((R)1 / s.gn);
+    s.e = s.g * qdiv((R)1, s.gn);
     s.n = mv(I.R2, s.e);
     s.xi2 = s.x - s.d * s.e;
     s.a2 = mtv(I.R2, s.xw1dot);
@@ -928,7 +940,7 @@ ATAC_HD void contact_forward(const InstR<R>& I, ContactFwd<R>& s, R* p6) {
     s.f[6] = -s.lam * s.ni2a.x; s.f[7] = -s.lam * s.ni2a.y; s.f[8] = -s.lam * s.ni2a.z;
     s.f[9] = -s.lam * s.e.x;    s.f[10] = -s.lam * s.e.y;   s.f[11] = -s.lam * s.e.z;
     if (p6) {
-        const R k = I.kn * s.d / I.cs;
+        const R k = qdiv(I.kn * s.d, I.cs);
         p6[0] = -k * s.ni1a.x; p6[1] = -k * s.ni1a.y; p6[2] = -k * s.ni1a.z;
         p6[3] = -k * s.n1.x;   p6[4] = -k * s.n1.y;   p6[5] = -k * s.n1.z;
         p6[6] = k * s.ni2a.x;  p6[7] = k * s.ni2a.y;  p6[8] = k * s.ni2a.z;
@@ -952,7 +964,7 @@ ATAC_HD void contact_forward(const InstR<R>& I, ContactFwd<R>& s, R* p6) {
             a1 = I.kt;
             a2 = I.kt;
         } else {
-            s.rho = s.fcn / s.tn;
+            s.rho = qdiv(s.fcn, s.tn);
             a1 = I.mu * s.rho;
             a2 = I.mu * s.rho;
         }
@@ -961,7 +973,7 @@ ATAC_HD void contact_forward(const InstR<R>& I, ContactFwd<R>& s, R* p6) {
         s.f[6] += a2 * s.T2a.x; s.f[7] += a2 * s.T2a.y; s.f[8] += a2 * s.T2a.z;
         s.f[9] += a2 * s.t2.x;  s.f[10] += a2 * s.t2.y; s.f[11] += a2 * s.t2.z;
         if (p6) {
-            const R b1 = a1 / I.cs, b2 = a2 / I.cs;
+            const R b1 = qdiv(a1, I.cs), b2 = qdiv(a2, I.cs);
             p6[0] -= b1 * s.T1a.x; p6[1] -= b1 * s.T1a.y; p6[2] -= b1 * s.T1a.z;
             p6[3] -= b1 * s.t1.x;  p6[4] -= b1 * s.t1.y;  p6[5] -= b1 * s.t1.z;
             p6[6] += b2 * s.T2a.x; p6[7] += b2 * s.T2a.y; p6[8] += b2 * s.T2a.z;
@@ -1006,7 +1018,7 @@ ATAC_HD void collision_jvp(const InstR<R>& I, const ContactFwd<R>& s, v3<R> dxw,
     v3<R> dx = mtv(I.R2, dxw) + cross(s.x, om2) - v2;          // dx_dxw, dx_dw2 = skew(x), dx_dv2 = -I
     o.dd = dot(s.g, dx);
     v3<R> dg = symv(s.H, dx);
-    o.de = (dg - s.e * dot(s.e, dg)) * ((R)1 / s.gn);           // de_ddd_dx * H * dx
+    o.de = (dg - s.e * dot(s.e, dg)) * qdiv((R)1, s.gn);           // de_ddd_dx * H * dx
     o.dn = mv(I.R2, o.de + cross(om2, s.e));
     o.dxi2 = dx - s.e * o.dd - s.d * o.de;
     v3<R> dxdot = cross(dx, I.w2) + cross(s.x, w2p) + cross(s.a2, om2) + mtv(I.R2, dxwdot) - v2p;
@@ -1043,9 +1055,9 @@ ATAC_HD void contact_jvp(const InstR<R>& I, const ContactFwd<R>& s, const Tangen
         } else {
             // fc1 = lam [ni1a; n1]; its tangent is [a1; l1] before friction is added
             R fc_dot = s.lam * (dot(s.ni1a, a1) + dot(s.n1, l1));
-            R dfcn = fc_dot / s.fcn;
-            R dtn = dot(s.tdot, dtdot) / s.tn;
-            R drho = dfcn / s.tn - s.fcn * dtn / (s.tn * s.tn);
+            R dfcn = qdiv(fc_dot, s.fcn);
+            R dtn = qdiv(dot(s.tdot, dtdot), s.tn);
+            R drho = qdiv(dfcn, s.tn) - qdiv(s.fcn * dtn, s.tn * s.tn);
             a1 = a1 - I.mu * (drho * s.T1a + s.rho * dT1a);
             l1 = l1 - I.mu * (drho * s.t1 + s.rho * dt1);
             a2 = a2 + I.mu * (drho * s.T2a + s.rho * dT2a);
@@ -1079,12 +1091,12 @@ ATAC_HD void contact_vjp(const InstR<R>& I, const ContactFwd<R>& s, const R gb[1
             R rhob = -I.mu * (dot(f1a, s.T1a) + dot(f1l, s.t1)) + I.mu * (dot(f2a, s.T2a) + dot(f2l, s.t2));
             T1ab = -I.mu * s.rho * f1a; t1b = -I.mu * s.rho * f1l;
             T2ab = I.mu * s.rho * f2a;  t2b = I.mu * s.rho * f2l;
-            R fcnb = rhob / s.tn;
-            R tnb = -rhob * s.fcn / (s.tn * s.tn);
-            R k = fcnb * s.lam / s.fcn;            // fc1 = lam [ni1a; n1]
+            R fcnb = qdiv(rhob, s.tn);
+            R tnb = qdiv(-rhob * s.fcn, s.tn * s.tn);
+            R k = qdiv(fcnb * s.lam, s.fcn);       // fc1 = lam [ni1a; n1]
             fc1ab += k * s.ni1a;
             fc1lb += k * s.n1;
-            tdotb += (tnb / s.tn) * s.tdot;
+            tdotb += qdiv(tnb, s.tn) * s.tdot;
         }
         t1b += cross(T1ab, s.xi1);
         tdotb += mv(I.R1, t1b);
@@ -1125,7 +1137,7 @@ ATAC_HD void contact_vjp(const InstR<R>& I, const ContactFwd<R>& s, const R gb[1
     v3<R> m = mtv(I.R2, nb);
     eb += m;
     om2b += cross(s.e, m);
-    gtb += (eb - s.e * dot(s.e, eb)) * ((R)1 / s.gn);
+    gtb += (eb - s.e * dot(s.e, eb)) * qdiv((R)1, s.gn);
     xb += symv(s.H, gtb);
     xb += db * s.g;
     v3<R> xw1b = mv(I.R2, xb);
