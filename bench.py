@@ -335,10 +335,26 @@ def main():
     # write the 26 gradients per instance
     step_bytes = (det_bytes + (ACT_BYTES + 12 + RECORD_BYTES) * n_active + RECORD_BYTES * n_active
                   + 8 * (12 + 26) * n_chunks_lb + 8 * 12 * b.n * 2 + 8 * 26 * b.n)
+    # a plain device-to-device copy of 1 GiB (read + write) on the same GPU: the bandwidth a
+    # streaming kernel reaches here, reported beside the published peak
+    src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    c0.record()
+    for _ in range(10):
+        dst.copy_(src)
+    c1.record()
+    torch.cuda.synchronize()
+    copy_gbs = 2 * src.numel() * 4 / (c0.elapsed_time(c1) / 10 * 1e-3) / 1e9
+    del src, dst
+    log(f"device copy: {copy_gbs:.0f} GB/s")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
                 "kernel": "detect_kernel", "algorithmic_bytes_per_launch": int(det_bytes),
-                "touched_grid_values": touched, "launch_ms": round(det_ms, 4)}
+                "touched_grid_values": touched, "launch_ms": round(det_ms, 4),
+                "measured_copy_GBps": round(copy_gbs, 1), "frac_of_measured_copy": round(achieved / copy_gbs, 4)}
 
     plain = None
     if not args.no_plain:
