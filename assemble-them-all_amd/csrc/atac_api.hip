@@ -178,6 +178,14 @@ int atac_grid_create(const float* values, int32_t ni, int32_t nj, int32_t nk, co
     for (int a = 0; a < 3; ++a)
         if (!(dxyz[a] > 0.0) || !(max_box[a] >= min_box[a]))
             return fail(ATAC_ERR_INVALID, "atac_grid_create: spacing must be > 0 and max_box >= min_box");
+    // size limits, checked before anything is allocated or read: the detection
+    // addresses the corner-packed cells (32 B per node) with 32-bit byte offsets
+    // and forms block-table offsets with 24-bit products
+    if ((int64_t)ni * nj * nk * 32 >= ((int64_t)1 << 32))
+        return fail(ATAC_ERR_INVALID, "atac_grid_create: grids hold fewer than 2^27 nodes");
+    if (ni >= (1 << 24) || nj >= (1 << 24) || nk >= (1 << 24) ||
+        (int64_t)((nj - 1) / kBlk + 1) * ((nk - 1) / kBlk + 1) >= (1 << 24))
+        return fail(ATAC_ERR_INVALID, "atac_grid_create: n_y * n_z / 4 and each n stay below 2^24");
     auto* g = new atac_grid_s();
     g->count = (int64_t)ni * nj * nk;
     int rc = dalloc(&g->d_values, (size_t)g->count);
@@ -201,17 +209,6 @@ int atac_grid_create(const float* values, int32_t ni, int32_t nj, int32_t nk, co
         // by 32-bit byte offsets in the detection kernel
         const int64_t nc = (int64_t)ni * nj * nk;
         const int64_t nb = blk_outside(G);   // entries of the block table before the outside entry
-        if (nc * 32 >= ((int64_t)1 << 32)) {
-            dfree(g->d_values);
-            delete g;
-            return fail(ATAC_ERR_INVALID, "atac_grid_create: grids hold fewer than 2^27 nodes");
-        }
-        // the detection forms table offsets with 24-bit products
-        if (ni >= (1 << 24) || nj >= (1 << 24) || nk >= (1 << 24) || (int64_t)G.nb[1] * G.nb[2] >= (1 << 24)) {
-            dfree(g->d_values);
-            delete g;
-            return fail(ATAC_ERR_INVALID, "atac_grid_create: n_y * n_z / 4 and each n stay below 2^24");
-        }
         rc = dalloc(&g->d_cells, (size_t)(2 * nc));
         if (!rc) rc = dalloc(&g->d_blk, (size_t)(nb + 1));   // + the entry of points outside the box
         if (!rc) {
