@@ -20,6 +20,8 @@ EVAL_CHECKPOINT = 4
 EVAL_PRECISE = 8
 EVAL_ALL_ACTIVE = 16
 
+BVH_NOALLOC_MAX_POINTS = 1048576   # ATAC_BVH_NOALLOC_MAX_POINTS
+
 EXPORTED = [
     "atac_last_error", "atac_version", "atac_device_init",
     "atac_grid_create", "atac_grid_destroy",
@@ -31,7 +33,7 @@ EXPORTED = [
     "atac_batch_eval_host", "atac_batch_vjp_host", "atac_sdf_distance_host", "atac_sdf_collision_host",
     "atac_sdf_from_mesh", "atac_sdf_mesh_grid",
     "atac_mesh_create", "atac_mesh_destroy", "atac_mesh_distance_normal", "atac_mesh_distance_normal_host",
-    "atac_bvh_force_host",
+    "atac_bvh_force_host", "atac_bvh_force_noalloc",
 ]
 
 
@@ -117,6 +119,7 @@ def lib():
     L.atac_mesh_distance_normal.argtypes = [vp, dp, C.c_double, vp, i64, vp, vp, vp, vp]
     L.atac_mesh_distance_normal_host.argtypes = [vp, dp, C.c_double, vp, i64, vp, vp, vp]
     L.atac_bvh_force_host.argtypes = [vp, C.POINTER(Instance), vp, i64, u32, vp, vp, vp, vp, C.POINTER(i64)]
+    L.atac_bvh_force_noalloc.argtypes = [vp, C.POINTER(Instance), vp, i64, u32, vp, vp, vp]
     for name in EXPORTED:
         if name not in ("atac_last_error",):
             getattr(L, name).restype = C.c_int

@@ -162,6 +162,7 @@ int atac_device_init(int32_t device, int32_t* n_cu) {
     ATAC_HIP(hipGetDeviceCount(&count));
     if (device < 0 || device >= count) return fail(ATAC_ERR_INVALID, "atac_device_init: no such device");
     ATAC_HIP(hipSetDevice(device));
+    ATAC_HIP(bvh_scratch_setup(device));   // the device's fixed scratch (atac_bvh_force_noalloc)
     if (n_cu) {
         hipDeviceProp_t prop;
         ATAC_HIP(hipGetDeviceProperties(&prop, device));
@@ -1212,5 +1213,37 @@ int atac_bvh_force_host(atac_mesh m, const atac_instance* inst, const double* xi
     if (flags & ATAC_EVAL_DERIV)
         for (int k = 0; k < 144; ++k) { Km[k] = h[24 + k]; Dm[k] = h[168 + k]; }
     if (n_active) *n_active = na;
+    return ATAC_OK;
+}
+
+int atac_bvh_force_noalloc(atac_mesh m, const atac_instance* inst, const double* xi, int64_t n, uint32_t flags,
+                           double* out, int64_t* n_active, void* stream) {
+    if (n > kBvhNoAllocMaxPoints)
+        return fail(ATAC_ERR_INVALID, "atac_bvh_force_noalloc: " + std::to_string(n) + " points; at most " +
+                                          std::to_string(kBvhNoAllocMaxPoints) + " fit its fixed scratch");
+    if (!m || !inst || !out || n < 0 || (n > 0 && !xi))
+        return fail(ATAC_ERR_INVALID, "atac_bvh_force_noalloc: bad arguments");
+    if (flags & ~(ATAC_EVAL_DERIV | ATAC_EVAL_P6 | ATAC_EVAL_PRECISE))
+        return fail(ATAC_ERR_INVALID, "atac_bvh_force_noalloc: unsupported flag");
+    int dev = 0;
+    ATAC_HIP(hipGetDevice(&dev));
+    ATAC_HIP(bvh_scratch_setup(dev));   // no-op after the device's first call (or atac_device_init)
+    double* part = static_cast<double*>(bvh_scratch_address(dev));
+    if (!part) return fail(ATAC_ERR_HIP, "atac_bvh_force_noalloc: no scratch address for the device");
+    GridDev unit{};
+    for (int a = 0; a < 3; ++a) { unit.d[a] = 1.0; unit.inv_d[a] = 1.0; }
+    BvhForceArgs a{};
+    compose_instance(*inst, unit, nullptr, nullptr, nullptr, n, a.in);
+    a.M = m->dev;
+    a.n = n;
+    a.flags = flags;
+    a.xi = xi;
+    a.out = out;
+    a.n_active = n_active;
+    // every workgroup writes its whole row and count, and the reduction reads
+    // only the rows of this launch: nothing needs clearing between calls
+    a.part = part;
+    a.part_cnt = reinterpret_cast<int64_t*>(part + kBvhScratchBlocks * kBvhOut);
+    ATAC_HIP(launch_bvh_force(a, (hipStream_t)stream));
     return ATAC_OK;
 }

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include <utility>
 #include "atac.h"
@@ -309,6 +310,43 @@ hipError_t launch_mesh_distance(const MeshQueryArgs& a, hipStream_t st) {
     if (a.n <= 0) return hipSuccess;
     hipLaunchKernelGGL(mesh_distance_kernel, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, st, a);
     return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ fixed scratch
+// The partial rows and contact counts of the force launches of
+// atac_bvh_force_noalloc: kBvhScratchBlocks workgroups' rows (kBvhOut doubles
+// each), then their counts (one 8-byte word each). One copy per device, held
+// until the process ends. Host code never passes `scratch` itself to a kernel
+// (on the host it names a stand-in, not device memory): it uses the address
+// hipGetSymbolAddress gave for the device, kept in g_scratch_addr.
+static __device__ __align__(256) double scratch[kBvhScratchBlocks * (kBvhOut + 1)];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kScratchDevices] = {};
+}  // namespace
+
+hipError_t bvh_scratch_setup(int device) {
+    if (device < 0 || device >= kScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_addr[device]) return hipSuccess;
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device) return hipErrorInvalidDevice;   // the lookup needs the device current
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_addr[device] = p;
+    return hipSuccess;
+}
+
+void* bvh_scratch_address(int device) {
+    if (device < 0 || device >= kScratchDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_addr[device];
 }
 
 hipError_t launch_bvh_force(const BvhForceArgs& a, hipStream_t st) {

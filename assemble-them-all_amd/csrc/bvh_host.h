@@ -52,4 +52,14 @@ inline int64_t bvh_force_blocks(int64_t n) { return n > 0 ? (n + kBvhBlockPoints
 hipError_t launch_mesh_distance(const MeshQueryArgs& a, hipStream_t st);
 hipError_t launch_bvh_force(const BvhForceArgs& a, hipStream_t st);
 
+// Fixed scratch of atac_bvh_force_noalloc: room for the partial rows and counts of
+// kBvhScratchBlocks force workgroups (kBvhNoAllocMaxPoints points), per device.
+constexpr int64_t kBvhNoAllocMaxPoints = (int64_t)1 << 20;
+constexpr int64_t kBvhScratchBlocks = kBvhNoAllocMaxPoints / kBvhBlockPoints;
+// Looks up the device's copy of the scratch array; `device` must be current.
+// Idempotent per device.
+hipError_t bvh_scratch_setup(int device);
+// The device address from bvh_scratch_setup, or nullptr before it.
+void* bvh_scratch_address(int device);
+
 }  // namespace atac

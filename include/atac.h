@@ -246,6 +246,22 @@ int atac_mesh_distance_normal_host(atac_mesh m, const double E2[16], double col_
 int atac_bvh_force_host(atac_mesh m, const atac_instance* inst, const double* xi, int64_t n, uint32_t flags,
                         double* fm, double* Km, double* Dm, double* dfm_dp6, int64_t* n_active);
 
+/* No counterpart in the reference. atac_bvh_force_host for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): it calls no allocating routine. The partial
+ * rows of its force workgroups live in a fixed scratch array of each device
+ * (1.3 MB, held until the process ends), so n is at most
+ * ATAC_BVH_NOALLOC_MAX_POINTS; larger n is refused with ATAC_ERR_INVALID.
+ * Every call on a device shares that device's array: calls that overlap on one
+ * GPU (other streams, other threads) race and must be serialised by the caller.
+ * xi: device, 3*n FP64 (contact body frame); out: device, 312 doubles
+ * (fm 12 | dfm_dp6 12 | Km 144 | Dm 144, row-major; the blocks flags do not ask
+ * for hold zeros); n_active: device, 1 (may be NULL). inst is a host pointer.
+ * Asynchronous on stream, same results bit for bit as atac_bvh_force_host. */
+#define ATAC_BVH_NOALLOC_MAX_POINTS 1048576
+int atac_bvh_force_noalloc(atac_mesh m, const atac_instance* inst, const double* xi, int64_t n, uint32_t flags,
+                           double* out, int64_t* n_active, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -61,3 +61,15 @@ def test_package_imports_without_loading_library():
     import atac
     importlib.reload(atac)
     assert atac.EVAL_DERIV == 1 and atac.EVAL_CHECKPOINT == 4
+
+
+def test_noalloc_bvh_force_refuses_sizes_beyond_its_scratch():
+    """atac_bvh_force_noalloc checks the point count against its fixed scratch
+    before anything else (no device call is made)."""
+    from atac import _lib
+    L = _lib.lib()
+    rc = L.atac_bvh_force_noalloc(None, None, None, _lib.BVH_NOALLOC_MAX_POINTS + 1, 0, None, None, None)
+    assert rc == _lib.ATAC_ERR_INVALID
+    assert b"fixed scratch" in L.atac_last_error()
+    rc = L.atac_bvh_force_noalloc(None, None, None, 16, 0, None, None, None)
+    assert rc == _lib.ATAC_ERR_INVALID and b"bad arguments" in L.atac_last_error()
