@@ -950,15 +950,17 @@ int oracle_eval_instance(const oracle_grid* grids, const oracle_instance* in, un
 }
 
 // Many independent force objects, OpenMP over instances (threads <= 0: runtime default).
+// scale (may be null): 3 per instance, as oracle_eval_instance's.
 int oracle_eval_batch(const oracle_grid* grids, const oracle_instance* inst, int64_t n_inst, unsigned flags, int threads,
-                      double* fm, double* Km, double* Dm, double* dfm_dp6, int64_t* n_active) {
+                      double* fm, double* Km, double* Dm, double* dfm_dp6, int64_t* n_active, double* scale) {
 #ifdef _OPENMP
     if (threads > 0) omp_set_num_threads(threads);
 #pragma omp parallel for schedule(dynamic, 1)
 #endif
     for (int64_t i = 0; i < n_inst; ++i) {
         oracle_eval_instance(grids, &inst[i], flags, fm + 12 * i, Km ? Km + 144 * i : nullptr, Dm ? Dm + 144 * i : nullptr,
-                             dfm_dp6 ? dfm_dp6 + 12 * i : nullptr, nullptr, n_active ? n_active + i : nullptr, nullptr);
+                             dfm_dp6 ? dfm_dp6 + 12 * i : nullptr, nullptr, n_active ? n_active + i : nullptr,
+                             scale ? scale + 3 * i : nullptr);
     }
     return 0;
 }

@@ -79,7 +79,7 @@ def lib():
         L.oracle_eval_instance.argtypes = [C.POINTER(OracleGrid), C.POINTER(OracleInstance), C.c_uint, dp, dp, dp, dp,
                                            C.POINTER(C.c_uint8), C.POINTER(C.c_int64), dp]
         L.oracle_eval_batch.argtypes = [C.POINTER(OracleGrid), C.POINTER(OracleInstance), C.c_int64, C.c_uint, C.c_int,
-                                        dp, dp, dp, dp, C.POINTER(C.c_int64)]
+                                        dp, dp, dp, dp, C.POINTER(C.c_int64), dp]
         L.oracle_force_fixed_contacts.argtypes = [C.POINTER(OracleGrid), dp, C.c_int64, dp, dp, dp, dp, C.c_double, C.c_int,
                                                   C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                                   C.c_uint, dp, dp, dp, dp]
@@ -218,10 +218,12 @@ def eval_batch(grids, instances, flags=0, threads=0):
     Dm = np.zeros((n, 144)) if flags & FLAG_DERIV else None
     p6 = np.zeros((n, 12)) if flags & FLAG_P6 else None
     na = np.zeros(n, dtype=np.int64)
+    scale = np.zeros((n, 3))
     lib().oracle_eval_batch(garr, iarr, n, flags, threads, _dp(fm), _dp(Km) if Km is not None else None,
                             _dp(Dm) if Dm is not None else None, _dp(p6) if p6 is not None else None,
-                            na.ctypes.data_as(C.POINTER(C.c_int64)))
-    out = {"fm": fm, "n_active": na}
+                            na.ctypes.data_as(C.POINTER(C.c_int64)), _dp(scale))
+    # scale_*: per instance, the sum over its contacts of ||fm_c||_2, ||Km_c||_F, ||Dm_c||_F
+    out = {"fm": fm, "n_active": na, "scale_fm": scale[:, 0], "scale_Km": scale[:, 1], "scale_Dm": scale[:, 2]}
     if Km is not None:
         out["Km"] = Km.reshape(n, 12, 12)
         out["Dm"] = Dm.reshape(n, 12, 12)
