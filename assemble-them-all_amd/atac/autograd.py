@@ -92,7 +92,7 @@ class ContactWrench(torch.autograd.Function):
     def backward(ctx, g_fm):
         batch = ctx.batch
         if batch.generation != ctx.generation:
-            raise RuntimeError("contact_wrench: the batch was evaluated again after this forward pass; "
+            raise RuntimeError("contact_wrench: the batch was evaluated again or given new instances after this forward pass; "
                                "its adjoint records are gone")
         E1, E2 = ctx.saved_tensors
         g = batch.vjp(g_fm.contiguous())

@@ -166,6 +166,7 @@ class ContactBatch:
         t.blocks_per_cu = int(tuning.get("blocks_per_cu", 0))
         t.order = int(tuning.get("order", 0))
         t.points_per_lane = int(tuning.get("points_per_lane", 0))
+        t.max_active = int(tuning.get("max_active", 0))
         return t
 
     def close(self):
@@ -187,7 +188,7 @@ class ContactBatch:
         check(lib().atac_batch_info(self.handle, C.byref(n), C.byref(nt), C.byref(pb), C.byref(tu)))
         return {"n_points": int(n.value), "n_tiles": int(nt.value), "persistent_blocks": int(pb.value),
                 "tile_points": tu.tile_points, "detect_block": tu.detect_block, "blocks_per_cu": tu.blocks_per_cu,
-                "order": tu.order, "points_per_lane": tu.points_per_lane}
+                "order": tu.order, "points_per_lane": tu.points_per_lane, "max_active": tu.max_active}
 
     def retune(self, tuning):
         self.generation += 1
@@ -198,6 +199,7 @@ class ContactBatch:
         """Instances as a sequence of Instance records (or a ctypes array of them)."""
         arr = instances if isinstance(instances, C.Array) else (L.Instance * self.n)(*instances)
         self._inst = arr
+        self.generation += 1   # the adjoint records of the last evaluation belong to the old instances
         check(lib().atac_batch_set_instances(self.handle, arr, self.n, _stream()))
 
     def alloc_outputs(self, deriv=False, p6=False, device="cuda"):
@@ -241,6 +243,17 @@ class ContactBatch:
         o = self._outputs_struct(out)
         check(lib().atac_eval_plain(self.handle, self.flags(deriv, p6, False, precise), C.byref(o), _stream()))
         return out
+
+    def eval_plain_vjp(self, g_fm, precise=False, out=None, grads=None):
+        """The plain per-instance path with the outputs of eval() + vjp(g_fm)."""
+        g_fm = g_fm.to(dtype=torch.float64).contiguous()
+        out = out if out is not None else self.alloc_outputs()
+        grads = grads if grads is not None else self.alloc_gradients(g_fm.device)
+        o = self._outputs_struct(out)
+        gs = self._grad_struct(grads)
+        check(lib().atac_eval_plain_vjp(self.handle, self.flags(precise=precise), _ptr(g_fm), C.byref(o), C.byref(gs),
+                                        _stream()))
+        return out, grads
 
     def alloc_gradients(self, device="cuda"):
         f64 = dict(dtype=torch.float64, device=device)
@@ -290,6 +303,12 @@ class ContactBatch:
         check(lib().atac_batch_timing(self.handle, 1 if enable else 0, ms.ctypes.data_as(C.c_void_p),
                                       n.ctypes.data_as(C.c_void_p)))
         return {s: (float(ms[k]), int(n[k])) for k, s in enumerate(self.STAGES)}
+
+    def workspace(self):
+        """The record pool: {capacity, used (contacts of the last evaluation), overflowed}. Synchronous."""
+        cap, used, ovf = C.c_int64(), C.c_int64(), C.c_int32()
+        check(lib().atac_batch_workspace(self.handle, C.byref(cap), C.byref(used), C.byref(ovf)))
+        return {"capacity": int(cap.value), "used": int(used.value), "overflowed": bool(ovf.value)}
 
     def touched_values(self):
         c = C.c_int64()

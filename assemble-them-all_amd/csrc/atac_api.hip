@@ -54,7 +54,7 @@ struct atac_grid_s {
     GridDev dev;
     float* d_values = nullptr;
     float4* d_cells = nullptr;
-    float2* d_blk = nullptr;
+
     int64_t count = 0;
 };
 
@@ -70,6 +70,13 @@ struct atac_points_s {
 };
 
 struct atac_batch_s {
+    // the cell classification of one grid for one col_th (as the detection rounds it)
+    struct ClsTable {
+        int32_t grid;
+        float cth;
+        uint8_t* d;
+    };
+    std::vector<ClsTable> cls;
     int device = 0;
     int n_cu = 0;
     size_t lds_limit = 0;
@@ -86,18 +93,20 @@ struct atac_batch_s {
     int32_t* d_tq_begin = nullptr;    // nq + 1
     int32_t nq = 1;
     int32_t n_tiles = 0;
-    int64_t n_slots = 0;
-    int64_t chunk_cap = 0;            // work items if every point were active
+    int64_t n_slots = 0;              // points of the batch (queries per evaluation)
+    int64_t rec_cap = 0;              // record pool: the bound on active contacts per evaluation
+    int64_t slow_cap = 0;             // slow list entries
+    int64_t chunk_cap = 0;            // work items if the pool were full
     int32_t* d_tile_active = nullptr;
     int32_t* d_tile_chunk0 = nullptr;
     ChunkDev* d_chunks = nullptr;
     int32_t* d_counters = nullptr;    // CNT_WORDS
     // per-evaluation buffers, allocated on first use
-    uint32_t* d_act = nullptr;        // n_slots: active point offsets, tile slices
-    Rec32* d_rec32 = nullptr;         // n_slots
-    Rec64* d_rec64 = nullptr;         // n_slots
-    SlowDev* d_slow = nullptr;        // n_slots
-    float* d_slow_diag = nullptr;     // 4 x n_slots
+    uint32_t* d_act = nullptr;        // rec_cap: active point offsets, a tile's in one range
+    Rec32* d_rec32 = nullptr;         // rec_cap
+    Rec64* d_rec64 = nullptr;         // rec_cap
+    SlowDev* d_slow = nullptr;        // slow_cap
+    float* d_slow_diag = nullptr;     // 4 x slow_cap
     double* d_part = nullptr;         // chunk_cap x part_width
     int64_t part_width = 0;
     int grid_detect = 0, grid_contact[2][3] = {{0, 0, 0}, {0, 0, 0}}, grid_vjp[2] = {0, 0};
@@ -106,6 +115,8 @@ struct atac_batch_s {
     int wave_span = 0;
     bool rec_valid = false;           // records of the last evaluation are in place
     bool rec_precise = false;
+    bool rec_captured = false;        // the captured graph's evaluation writes records when it is launched
+    bool cap_precise = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     hipStream_t cap_stream = nullptr;
@@ -181,12 +192,11 @@ int atac_grid_create(const float* values, int32_t ni, int32_t nj, int32_t nk, co
             return fail(ATAC_ERR_INVALID, "atac_grid_create: spacing must be > 0 and max_box >= min_box");
     // size limits, checked before anything is allocated or read: the detection
     // addresses the corner-packed cells (32 B per node) with 32-bit byte offsets
-    // and forms block-table offsets with 24-bit products
+    // and forms the classification's brick index with 24-bit products
     if ((int64_t)ni * nj * nk * 32 >= ((int64_t)1 << 32))
         return fail(ATAC_ERR_INVALID, "atac_grid_create: grids hold fewer than 2^27 nodes");
-    if (ni >= (1 << 24) || nj >= (1 << 24) || nk >= (1 << 24) ||
-        (int64_t)((nj - 1) / kBlk + 1) * ((nk - 1) / kBlk + 1) >= (1 << 24))
-        return fail(ATAC_ERR_INVALID, "atac_grid_create: n_y * n_z / 4 and each n stay below 2^24");
+    if ((int64_t)((nj + 5) / 4) * ((nk + 5) / 4) >= (1 << 24) || ni >= (1 << 24) || nj >= (1 << 24) || nk >= (1 << 24))
+        return fail(ATAC_ERR_INVALID, "atac_grid_create: each n and (n_y + 2)(n_z + 2) / 16 stay below 2^24");
     auto* g = new atac_grid_s();
     g->count = (int64_t)ni * nj * nk;
     int rc = dalloc(&g->d_values, (size_t)g->count);
@@ -196,30 +206,26 @@ int atac_grid_create(const float* values, int32_t ni, int32_t nj, int32_t nk, co
     GridDev& G = g->dev;
     G.v = g->d_values;
     G.n[0] = ni; G.n[1] = nj; G.n[2] = nk;
-    G.fast = 0; G.cells = nullptr; G.blk = nullptr; G.pad = 0;
+    G.fast = 0; G.cells = nullptr;
     for (int a = 0; a < 3; ++a) {
         G.minb[a] = min_box[a];
         G.maxb[a] = max_box[a];
         G.d[a] = dxyz[a];
         G.inv_d[a] = 1.0 / dxyz[a];
-        G.nb[a] = (G.n[a] - 1) / kBlk + 1;
     }
     set_fd_reach(G);
     {
         // corner-packed cells, one per node (the last layer clamped), addressed
         // by 32-bit byte offsets in the detection kernel
         const int64_t nc = (int64_t)ni * nj * nk;
-        const int64_t nb = blk_outside(G);   // entries of the block table before the outside entry
         rc = dalloc(&g->d_cells, (size_t)(2 * nc));
-        if (!rc) rc = dalloc(&g->d_blk, (size_t)(nb + 1));   // + the entry of points outside the box
         if (!rc) {
-            e = launch_grid_setup(G, g->d_cells, g->d_blk, nullptr);
+            e = launch_grid_setup(G, g->d_cells, nullptr);
             if (e == hipSuccess) e = hipDeviceSynchronize();
             if (e != hipSuccess) rc = fail(ATAC_ERR_HIP, hipGetErrorString(e));
         }
-        if (rc) { dfree(g->d_values); dfree(g->d_cells); dfree(g->d_blk); delete g; return rc; }
+        if (rc) { dfree(g->d_values); dfree(g->d_cells); delete g; return rc; }
         G.cells = g->d_cells;
-        G.blk = g->d_blk;
         G.fast = 1;
     }
     *out = g;
@@ -228,7 +234,7 @@ int atac_grid_create(const float* values, int32_t ni, int32_t nj, int32_t nk, co
 
 int atac_grid_destroy(atac_grid g) {
     if (!g) return ATAC_OK;
-    dfree(g->d_values); dfree(g->d_cells); dfree(g->d_blk);
+    dfree(g->d_values); dfree(g->d_cells);
     delete g;
     return ATAC_OK;
 }
@@ -343,6 +349,40 @@ static void compose_for(atac_batch b, const atac_instance& s, InstDev& d) {
                      P.seg[s.segment + 1] - P.seg[s.segment], d);
 }
 
+// A classification table for every (grid, col_th) the instances use, built on
+// first use (synchronously) and set in their device records; tables no longer
+// used are freed once the device is idle.
+static int ensure_cls(atac_batch b) {
+    std::vector<char> used(b->cls.size(), 0);
+    for (size_t i = 0; i < b->inst_dev.size(); ++i) {
+        InstDev& d = b->inst_dev[i];
+        const float cth = (float)d.col_th;
+        size_t t = 0;
+        while (t < b->cls.size() && !(b->cls[t].grid == d.grid && __builtin_memcmp(&b->cls[t].cth, &cth, sizeof cth) == 0)) ++t;
+        if (t == b->cls.size()) {
+            const GridDev& G = b->grids[d.grid]->dev;
+            atac_batch_s::ClsTable ct{d.grid, cth, nullptr};
+            int rc = dalloc(&ct.d, (size_t)cls_bytes(G));
+            if (rc) return rc;
+            hipError_t e = launch_cls_setup(G, d.col_th, ct.d, nullptr);
+            if (e == hipSuccess) e = hipDeviceSynchronize();
+            if (e != hipSuccess) { dfree(ct.d); return fail(ATAC_ERR_HIP, hipGetErrorString(e)); }
+            b->cls.push_back(ct);
+            used.push_back(0);
+        }
+        used[t] = 1;
+        d.cls = b->cls[t].d;
+    }
+    bool idle = false;
+    for (size_t t = b->cls.size(); t-- > 0;) {
+        if (used[t]) continue;
+        if (!idle) { ATAC_HIP(hipDeviceSynchronize()); idle = true; }   // an earlier evaluation may still read it
+        dfree(b->cls[t].d);
+        b->cls.erase(b->cls.begin() + t);
+    }
+    return ATAC_OK;
+}
+
 static int check_instances(atac_batch b, const atac_instance* inst, int32_t n) {
     for (int32_t i = 0; i < n; ++i) {
         const atac_instance& s = inst[i];
@@ -364,6 +404,7 @@ static void release_plan(atac_batch b) {
     dfree(b->d_part);
     b->part_width = 0;
     b->rec_valid = false;
+    b->rec_captured = false;
     if (b->graph_exec) { (void)hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
     if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
 }
@@ -379,13 +420,15 @@ static int64_t part_width_of(int mode) {
 static int ensure_buffers(atac_batch b, int mode, bool precise) {
     int rc = ATAC_OK;
     if (b->n_slots > 0) {
-        if (!b->d_act) rc = dalloc(&b->d_act, (size_t)b->n_slots);
-        if (!rc && precise && !b->d_rec64) rc = dalloc(&b->d_rec64, (size_t)b->n_slots);
-        if (!rc && !precise && !b->d_rec32) rc = dalloc(&b->d_rec32, (size_t)b->n_slots);
-        if (!rc && !b->d_slow) rc = dalloc(&b->d_slow, (size_t)b->n_slots);
-        if (!rc && !precise && !b->d_slow_diag) rc = dalloc(&b->d_slow_diag, (size_t)b->n_slots * 4);
+        if (!b->d_act) rc = dalloc(&b->d_act, (size_t)b->rec_cap);
+        if (!rc && precise && !b->d_rec64) rc = dalloc(&b->d_rec64, (size_t)b->rec_cap);
+        if (!rc && !precise && !b->d_rec32) rc = dalloc(&b->d_rec32, (size_t)b->rec_cap);
+        if (!rc && !b->d_slow) rc = dalloc(&b->d_slow, (size_t)b->slow_cap);
+        if (!rc && !precise && !b->d_slow_diag) rc = dalloc(&b->d_slow_diag, (size_t)b->slow_cap * 4);
     }
-    const int64_t w = part_width_of(mode);
+    // wide enough for the adjoint too, so a VJP after a captured evaluation does
+    // not reallocate the rows (which would drop the captured graph)
+    const int64_t w = std::max<int64_t>(part_width_of(mode), part_width_of(3));
     if (!rc && b->chunk_cap > 0 && w > b->part_width) {
         // a captured graph holds the old buffer: it is dropped with it
         if (b->graph_exec) { (void)hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
@@ -451,7 +494,16 @@ static int plan(atac_batch b, const atac_tuning* t) {
         slot += d.pt_count;
     }
     b->n_slots = slot;
-    b->chunk_cap = chunks;
+    // the record pool bounds the active contacts of one evaluation: a record per
+    // point up to 4M points, a quarter of the points beyond (max_active sets it:
+    // > 0 that many, < 0 one per point); the slow list holds 1/16 of it
+    {
+        const int64_t dflt = std::min<int64_t>(slot, std::max<int64_t>((int64_t)1 << 22, slot / 4));
+        b->rec_cap = tu.max_active > 0 ? std::min<int64_t>(slot, tu.max_active) : (tu.max_active < 0 ? slot : dflt);
+        b->slow_cap = std::min<int64_t>(b->rec_cap, std::max<int64_t>(4096, b->rec_cap / 16));
+    }
+    // work items: at most one partial item per tile beyond the full ones of the pool
+    b->chunk_cap = std::min<int64_t>(chunks, b->rec_cap / kChunk + b->tiles.size());
     b->n_tiles = (int32_t)b->tiles.size();
     // tile queues: order 0 = grouped by grid and cut into one queue per XCD with
     // balanced point counts; 1 = one queue in instance order; 2 = one queue grouped by grid
@@ -481,6 +533,7 @@ static int plan(atac_batch b, const atac_tuning* t) {
     if (!rc) rc = dalloc(&b->d_tile_chunk0, b->n_tiles);
     if (!rc) rc = dalloc(&b->d_chunks, (size_t)b->chunk_cap);
     if (rc) return rc;
+    ATAC_HIP(hipMemset(b->d_counters, 0, sizeof(int32_t) * CNT_WORDS));   // no stale work-item count before the first evaluation
     ATAC_HIP(hipMemcpy(b->d_tq_begin, tq.data(), sizeof(int32_t) * tq.size(), hipMemcpyHostToDevice));
     if (b->n_tiles > 0) {
         ATAC_HIP(hipMemcpy(b->d_tiles, b->tiles.data(), sizeof(TileDev) * b->n_tiles, hipMemcpyHostToDevice));
@@ -541,6 +594,8 @@ int atac_batch_create(const atac_grid* grids, int32_t n_grids, const atac_points
     b->inst_dev.resize(n_inst);
     for (int i = 0; i < n_inst; ++i)
         compose_for(b, inst[i], b->inst_dev[i]);
+    rc = ensure_cls(b);
+    if (rc) { atac_batch_destroy(b); return rc; }
     std::vector<GridDev> gd(n_grids);
     for (int i = 0; i < n_grids; ++i) gd[i] = b->grids[i]->dev;
     rc = dalloc(&b->d_grids, n_grids);
@@ -565,6 +620,7 @@ int atac_batch_destroy(atac_batch b) {
     for (auto& u : b->ev_used) { (void)hipEventDestroy(u.second.first); (void)hipEventDestroy(u.second.second); }
     for (auto e : b->ev_pool) (void)hipEventDestroy(e);
     dfree(b->d_inst); dfree(b->d_grids);
+    for (auto& ct : b->cls) dfree(ct.d);
     dfree(b->h_out); dfree(b->h_nact); dfree(b->h_gfm); dfree(b->h_grad);
     if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
     delete b;
@@ -585,9 +641,23 @@ int atac_batch_set_instances(atac_batch b, const atac_instance* inst, int32_t n_
         b->inst_dev[i].tile_begin = tb;
         b->inst_dev[i].tile_count = tc;
     }
+    rc = ensure_cls(b);   // a new col_th needs its classification table
+    if (rc) return rc;
     b->inst.assign(inst, inst + n_inst);
-    if (n_inst)
-        ATAC_HIP(hipMemcpyAsync(b->d_inst, b->inst_dev.data(), sizeof(InstDev) * n_inst, hipMemcpyHostToDevice, (hipStream_t)stream));
+    // the records of the last evaluation belong to the old poses: the adjoint
+    // must not combine them with the new instance records
+    b->rec_valid = false;
+    if (n_inst) {
+        if (stream) {
+            ATAC_HIP(hipMemcpyAsync(b->d_inst, b->inst_dev.data(), sizeof(InstDev) * n_inst, hipMemcpyHostToDevice,
+                                    (hipStream_t)stream));
+        } else {
+            // no stream: a synchronous upload after everything already queued on
+            // the batch's own stream (the host-buffer entry points run there)
+            ATAC_HIP(hipStreamSynchronize(b->cap_stream));
+            ATAC_HIP(hipMemcpy(b->d_inst, b->inst_dev.data(), sizeof(InstDev) * n_inst, hipMemcpyHostToDevice));
+        }
+    }
     return ATAC_OK;
 }
 
@@ -625,6 +695,8 @@ static int enqueue_eval(atac_batch b, uint32_t flags, const atac_outputs* out, h
     const bool precise = (flags & ATAC_EVAL_PRECISE) != 0;
     if (out && out->dfm_dp6 && mode < 1) return fail(ATAC_ERR_INVALID, "dfm_dp6 requested without ATAC_EVAL_P6 or ATAC_EVAL_DERIV");
     if (out && (out->Km || out->Dm) && mode < 2) return fail(ATAC_ERR_INVALID, "Km/Dm requested without ATAC_EVAL_DERIV");
+    if ((flags & ATAC_EVAL_ALL_ACTIVE) && b->rec_cap < b->n_slots)
+        return fail(ATAC_ERR_INVALID, "ATAC_EVAL_ALL_ACTIVE needs a record per point: plan the batch with max_active < 0");
     int rc = ensure_buffers(b, mode, precise);
     if (rc) return rc;
     if (b->inst.empty()) return ATAC_OK;
@@ -635,21 +707,23 @@ static int enqueue_eval(atac_batch b, uint32_t flags, const atac_outputs* out, h
         da.q = tile_queues(b);
         da.tiles = b->d_tiles; da.wave_span = b->wave_span; da.inst = b->d_inst; da.grids = b->d_grids;
         da.all_active = (flags & ATAC_EVAL_ALL_ACTIVE) ? 1 : 0;
-        da.act = b->d_act; da.tile_active = b->d_tile_active; da.tile_chunk0 = b->d_tile_chunk0; da.chunks = b->d_chunks;
+        da.act = b->d_act; da.rec_cap = (int32_t)b->rec_cap; da.slow_cap = (int32_t)b->slow_cap;
+        da.chunk_cap = (int32_t)b->chunk_cap;
+        da.tile_active = b->d_tile_active; da.tile_chunk0 = b->d_tile_chunk0; da.chunks = b->d_chunks;
         da.slow = b->d_slow; da.counters = counters(b);
         hipEvent_t t0 = stage_begin(b, st);
         ATAC_HIP(launch_detect(da, b->tune.detect_block, b->tune.points_per_lane, b->grid_detect, b->lds_detect, st));
         stage_end(b, ST_DETECT, t0, st);
         SlowArgs sa{};
         sa.inst = b->d_inst; sa.grids = b->d_grids; sa.act = b->d_act; sa.rec = rec; sa.slow_diag = b->d_slow_diag;
-        sa.slow = b->d_slow; sa.counters = counters(b);
+        sa.slow = b->d_slow; sa.counters = counters(b); sa.slow_cap = (int32_t)b->slow_cap;
         hipEvent_t t2 = stage_begin(b, st);
         ATAC_HIP(launch_slow(sa, precise, b->grid_slow, st));
         stage_end(b, ST_SLOW, t2, st);
         ContactArgs ca{};
         ca.chunks = b->d_chunks; ca.inst = b->d_inst; ca.grids = b->d_grids; ca.act = b->d_act; ca.rec = rec;
         ca.slow_diag = b->d_slow_diag;
-        ca.counters = counters(b); ca.partial = b->d_part;
+        ca.counters = counters(b); ca.chunk_cap = (int32_t)b->chunk_cap; ca.partial = b->d_part;
         hipEvent_t t3 = stage_begin(b, st);
         ATAC_HIP(launch_contact(ca, mode, precise, b->grid_contact[precise ? 1 : 0][mode], st));
         stage_end(b, ST_CONTACT, t3, st);
@@ -663,26 +737,42 @@ static int enqueue_eval(atac_batch b, uint32_t flags, const atac_outputs* out, h
     ra.dst[2] = out ? out->Km : nullptr;      ra.begin[2] = 24;  ra.width[2] = 144;
     ra.dst[3] = out ? out->Dm : nullptr;      ra.begin[3] = 168; ra.width[3] = 144;
     ra.n_active = out ? out->n_active : nullptr;
+    ra.counters = counters(b);
     hipEvent_t t3 = stage_begin(b, st);
     ATAC_HIP(launch_reduce(ra, st));
     stage_end(b, ST_REDUCE, t3, st);
-    b->rec_valid = true;
-    b->rec_precise = precise;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    ATAC_HIP(hipStreamIsCapturing(st, &cs));
+    if (cs == hipStreamCaptureStatusNone) {
+        b->rec_valid = true;
+        b->rec_precise = precise;
+    } else {
+        // recorded only: the records exist once the graph has run (atac_batch_graph_launch)
+        b->rec_valid = false;
+        b->rec_captured = true;
+        b->cap_precise = precise;
+    }
     return ATAC_OK;
 }
 
+// The adjoint of the last evaluation that has run (or, while capturing, of the
+// evaluation recorded just before it in the same graph).
 static int enqueue_vjp(atac_batch b, const double* g_fm, const atac_gradients* g, hipStream_t st) {
-    if (!b->rec_valid) return fail(ATAC_ERR_STATE, "atac_batch_vjp: no evaluation of this batch to differentiate");
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    ATAC_HIP(hipStreamIsCapturing(st, &cs));
+    const bool capturing = cs != hipStreamCaptureStatusNone;
+    if (capturing ? !b->rec_captured : !b->rec_valid)
+        return fail(ATAC_ERR_STATE, "atac_batch_vjp: no evaluation of this batch to differentiate");
     if (!g_fm) return fail(ATAC_ERR_INVALID, "atac_batch_vjp: g_fm is null");
-    int rc = ensure_buffers(b, 3, b->rec_precise);
+    const bool precise = capturing ? b->cap_precise : b->rec_precise;
+    int rc = ensure_buffers(b, 3, precise);
     if (rc) return rc;
     if (b->inst.empty()) return ATAC_OK;
-    const bool precise = b->rec_precise;
     if (b->n_tiles > 0) {
         VjpArgs va{};
         va.chunks = b->d_chunks; va.inst = b->d_inst;
         va.rec = precise ? (const void*)b->d_rec64 : (const void*)b->d_rec32; va.slow_diag = b->d_slow_diag;
-        va.counters = counters(b); va.g_fm = g_fm; va.partial = b->d_part;
+        va.counters = counters(b); va.chunk_cap = (int32_t)b->chunk_cap; va.g_fm = g_fm; va.partial = b->d_part;
         hipEvent_t t0 = stage_begin(b, st);
         ATAC_HIP(launch_vjp(va, precise, b->grid_vjp[precise ? 1 : 0], st));
         stage_end(b, ST_VJP, t0, st);
@@ -697,6 +787,7 @@ static int enqueue_vjp(atac_batch b, const double* g_fm, const atac_gradients* g
     ra.dst[3] = g ? g->g_phi2 : nullptr; ra.begin[3] = 18; ra.width[3] = 6;
     ra.dst[4] = g ? g->g_kn : nullptr;   ra.begin[4] = 24; ra.width[4] = 1;
     ra.dst[5] = g ? g->g_kt : nullptr;   ra.begin[5] = 25; ra.width[5] = 1;
+    ra.counters = counters(b);
     hipEvent_t t1 = stage_begin(b, st);
     ATAC_HIP(launch_reduce(ra, st));
     stage_end(b, ST_VJP_REDUCE, t1, st);
@@ -741,6 +832,10 @@ int atac_batch_graph_capture(atac_batch b, uint32_t flags, const atac_outputs* o
 int atac_batch_graph_launch(atac_batch b, void* stream) {
     if (!b || !b->graph_exec) return fail(ATAC_ERR_STATE, "atac_batch_graph_launch: no captured graph");
     ATAC_HIP(hipGraphLaunch(b->graph_exec, (hipStream_t)stream));
+    if (b->rec_captured) {   // the graph's evaluation leaves its records for atac_batch_vjp
+        b->rec_valid = true;
+        b->rec_precise = b->cap_precise;
+    }
     return ATAC_OK;
 }
 
@@ -749,18 +844,25 @@ int atac_batch_active_ids(atac_batch b, int32_t inst, int64_t* ids, int64_t capa
     if (inst < 0 || inst >= (int32_t)b->inst.size()) return fail(ATAC_ERR_INVALID, "atac_batch_active_ids: bad instance");
     if (!b->rec_valid) return fail(ATAC_ERR_STATE, "atac_batch_active_ids: the batch has not been evaluated");
     ATAC_HIP(hipDeviceSynchronize());
+    int32_t ovf = 0;
+    ATAC_HIP(hipMemcpy(&ovf, b->d_counters + CNT_OVERFLOW, sizeof ovf, hipMemcpyDeviceToHost));
+    if (ovf) return fail(ATAC_ERR_STATE, "atac_batch_active_ids: the last evaluation overflowed its record pool");
     const InstDev& d = b->inst_dev[inst];
-    std::vector<int32_t> ta(d.tile_count);
-    if (d.tile_count)
+    std::vector<int32_t> ta(d.tile_count), tc(d.tile_count);
+    if (d.tile_count) {
         ATAC_HIP(hipMemcpy(ta.data(), b->d_tile_active + d.tile_begin, sizeof(int32_t) * d.tile_count, hipMemcpyDeviceToHost));
+        ATAC_HIP(hipMemcpy(tc.data(), b->d_tile_chunk0 + d.tile_begin, sizeof(int32_t) * d.tile_count, hipMemcpyDeviceToHost));
+    }
     const atac_points_s& P = *b->pts[b->inst[inst].points];
     const int64_t seg_begin = P.seg[b->inst[inst].segment];
     std::vector<int64_t> out;
     for (int k = 0; k < d.tile_count; ++k) {
-        const TileDev& t = b->tiles[d.tile_begin + k];
         if (!ta[k]) continue;
+        // the tile's first work item gives the start of its range in the pool
+        ChunkDev ch;
+        ATAC_HIP(hipMemcpy(&ch, b->d_chunks + tc[k], sizeof ch, hipMemcpyDeviceToHost));
         std::vector<uint32_t> off(ta[k]);
-        ATAC_HIP(hipMemcpy(off.data(), b->d_act + t.slot, sizeof(uint32_t) * ta[k], hipMemcpyDeviceToHost));
+        ATAC_HIP(hipMemcpy(off.data(), b->d_act + ch.rec0, sizeof(uint32_t) * ta[k], hipMemcpyDeviceToHost));
         for (uint32_t o : off) out.push_back(P.perm[seg_begin + (o & ~kActSlow)]);
     }
     std::sort(out.begin(), out.end());
@@ -828,6 +930,41 @@ int atac_eval_plain(atac_batch b, uint32_t flags, const atac_outputs* out, void*
     return ATAC_OK;
 }
 
+static int ensure_staging(atac_batch b);
+
+int atac_eval_plain_vjp(atac_batch b, uint32_t flags, const double* g_fm, const atac_outputs* out,
+                        const atac_gradients* grads, void* stream) {
+    if (!b || !g_fm || !out || !out->fm || !out->n_active || !grads)
+        return fail(ATAC_ERR_INVALID, "atac_eval_plain_vjp: g_fm, fm, n_active and the gradients are required");
+    if (flags & ~ATAC_EVAL_PRECISE) return fail(ATAC_ERR_INVALID, "atac_eval_plain_vjp: only ATAC_EVAL_PRECISE is accepted");
+    hipStream_t st = (hipStream_t)stream;
+    const int n = (int)b->inst.size();
+    if (n == 0) return ATAC_OK;
+    // the per-instance adjoint rows (kVjpOut each) in library staging, then split
+    int rc = ensure_staging(b);
+    if (rc) return rc;
+    double* G = b->h_grad;
+    ATAC_HIP(launch_zero(out->fm, (int64_t)n * 12 * 2, st));
+    ATAC_HIP(launch_zero(out->n_active, n, st));
+    ATAC_HIP(launch_zero(G, (int64_t)n * kVjpOut * 2, st));
+    for (int i = 0; i < n; ++i) {
+        PlainArgs pa{};
+        pa.inst = b->d_inst; pa.i = i; pa.grids = b->d_grids;
+        pa.fm = out->fm + 12 * (int64_t)i;
+        pa.n_active = out->n_active + i;
+        pa.g_fm = g_fm + 12 * (int64_t)i;
+        pa.grad = G + (int64_t)kVjpOut * i;
+        ATAC_HIP(launch_plain(pa, b->inst_dev[i].pt_count, (flags & ATAC_EVAL_PRECISE) != 0, st));
+    }
+    double* dst[6] = {grads->g_q1, grads->g_q2, grads->g_phi1, grads->g_phi2, grads->g_kn, grads->g_kt};
+    const int begin[6] = {0, 6, 12, 18, 24, 25}, width[6] = {6, 6, 6, 6, 1, 1};
+    for (int s = 0; s < 6; ++s)
+        if (dst[s])
+            ATAC_HIP(hipMemcpy2DAsync(dst[s], sizeof(double) * width[s], G + begin[s], sizeof(double) * kVjpOut,
+                                      sizeof(double) * width[s], n, hipMemcpyDeviceToDevice, st));
+    return ATAC_OK;
+}
+
 // ---------------------------------------------------------------- per-point API
 static PointArgs point_args(atac_grid g, const double E2[16], const double* phi2, double col_th) {
     PointArgs a{};
@@ -863,6 +1000,17 @@ int atac_sdf_collision(atac_grid g, const double E2[16], const double phi2[6], d
     PointArgs a = point_args(g, E2, phi2, col_th);
     a.xw = xw; a.xwdot = xw_dot; a.n = n; a.out = values; a.out2 = derivs;
     ATAC_HIP(launch_collision(a, (flags & ATAC_EVAL_PRECISE) != 0, (hipStream_t)stream));
+    return ATAC_OK;
+}
+
+int atac_batch_workspace(atac_batch b, int64_t* capacity, int64_t* used, int32_t* overflowed) {
+    if (!b) return fail(ATAC_ERR_INVALID, "atac_batch_workspace: null batch");
+    int32_t c[CNT_WORDS] = {0, 0, 0, 0};
+    ATAC_HIP(hipDeviceSynchronize());
+    ATAC_HIP(hipMemcpy(c, b->d_counters, sizeof c, hipMemcpyDeviceToHost));
+    if (capacity) *capacity = b->rec_cap;
+    if (used) *used = c[CNT_RECS];
+    if (overflowed) *overflowed = c[CNT_OVERFLOW] != 0 ? 1 : 0;
     return ATAC_OK;
 }
 
@@ -920,7 +1068,12 @@ int atac_batch_eval_host(atac_batch b, uint32_t flags, double* fm, double* Km, d
     if (Km) ATAC_HIP(hipMemcpyAsync(Km, o.Km, sizeof(double) * 144 * n, hipMemcpyDeviceToHost, st));
     if (Dm) ATAC_HIP(hipMemcpyAsync(Dm, o.Dm, sizeof(double) * 144 * n, hipMemcpyDeviceToHost, st));
     if (n_active) ATAC_HIP(hipMemcpyAsync(n_active, o.n_active, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
+    int32_t ovf = 0;
+    ATAC_HIP(hipMemcpyAsync(&ovf, b->d_counters + CNT_OVERFLOW, sizeof ovf, hipMemcpyDeviceToHost, st));
     ATAC_HIP(hipStreamSynchronize(st));
+    if (ovf)
+        return fail(ATAC_ERR_NOMEM, "atac_batch_eval_host: more active contacts than the record pool holds (" +
+                                        std::to_string(b->rec_cap) + "); plan the batch with a larger max_active");
     return ATAC_OK;
 }
 

@@ -25,7 +25,7 @@ inline void compose_instance(const atac_instance& s, const GridDev& G, const flo
         }
         double t = 0.0;
         for (int k = 0; k < 3; ++k) t += R2[3 * k + r] * (p1[k] - p2[k]);
-        d.b[r] = (t - G.minb[r]) * G.inv_d[r];
+        d.bs[r] = (t - G.minb[r]) * G.inv_d[r] + 1.0;
         d.cmax[r] = (G.maxb[r] - G.minb[r]) * G.inv_d[r];
     }
     std::memcpy(d.R1, R1, sizeof R1);
@@ -57,7 +57,8 @@ inline void compose_instance(const atac_instance& s, const GridDev& G, const flo
     }
     d.fkn = (float)d.kn; d.fkt = (float)d.kt; d.fmu = (float)d.mu;
     d.fdamping = (float)d.damping; d.fcontact_scale = (float)d.contact_scale;
-    block_thresholds(d.col_th, d.blk_off, d.blk_on);
+    d.fpad0 = 0.0f;
+    d.cls = nullptr;   // set by the batch (its table for this grid and col_th)
     for (float& f : d.fpad) f = 0.0f;
 }
 

@@ -30,7 +30,11 @@ struct DetectArgs {
     int32_t all_active;         // every point is a contact (fixed contact lists)
     const InstDev* inst;
     const GridDev* grids;
-    uint32_t* act;              // per slot: offset of the active point within its instance (| kActSlow)
+    uint32_t* act;              // pool (rec_cap entries): offset of an active point within its instance (| kActSlow)
+    int32_t rec_cap;
+    int32_t slow_cap;           // entries of the slow list
+    int32_t chunk_cap;          // entries of the work-item list
+    int32_t pad;
     int32_t* tile_active;       // n_tiles
     int32_t* tile_chunk0;       // n_tiles: first work item of the tile
     ChunkDev* chunks;           // work items, pushed per tile
@@ -42,10 +46,12 @@ struct SlowArgs {
     const InstDev* inst;
     const GridDev* grids;
     const uint32_t* act;
-    void* rec;                  // Rec32 or Rec64 per slot
+    void* rec;                  // Rec32 or Rec64: the slow contacts' records are written here
     float* slow_diag;           // 4 per slow contact (FP32 records)
     const SlowDev* slow;
     const int32_t* counters;
+    int32_t slow_cap;
+    int32_t pad;
 };
 
 struct ContactArgs {
@@ -56,6 +62,8 @@ struct ContactArgs {
     void* rec;                  // fast-path records are written here, slow ones read
     const float* slow_diag;
     int32_t* counters;
+    int32_t chunk_cap;
+    int32_t pad;
     double* partial;            // per work item: fm 12 | p6 12 | Km 144 | Dm 144 (as the mode has them)
 };
 
@@ -65,6 +73,8 @@ struct VjpArgs {
     const void* rec;
     const float* slow_diag;
     int32_t* counters;
+    int32_t chunk_cap;
+    int32_t pad;
     const double* g_fm;         // n_inst x 12
     double* partial;            // per work item: kVjpOut
 };
@@ -85,6 +95,7 @@ struct ReduceArgs {
     int32_t begin[6];
     int32_t width[6];
     int32_t* n_active;          // optional: active contacts per instance
+    const int32_t* counters;    // CNT_OVERFLOW set: every output is NaN (the records were incomplete)
 };
 
 struct PlainArgs {
@@ -97,6 +108,8 @@ struct PlainArgs {
     double* Km;
     double* Dm;
     int32_t* n_active;
+    const double* g_fm;         // optional: d(loss)/d fm of the instance (12) ...
+    double* grad;               // ... and the adjoint it gives (kVjpOut, as atac_batch_vjp's)
 };
 
 struct PointArgs {
@@ -124,7 +137,9 @@ const void* contact_kernel_ptr(int mode, bool precise);
 hipError_t launch_vjp(const VjpArgs& a, bool precise, int grid, hipStream_t st);
 const void* vjp_kernel_ptr(bool precise);
 hipError_t launch_reduce(const ReduceArgs& a, hipStream_t st);
-hipError_t launch_grid_setup(const GridDev& G, float4* cells, float2* blk, hipStream_t st);
+hipError_t launch_grid_setup(const GridDev& G, float4* cells, hipStream_t st);
+// the cell classification of a grid for one col_th (cls_bytes(G) bytes)
+hipError_t launch_cls_setup(const GridDev& G, double col_th, uint8_t* cls, hipStream_t st);
 hipError_t launch_reset(int32_t* c, int n, hipStream_t st);
 hipError_t launch_zero(void* p, int64_t n_words, hipStream_t st);
 hipError_t launch_plain(const PlainArgs& a, int64_t n_points, bool precise, hipStream_t st);

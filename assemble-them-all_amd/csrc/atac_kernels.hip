@@ -153,6 +153,7 @@ __global__ __launch_bounds__(BLOCK) void detect_kernel(DetectArgs a) {
     constexpr int CAP = detect_ring_cap(PPL);
     extern __shared__ __align__(16) unsigned char s_lds[];
     __shared__ int s_wcount[NW];
+    __shared__ int s_base;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int span = a.wave_span;
     uint16_t* s_queue = reinterpret_cast<uint16_t*>(s_lds);   // [NW][span] active offsets, bit 15 = slow path
@@ -261,27 +262,50 @@ __global__ __launch_bounds__(BLOCK) void detect_kernel(DetectArgs a) {
         int cnt = 0;
 #pragma unroll
         for (int k = 0; k < NW; ++k) cnt += s_wcount[k];
-        uint32_t* act = a.act + t.slot;
-        const uint32_t base = (uint32_t)t.begin;
-        for (int qi = tid; qi < cnt; qi += BLOCK) {
-            int rem = qi, ww = 0;
-            while (rem >= s_wcount[ww]) { rem -= s_wcount[ww]; ++ww; }
-            const uint32_t e = s_queue[ww * span + rem];
-            const bool slow = (e >> 15) != 0u;
-            act[qi] = (base + (e & 0x7fffu)) | (slow ? kActSlow : 0u);
-            if (slow) {
-                const int si = atomicAdd(a.counters + CNT_SLOW, 1);
-                a.slow[si] = SlowDev{(uint32_t)(t.slot + qi), t.inst};
-            }
-        }
+        // the tile's records: one range of the pool, reserved by one atomic; a
+        // tile that does not fit marks the evaluation as overflowed and writes
+        // nothing (its outputs then come out NaN)
         if (tid == 0) {
-            a.tile_active[tile] = cnt;
+            int base = -1;
             if (cnt > 0) {
                 const int nch = (cnt + kChunk - 1) / kChunk;
-                const int c0 = atomicAdd(a.counters + CNT_CHUNKS, nch);
-                a.tile_chunk0[tile] = c0;
-                for (int k = 0; k < nch; ++k)
-                    a.chunks[c0 + k] = ChunkDev{t.inst, min(kChunk, cnt - k * kChunk), t.slot + (int64_t)k * kChunk};
+                const unsigned long long old = atomicAdd(reinterpret_cast<unsigned long long*>(a.counters + CNT_RECS),
+                                                         ((unsigned long long)nch << 32) | (unsigned long long)cnt);
+                const int r0 = (int)(uint32_t)old, c0 = (int)(uint32_t)(old >> 32);
+                const bool ok = r0 <= a.rec_cap - cnt && c0 <= a.chunk_cap - nch;
+                if (ok) {
+                    base = r0;
+                    a.tile_chunk0[tile] = c0;
+                } else {
+                    atomicOr(a.counters + CNT_OVERFLOW, 1);
+                }
+                // the items this tile took: its ranges, or (overflowed) empty ones
+                for (int k = 0; k < nch && c0 + k < a.chunk_cap; ++k)
+                    a.chunks[c0 + k] = ok ? ChunkDev{t.inst, min(kChunk, cnt - k * kChunk), (int64_t)base + (int64_t)k * kChunk}
+                                          : ChunkDev{t.inst, 0, 0};
+            }
+            s_base = base;
+            a.tile_active[tile] = base >= 0 ? cnt : 0;
+        }
+        __syncthreads();
+        const int base = s_base;
+        if (base >= 0) {
+            // the tile's active offsets, in ascending point order, at its pool range;
+            // the contact pass builds their records (the slow pass those of the
+            // contacts whose FD stencil leaves the cell)
+            uint32_t* act = a.act + base;
+            const uint32_t tb = (uint32_t)t.begin;
+            for (int qi = tid; qi < cnt; qi += BLOCK) {
+                int rem = qi, ww = 0;
+                while (rem >= s_wcount[ww]) { rem -= s_wcount[ww]; ++ww; }
+                const uint32_t e = s_queue[ww * span + rem];
+                const bool slow = (e >> 15) != 0u;
+                act[qi] = (tb + (e & 0x7fffu)) | (slow ? kActSlow : 0u);
+                if (slow) {
+                    const int si = atomicAdd(a.counters + CNT_SLOW, 1);
+                    if (si < a.slow_cap) a.slow[si] = SlowDev{(uint32_t)(base + qi), t.inst};
+                    else atomicOr(a.counters + CNT_OVERFLOW, 1);
+                }
             }
         }
         __syncthreads();
@@ -307,7 +331,7 @@ __global__ __launch_bounds__(64) void slow_kernel(SlowArgs a) {
     constexpr int GROUPS = 64 / kStencil;
     __shared__ double s_v[GROUPS][kStencil];
     const int lane = threadIdx.x, grp = lane / kStencil, st = lane % kStencil;
-    const int n = a.counters[CNT_SLOW];
+    const int n = min(a.counters[CNT_SLOW], a.slow_cap);
     for (int base = (int)blockIdx.x * GROUPS; base < n; base += (int)gridDim.x * GROUPS) {
         const int i = base + grp;
         const bool mine = grp < GROUPS && i < n;
@@ -383,7 +407,7 @@ __global__ __launch_bounds__(256) void contact_kernel(ContactArgs a) {
     __shared__ double s_km[DERIV ? 4 : 1][DERIV ? 288 : 1];
     __shared__ double s_red[4][64 * NF];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int total = a.counters[CNT_CHUNKS];
+    const int total = min(a.counters[CNT_CHUNKS], a.chunk_cap);
     for (int c = wave_id(); c < total; c += wave_count()) {
         const ChunkDev ch = as_const(a.chunks)[c];
         const InstDev in = as_const(a.inst)[ch.inst];
@@ -479,7 +503,7 @@ __global__ __launch_bounds__(256) void vjp_kernel(VjpArgs a) {
     using Rec = typename RecOf<R>::type;
     __shared__ double s_red[4][64 * kVjpOut];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int total = a.counters[CNT_CHUNKS];
+    const int total = min(a.counters[CNT_CHUNKS], a.chunk_cap);
     for (int c = wave_id(); c < total; c += wave_count()) {
         const ChunkDev ch = as_const(a.chunks)[c];
         const InstDev in = as_const(a.inst)[ch.inst];
@@ -531,6 +555,8 @@ __global__ __launch_bounds__(256) void reduce_kernel(ReduceArgs a) {
     const InstDev in = as_const(a.inst)[i];
     const int tb = in.tile_begin, nt = in.tile_count;
     const auto part = as_global(a.partial);
+    // an evaluation whose records did not fit is incomplete: NaN outputs, n_active -1
+    const bool ovf = a.counters[CNT_OVERFLOW] != 0;
     int nact = 0;
     for (int comp0 = 0; comp0 < a.nacc; comp0 += 64) {
         const int comp = comp0 + lane;
@@ -575,6 +601,7 @@ __global__ __launch_bounds__(256) void reduce_kernel(ReduceArgs a) {
                 if (nr < kRedGroup) break;
             }
         }
+        if (ovf) sum = __builtin_nan("");
         if (mine) {
             for (int sg = 0; sg < a.nseg; ++sg) {
                 const int rel = comp - a.begin[sg];
@@ -584,7 +611,7 @@ __global__ __launch_bounds__(256) void reduce_kernel(ReduceArgs a) {
     }
     if (a.n_active) {
         const int n = (int)wave_sum((double)nact);
-        if (lane == 0) a.n_active[i] = n;
+        if (lane == 0) a.n_active[i] = ovf ? -1 : n;
     }
 }
 
@@ -595,17 +622,19 @@ __global__ void build_cells_kernel(GridDev G, float4* cells) {
         pack_cell(G, (int)(c % ci), (int)((c / ci) % cj), (int)(c / (ci * cj)), cells + 2 * c);
 }
 
-__global__ void build_blocks_kernel(GridDev G, float2* blk) {
-    const int64_t ne = (int64_t)blk_outside(G);   // + the outside entry
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e <= ne; e += (int64_t)gridDim.x * blockDim.x)
-        blk[e] = blk_fill(G, (uint32_t)e);
+__global__ void build_cls_kernel(GridDev G, double col_th, uint8_t* cls) {
+    const int64_t ne = (int64_t)cls_bytes(G);
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x)
+        cls[e] = cls_fill(G, col_th, (uint32_t)e);
 }
 
-hipError_t launch_grid_setup(const GridDev& G, float4* cells, float2* blk, hipStream_t st) {
+hipError_t launch_grid_setup(const GridDev& G, float4* cells, hipStream_t st) {
     hipLaunchKernelGGL(build_cells_kernel, dim3(1024), dim3(256), 0, st, G, cells);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(build_blocks_kernel, dim3(1024), dim3(256), 0, st, G, blk);
+    return hipGetLastError();
+}
+
+hipError_t launch_cls_setup(const GridDev& G, double col_th, uint8_t* cls, hipStream_t st) {
+    hipLaunchKernelGGL(build_cls_kernel, dim3(1024), dim3(256), 0, st, G, col_th, cls);
     return hipGetLastError();
 }
 
@@ -642,6 +671,14 @@ __global__ void plain_kernel(PlainArgs a) {
             double* dst = (j < 12) ? a.Km + j : a.Dm + (j - 12);
             for (int r = 0; r < 12; ++r) atomicAdd(dst + 12 * r, (double)col[r]);
         }
+    }
+    if (a.grad) {
+        R gb[12], v[kVjpOut];
+        for (int k = 0; k < 12; ++k) gb[k] = (R)a.g_fm[k];
+        contact_vjp<R>(I, s, gb, v);
+        v[24] *= (R)in.kn_chain;   // d/d kn = kn_chain * d/d kn_eff (ForceGeneralSDFContact.cpp:155-156)
+        v[25] *= (R)in.kn_chain;
+        for (int k = 0; k < kVjpOut; ++k) atomicAdd(a.grad + k, (double)v[k]);
     }
     atomicAdd(a.n_active, 1);
 }
@@ -741,10 +778,9 @@ __global__ void touched_kernel(TouchedArgs a) {
     const InstDev& in = a.inst[inst_i];
     for (int64_t o = gid; o < in.pt_count; o += (int64_t)gridDim.x * blockDim.x) {
         const GridDev G = a.grids[in.grid];
-        const double x = in.px[o], y = in.py[o], z = in.pz[o];
-        const double c0 = in.A[0] * x + in.A[1] * y + in.A[2] * z + in.b[0];
-        const double c1 = in.A[3] * x + in.A[4] * y + in.A[5] * z + in.b[1];
-        const double c2 = in.A[6] * x + in.A[7] * y + in.A[8] * z + in.b[2];
+        double cc[3];
+        grid_coords(in, in.px[o], in.py[o], in.pz[o], cc);
+        const double c0 = cc[0], c1 = cc[1], c2 = cc[2];
         if (!(c0 >= 0.0 && c0 <= in.cmax[0] && c1 >= 0.0 && c1 <= in.cmax[1] && c2 >= 0.0 && c2 <= in.cmax[2])) continue;
         int i = (int)c0, j = (int)c1, k = (int)c2;
         i = i < G.n[0] - 1 ? i : G.n[0] - 1;

@@ -19,29 +19,20 @@ struct GridDev {
     // interpolates (v000 v100 v010 v110 | v001 v101 v011 v111, offsets clamped at
     // the last node) in 2 x float4, node-index order
     const float4* cells;
-    // conservative bounds of the detection value per block of kBlk^3 cells: the
-    // (min, max) of every node the block's cells read (with the detection's clamp
-    // at the last node) widened by kBlkTol * max(|min|, |max|) and rounded
-    // outward, or (-inf, +inf) if one of them is not finite; blocks grouped in
-    // bricks of 2x2x2, then one more entry, (+inf, +inf), the bound of points
-    // outside the box (blk_entry / blk_outside)
-    const float2* blk;
-    int32_t nb[3];      // blocks per axis: (n - 1) / kBlk + 1
-    int32_t pad;
+
     // the FD stencil's reach in cell units, 1e-5 * inv_d + 1e-9 (fd_m), and 1 - fd_m
     double fd_m[3];
     double fd_1m[3];
 };
 
-constexpr int kBlk = 2;
-
 // One force object = (contact body points, SDF body grid, poses, parameters).
 // Everything the kernels read per instance, in FP64 as the reference keeps it.
 struct InstDev {
-    // detection map straight into grid index space: c = A * xi + b
-    // (A = diag(1/d) R2^T R1, b = diag(1/d) (R2^T (p1 - p2) - min_box))
+    // detection map straight into grid index space, shifted by one cell:
+    // c' = A * xi + bs, c = c' - 1 (A = diag(1/d) R2^T R1,
+    // bs = diag(1/d) (R2^T (p1 - p2) - min_box) + 1)
     double A[9];
-    double b[3];
+    double bs[3];
     double cmax[3];     // (max_box - min_box) / d: the inside test in index space
     // poses (row-major R) and body-frame twists [w; v]
     double R1[9], p1[3], w1[3], v1[3];
@@ -62,10 +53,12 @@ struct InstDev {
     // arithmetic (read as wave-uniform scalars)
     float fR1[9], fR2[9], fw1[3], fv1[3], fw2[3], fv2[3];
     float fkn, fkt, fmu, fdamping, fcontact_scale;
-    // thresholds of the widened block bounds (block_decision): a block with
-    // min > blk_off holds no active point, one with max < blk_on only active ones
-    float blk_off, blk_on;
-    float fpad[11];
+    float fpad0;
+    // per-cell classification of the instance's grid for its col_th (cls_fill;
+    // bricked, cls_offset): 0 no active point, 1 only active points, 2 open
+    const uint8_t* cls;
+    double dpad[3];
+    float fpad[4];
 };
 static_assert(sizeof(InstDev) == 704, "InstDev layout");
 
@@ -78,10 +71,11 @@ struct TileDev {
     int32_t pad;
 };
 
-// Active-contact record, written by the detection pass for every point with
-// d < 0 at rec[tile.slot + k] (k-th active point of the tile, ascending point
-// order) and read by the contact and adjoint passes (it is also the adjoint's
-// checkpoint: the same d, grad phi and Hessian are used by both).
+// Active-contact record, one per point with d < 0, at the slot detection gave
+// it in the batch's record pool (a tile's contacts in ascending point order in
+// one range): written by the contact pass (the slow pass for contacts whose FD
+// stencil leaves the cell) and read by the contact and adjoint passes (it is
+// also the adjoint's checkpoint: the same d, grad phi and Hessian are used by both).
 //   a = (xi1.x, xi1.y, xi1.z, bits(point offset within the instance))
 //   b = (d, g.x, g.y, g.z)            d = query_SDF(x) + col_th, g = grad phi
 //   c = (Hxy, Hxz, Hyz, bits(slow))   slow = -1, or the index of the contact in
@@ -113,11 +107,13 @@ struct ChunkDev {
 
 constexpr int kChunk = 256;         // records per work item
 
-// Active list: per slot, the point's offset within its instance, with kActSlow
-// set when its FD stencil leaves the cell (its record comes from the slow pass).
+// Active list (the record pool's slots): the point's offset within its
+// instance, with kActSlow set when its FD stencil leaves the cell (its record
+// comes from the slow pass).
 constexpr uint32_t kActSlow = 0x80000000u;
 
-// Slow-list entry: the record slot and the instance of a slow-path contact.
+// Slow-list entry: the record slot and the instance of a contact whose FD
+// stencil leaves its cell (its record is written by the slow pass).
 struct SlowDev {
     uint32_t slot;
     int32_t inst;
@@ -125,9 +121,14 @@ struct SlowDev {
 
 // Device counters of one evaluation (reset before detection).
 enum {
-    CNT_CHUNKS = 0,                 // work items pushed by detection
-    CNT_SLOW = 1,                   // contacts in the slow list
-    CNT_WORDS = 2
+    // records reserved from the pool and work items pushed, the low and high
+    // word of one 64-bit counter (a tile takes both with one atomic); both may
+    // run past their capacity
+    CNT_RECS = 0,
+    CNT_CHUNKS = 1,
+    CNT_SLOW = 2,                   // contacts in the slow list (also past its capacity)
+    CNT_OVERFLOW = 3,               // non-zero: the record pool or the slow list was too small
+    CNT_WORDS = 4
 };
 
 constexpr int kFmOut = 12;          // wrench of both bodies

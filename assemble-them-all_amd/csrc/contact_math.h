@@ -292,6 +292,16 @@ inline int cvt_sat_i32(double c) {
 }
 inline uint32_t mad24(uint32_t a, uint32_t b, uint32_t c) { return (a & 0xffffffu) * (b & 0xffffffu) + c; }
 #endif
+// v clamped to [0, hi] (one v_med3_i32)
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ int med3_i32(int v, int hi) {
+    int r;
+    asm("v_med3_i32 %0, %1, 0, %2" : "=v"(r) : "v"(v), "v"(hi));
+    return r;
+}
+#else
+inline int med3_i32(int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); }
+#endif
 
 // FP32 roundings of an FP64 value toward -inf and toward +inf.
 ATAC_HD float f32_down(double x) {
@@ -305,57 +315,73 @@ ATAC_HD float f32_up(double x) {
     return f;
 }
 
-// Margin of the block bounds (relative to the magnitude of the values they
-// bound); block_decision gives the reason for its size.
-constexpr double kBlkTol = 4e-6;
+// Margin of the cell bounds (relative to the magnitude of the values they
+// bound); the classification's comment gives the reason for its size.
+constexpr double kBoundTol = 4e-6;
 
-ATAC_HD float2 block_bounds(const GridDev& G, int bi, int bj, int bk) {
-    // nodes read by the block's cells: [b*kBlk, min(b*kBlk + kBlk, n - 1)] per axis
-    const int i0 = bi * kBlk, i1 = i0 + kBlk < G.n[0] - 1 ? i0 + kBlk : G.n[0] - 1;
-    const int j0 = bj * kBlk, j1 = j0 + kBlk < G.n[1] - 1 ? j0 + kBlk : G.n[1] - 1;
-    const int k0 = bk * kBlk, k1 = k0 + kBlk < G.n[2] - 1 ? k0 + kBlk : G.n[2] - 1;
+// Cell classification of a grid for one col_th, one byte per cell of the grid
+// padded by one cell on every side (padded index i' = i + 1 in [0, n + 1] per
+// axis, i the cell index floor(c)): kClsInactive (no point of the cell is
+// active), kClsActive (every inside point is), kClsOpen (the exact path decides).
+// The trilinear value of a point is a convex combination of its cell's 8 nodes,
+// so it lies in their (min, max), up to the FP32 rounding of the 7 lerps and the
+// col_th add: at most 13 units of 2^-24 of M + |col_th|, M = max|node| (each lerp
+// adds <= 4 units of its largest input; the add 1 unit). A cell is inactive when
+// min - kBoundTol*M > -col_th + kBoundTol*|col_th| and active when
+// max + kBoundTol*M < -col_th - kBoundTol*|col_th| (rounded outward): the value
+// clears the threshold by 4e-6 (M + |col_th|), 5x the rounding bound, so the
+// class decides the point exactly as the interpolation. The stream finds the cell
+// with the shifted map (cls_offset), the same cell grid_coords gives. A cell the
+// box face cuts, and the clamped layer beyond the last node, are open, so the box
+// test of their points is the exact one; cells outside the box are inactive; a
+// cell with a non-finite node is open.
+constexpr uint8_t kClsInactive = 0, kClsActive = 1, kClsOpen = 2;
+
+ATAC_HD int32_t cls_pad(const GridDev& G, int a) { return G.n[a] + 2; }
+ATAC_HD uint32_t cls_bricks(const GridDev& G, int a) { return ((uint32_t)cls_pad(G, a) + 3u) >> 2; }
+ATAC_HD uint32_t cls_bytes(const GridDev& G) { return cls_bricks(G, 0) * cls_bricks(G, 1) * cls_bricks(G, 2) * 64u; }
+ATAC_HD double grid_cmax(const GridDev& G, int a) { return (G.maxb[a] - G.minb[a]) * G.inv_d[a]; }   // as compose_instance
+
+// Byte offset of padded cell (i', j', k'): bricks of 4x4x4 cells (one 64-byte
+// line), bricks x-fastest, cells in a brick x-fastest, so the 64 Morton
+// neighbours a load instruction covers share one or two lines.
+ATAC_HD uint32_t cls_entry(const GridDev& G, uint32_t i, uint32_t j, uint32_t k) {
+    const uint32_t brick = mad24(mad24(k >> 2, cls_bricks(G, 1), j >> 2), cls_bricks(G, 0), i >> 2);
+    return (brick << 6) | ((k & 3u) << 4) | ((j & 3u) << 2) | (i & 3u);
+}
+
+// The classification of byte e of the table for col_th (cells past the padded
+// grid in the last bricks are inactive).
+ATAC_HD uint8_t cls_fill(const GridDev& G, double col_th, uint32_t e) {
+    const uint32_t brick = e >> 6, b0 = cls_bricks(G, 0), b1 = cls_bricks(G, 1);
+    const int ip[3] = {(int)(4u * (brick % b0) + (e & 3u)), (int)(4u * ((brick / b0) % b1) + ((e >> 2) & 3u)),
+                       (int)(4u * (brick / (b0 * b1)) + ((e >> 4) & 3u))};
+    bool open = false;
+    for (int a = 0; a < 3; ++a) {
+        if (ip[a] >= cls_pad(G, a)) return kClsInactive;    // brick padding
+        const int i = ip[a] - 1;
+        const double cm = grid_cmax(G, a);
+        if (i < 0 || (double)i > cm) return kClsInactive;   // outside the box
+        if (i >= G.n[a] || (double)i + 1.0 > cm) open = true;   // cut by the box face, or beyond the last node
+    }
+    if (open) return kClsOpen;
+    const int c[3] = {ip[0] - 1, ip[1] - 1, ip[2] - 1};
     float lo = INFINITY, hi = -INFINITY;
     bool finite = true;
-    for (int k = k0; k <= k1; ++k)
-        for (int j = j0; j <= j1; ++j)
-            for (int i = i0; i <= i1; ++i) {
-                const float v = G.v[grid_index(G, i, j, k)];
+    for (int kk = c[2]; kk <= (c[2] + 1 < G.n[2] - 1 ? c[2] + 1 : G.n[2] - 1); ++kk)
+        for (int jj = c[1]; jj <= (c[1] + 1 < G.n[1] - 1 ? c[1] + 1 : G.n[1] - 1); ++jj)
+            for (int ii = c[0]; ii <= (c[0] + 1 < G.n[0] - 1 ? c[0] + 1 : G.n[0] - 1); ++ii) {
+                const float v = G.v[grid_index(G, ii, jj, kk)];
                 finite = finite && std::isfinite(v);
                 lo = fminf(lo, v);
                 hi = fmaxf(hi, v);
             }
-    if (!finite) return make_float2(-INFINITY, INFINITY);
-    const double m = kBlkTol * fmax(fabs((double)lo), fabs((double)hi));
-    return make_float2(f32_down((double)lo - m), f32_up((double)hi + m));
-}
-
-// Block table layout: blocks in bricks of 2x2x2 (8 entries, one 64-byte line),
-// bricks x-fastest, so the blocks of nearby points share lines. After the bricks
-// comes the entry of points outside the box.
-ATAC_HD uint32_t blk_bricks(const GridDev& G, int a) { return ((uint32_t)G.nb[a] + 1u) >> 1; }
-ATAC_HD uint32_t blk_outside(const GridDev& G) { return blk_bricks(G, 0) * blk_bricks(G, 1) * blk_bricks(G, 2) * 8u; }
-ATAC_HD uint32_t blk_entry(const GridDev& G, uint32_t bx, uint32_t by, uint32_t bz) {
-    const uint32_t brick = mad24(mad24(bz >> 1, blk_bricks(G, 1), by >> 1), blk_bricks(G, 0), bx >> 1);
-    return brick * 8u + ((bz & 1u) << 2 | (by & 1u) << 1 | (bx & 1u));
-}
-// Entry e (<= blk_outside) of the table: the block's bounds, (+inf, +inf) for
-// the outside entry and for the padding of the bricks past the last block.
-ATAC_HD float2 blk_fill(const GridDev& G, uint32_t e) {
-    if (e >= blk_outside(G)) return make_float2(INFINITY, INFINITY);
-    const uint32_t brick = e >> 3, t0 = blk_bricks(G, 0), t1 = blk_bricks(G, 1);
-    const uint32_t bx = 2u * (brick % t0) + (e & 1u), by = 2u * ((brick / t0) % t1) + ((e >> 1) & 1u),
-                   bz = 2u * (brick / (t0 * t1)) + ((e >> 2) & 1u);
-    if (bx >= (uint32_t)G.nb[0] || by >= (uint32_t)G.nb[1] || bz >= (uint32_t)G.nb[2]) return make_float2(INFINITY, INFINITY);
-    return block_bounds(G, (int)bx, (int)by, (int)bz);
-}
-
-// The thresholds block_decision compares an instance's block bounds with:
-// -col_th -+ kBlkTol * |col_th|, for col_th as the detection adds it (FP32),
-// rounded outward.
-ATAC_HD void block_thresholds(double col_th, float& off, float& on) {
-    const double c = (double)(float)col_th, m = kBlkTol * fabs(c);
-    off = f32_up(m - c);
-    on = f32_down(-c - m);
+    if (!finite) return kClsOpen;
+    const double m = kBoundTol * fmax(fabs((double)lo), fabs((double)hi));
+    const double cth = (double)(float)col_th, mt = kBoundTol * fabs(cth);   // col_th as the detection adds it
+    if ((double)f32_down((double)lo - m) > (double)f32_up(mt - cth)) return kClsInactive;
+    if ((double)f32_up((double)hi + m) < (double)f32_down(-cth - mt)) return kClsActive;
+    return kClsOpen;
 }
 
 // FP64 trilinear value of a cell from its FP32 corners, in the reference's
@@ -397,9 +423,12 @@ ATAC_HD void sdf_cell32(const GridDev& G, const Corners& c, const float f[3], fl
 // x is outside [min_box, max_box] (the reference returns FLT_MAX + col_th).
 ATAC_HD void grid_coords(const InstDev& in, float px, float py, float pz, double c[3]) {
     const double x = px, y = py, z = pz;   // three fused FP64 steps per axis, host and device alike
-    c[0] = fma(in.A[2], z, fma(in.A[1], y, fma(in.A[0], x, in.b[0])));
-    c[1] = fma(in.A[5], z, fma(in.A[4], y, fma(in.A[3], x, in.b[1])));
-    c[2] = fma(in.A[8], z, fma(in.A[7], y, fma(in.A[6], x, in.b[2])));
+    // the shifted map c' = A xi + (b + 1), then c = c' - 1: exact whenever c' >= 1
+    // (c' and 1 are multiples of ulp(c')), so floor(c) = floor(c') - 1 for every
+    // point the box can hold and the stream's cell (cls_offset) is this one
+    c[0] = fma(in.A[2], z, fma(in.A[1], y, fma(in.A[0], x, in.bs[0]))) - 1.0;
+    c[1] = fma(in.A[5], z, fma(in.A[4], y, fma(in.A[3], x, in.bs[1]))) - 1.0;
+    c[2] = fma(in.A[8], z, fma(in.A[7], y, fma(in.A[6], x, in.bs[2]))) - 1.0;
 }
 
 ATAC_HD float detect_value(const GridDev& G, const InstDev& in, float px, float py, float pz) {
@@ -463,35 +492,27 @@ ATAC_HD bool detect_interp(const GridDev& G, const InstDev& in, float px, float 
     return cell_value(q[0], q[1], fx, fy, fz, (float)in.col_th) < 0.0f;
 }
 
-// The detection decision detect_value(...) < 0, bit for bit, with fewer grid
-// reads. The trilinear value is a convex combination of its 8 corners, so it lies
-// in the (min, max) of the block holding the cell, up to the FP32 rounding of the
-// 7 lerps and the col_th add: at most 13 units of 2^-24 of M + |col_th|, M =
-// max|node| of the block (each lerp adds <= 4 units of its largest input; the add
-// 1 unit). The table holds min - kBlkTol*M and max + kBlkTol*M, the instance
-// -col_th -+ kBlkTol*|col_th| (all rounded outward), so min' > off means
-// min + col_th > 4e-6 (M + |col_th|), 5x the rounding bound: every point of the
-// block is inactive (and max' < on: every one active). The FP64 set-up arithmetic
-// of these values stays far inside the margin. Otherwise (-1) detect_interp
-// decides the point. The two outcomes exclude each other (min' <= max',
-// on <= off), and the entry of points outside the box, (+inf, +inf), is inactive.
-ATAC_HD int block_decision(float2 b, const InstDev& in) {   // 1 active, 0 inactive, -1 open
-    return b.x > in.blk_off ? 0 : (b.y < in.blk_on ? 1 : -1);
+// The table byte offset of a point: its grid coordinates by the shifted map
+// c' = A xi + (b + 1), truncated per axis (a negative or NaN coordinate gives
+// 0, the outside layer) and clamped to the padded grid.
+ATAC_HD uint32_t cls_offset(const GridDev& G, const InstDev& in, float px, float py, float pz) {
+    const double x = px, y = py, z = pz;
+    const double c0 = fma(in.A[2], z, fma(in.A[1], y, fma(in.A[0], x, in.bs[0])));
+    const double c1 = fma(in.A[5], z, fma(in.A[4], y, fma(in.A[3], x, in.bs[1])));
+    const double c2 = fma(in.A[8], z, fma(in.A[7], y, fma(in.A[6], x, in.bs[2])));
+    const uint32_t i = (uint32_t)med3_i32(cvt_sat_i32(c0), G.n[0] + 1);
+    const uint32_t j = (uint32_t)med3_i32(cvt_sat_i32(c1), G.n[1] + 1);
+    const uint32_t k = (uint32_t)med3_i32(cvt_sat_i32(c2), G.n[2] + 1);
+    return cls_entry(G, i, j, k);
 }
 
-
+// The detection decision detect_value(...) < 0, bit for bit, through the cell
+// classification (the interpolation only for open cells).
 ATAC_HD bool detect_active(const GridDev& G, const InstDev& in, float px, float py, float pz, double cc[3]) {
     grid_coords(in, px, py, pz, cc);
-    if (!G.fast) return detect_value(G, in, px, py, pz) < 0.0f;
-    const double c0 = cc[0], c1 = cc[1], c2 = cc[2];
-    const bool inside = c0 >= 0.0 && c0 <= in.cmax[0] && c1 >= 0.0 && c1 <= in.cmax[1] && c2 >= 0.0 && c2 <= in.cmax[2];
-    if (!inside) return false;
-    int i = (int)c0, j = (int)c1, k = (int)c2;
-    i = i < G.n[0] - 1 ? i : G.n[0] - 1;
-    j = j < G.n[1] - 1 ? j : G.n[1] - 1;
-    k = k < G.n[2] - 1 ? k : G.n[2] - 1;
-    const int dec = block_decision(ATAC_GLOBAL(G.blk)[blk_entry(G, i / kBlk, j / kBlk, k / kBlk)], in);
-    if (dec >= 0) return dec == 1;
+    if (!G.fast || !in.cls) return detect_value(G, in, px, py, pz) < 0.0f;
+    const uint8_t e = in.cls[cls_offset(G, in, px, py, pz)];
+    if (e != kClsOpen) return e == kClsActive;
     return detect_interp(G, in, px, py, pz, cc);
 }
 
@@ -559,65 +580,75 @@ __device__ __forceinline__ float2 load_f2_at(global_ptr<T> base, uint32_t byte_o
 }
 #endif
 
+// A pointer the wavefront holds in scalar registers.
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+__device__ __forceinline__ const T* uniform_ptr(const T* p) {
+    const uint64_t v = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v), hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (const T*)(((uint64_t)hi << 32) | lo);
+}
+#else
+template <typename T>
+ATAC_HD const T* uniform_ptr(const T* p) { return p; }
+#endif
+
+// Byte `off` of a classification table of `bytes` bytes: a buffer load with the
+// table as a scalar resource and a 32-bit lane offset (bytes past the end read 0).
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ uint32_t cls_load(const uint8_t* tab, uint32_t bytes, uint32_t off) {
+    const __amdgpu_buffer_rsrc_t r =
+        __builtin_amdgcn_make_buffer_rsrc((void*)uniform_ptr(tab), (short)0, (int)bytes, 0x00020000);
+    return __builtin_amdgcn_raw_buffer_load_b8(r, (int)off, 0, 0);
+}
+#else
+inline uint32_t cls_load(const uint8_t* tab, uint32_t bytes, uint32_t off) { return off < bytes ? tab[off] : 0u; }
+#endif
+
 // Stream pass of the detection of PPL points (detect_active without the cell
-// reads): once the points' block-bound offsets are computed, `before()` runs (the
-// caller issues reads that can share this step's wait), the block-bound reads are
+// reads): once the points' cell-bound offsets are computed, `before()` runs (the
+// caller issues reads that can share this step's wait), the cell-bound reads are
 // issued, then `issue()` (the caller's next loads go out behind them, so waiting
 // for the bounds does not wait for them), then the bounds are applied. Returns
 // the candidates (bit k): the points they settle as active and the ones they
 // leave open (also in `open`); candidate_active finishes both. Only what every
-// point needs is done here, without branches, so a wavefront's lanes stay busy:
-// a point outside the box (or not valid) reads the table's outside entry, which
-// settles it as inactive, and the per-candidate work runs later, densely, one
-// candidate per lane. With all_active every valid point is a settled candidate
-// and nothing is read; on a grid without the table every inside point is open.
+// point needs is done here, without branches or box tests, so a wavefront's lanes
+// stay busy: the padded table gives every point an entry (outside the box:
+// kBoundOutside, on a cell the box face cuts: kBoundOpen), and a point that is not
+// valid reads entry 0, which is outside. The per-candidate work runs later,
+// densely, one candidate per lane. With all_active every valid point is a
+// settled candidate and nothing is read; on a grid without the table every valid
+// point is open.
 template <int PPL, typename F0, typename F>
 ATAC_HD unsigned detect_candidates(const GridDev& G, const InstDev& in, const float (&x)[PPL], const float (&y)[PPL],
                                    const float (&z)[PPL], unsigned valid, bool all_active, unsigned& open,
                                    F0&& before, F&& issue) {
-    static_assert(kBlk == 2, "block index = cell index / 2");
-    const uint32_t outside = blk_outside(G) * 8u;
-    unsigned inside = 0;
     uint32_t boff[PPL];
 #pragma unroll
-    for (int k = 0; k < PPL; ++k) {
-        double c[3];
-        grid_coords(in, x[k], y[k], z[k], c);
-        bool ins = ((valid >> k) & 1u) != 0u;
-        uint32_t bi[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            ins = ins && c[a] >= 0.0 && c[a] <= in.cmax[a];
-            const int i = cvt_sat_i32(c[a]);   // inside: floor; else unused
-            bi[a] = (uint32_t)(i < G.n[a] - 1 ? i : G.n[a] - 1) >> 1;
-        }
-        if (ins) inside |= 1u << k;
-        boff[k] = ins ? blk_entry(G, bi[0], bi[1], bi[2]) * 8u : outside;
-    }
+    for (int k = 0; k < PPL; ++k) boff[k] = ((valid >> k) & 1u) ? cls_offset(G, in, x[k], y[k], z[k]) : 0u;
     sched_fence();   // the points are in: the reads below wait for nothing earlier
     before();
-    float2 bb[PPL];
-    const bool use_blk = G.fast && !all_active;
-    if (use_blk) {
+    uint32_t cl[PPL];
+    const bool use_tab = G.fast && in.cls && !all_active;
+    if (use_tab) {
+        const uint32_t bytes = cls_bytes(G);
 #pragma unroll
-        for (int k = 0; k < PPL; ++k) bb[k] = load_f2_at(ATAC_GLOBAL(G.blk), boff[k]);
+        for (int k = 0; k < PPL; ++k) cl[k] = cls_load(in.cls, bytes, boff[k]);
     }
     issue();
     if (all_active) {
         open = 0u;
         return valid;
     }
-    if (!use_blk) {
-        open = inside;
-        return inside;
+    if (!use_tab) {
+        open = valid;
+        return valid;
     }
-    const float off = in.blk_off, on = in.blk_on;
     unsigned act = 0, opn = 0;
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
-        const bool a_k = bb[k].y < on, o_k = !(bb[k].x > off) && !a_k;   // block_decision 1 / -1
-        act |= (a_k ? 1u : 0u) << k;
-        opn |= (o_k ? 1u : 0u) << k;
+        act |= (cl[k] == kClsActive ? 1u : 0u) << k;
+        opn |= (cl[k] == kClsOpen ? 1u : 0u) << k;
     }
     open = opn;
     return act | opn;
@@ -632,7 +663,7 @@ ATAC_HD unsigned detect_candidates(const GridDev& G, const InstDev& in, const fl
 struct CandRead {
     float4 lo, hi;
     float f[3];
-    bool open, off_cell;
+    bool open, off_cell, inside;
 };
 
 // candidate_read without the cell read: returns the byte offset of the open
@@ -643,20 +674,24 @@ ATAC_HD uint32_t candidate_locate(const GridDev& G, const InstDev& in, float x, 
     // truncation per axis serves the cell (clamped at the last node, as
     // detect_value clamps), its fractions and fast_cell_c's test: the fraction in
     // the clamped cell is fast_cell_c's whenever that test can pass (i <= n - 2).
+    // An open candidate may lie on a cell the box face cuts: the exact box test
+    // decides it (c >= 0 holds: its cell index is not negative).
     double c[3];
     grid_coords(in, x, y, z, c);
-    bool ok = true;
+    bool ok = true, inside = true;
     uint32_t idx[3];
     for (int a = 0; a < 3; ++a) {
         const int i = cvt_sat_i32(c[a]);
         const int ic = i < G.n[a] - 1 ? i : G.n[a] - 1;
         const double fr = c[a] - (double)ic, m = G.fd_m[a];
         ok = ok && c[a] >= 0.0 && i <= G.n[a] - 2 && fr >= m && fr <= G.fd_1m[a] && c[a] + m <= in.cmax[a];
+        inside = inside && c[a] >= 0.0 && c[a] <= in.cmax[a];
         idx[a] = (uint32_t)ic;
         r.f[a] = (float)fr;
     }
     r.off_cell = !ok;
     r.open = open;
+    r.inside = inside;
     return open ? (idx[0] + (uint32_t)G.n[0] * mad24(idx[2], (uint32_t)G.n[1], idx[1])) * 32u : 0u;
 }
 
@@ -669,7 +704,7 @@ ATAC_HD void candidate_read(const GridDev& G, const InstDev& in, float x, float 
 }
 
 ATAC_HD bool candidate_finish(const CandRead& r, float cth, bool& slow) {
-    const bool act = r.open ? cell_value(r.lo, r.hi, r.f[0], r.f[1], r.f[2], cth) < 0.0f : true;
+    const bool act = r.open ? r.inside && cell_value(r.lo, r.hi, r.f[0], r.f[1], r.f[2], cth) < 0.0f : true;
     slow = act && r.off_cell;
     return act;
 }

@@ -60,6 +60,11 @@ typedef struct {
     int32_t order;         /* work queues: 0 one per XCD, items grouped by grid (default);
                               1 one queue in instance order; 2 one queue grouped by grid */
     int32_t points_per_lane; /* points a detection lane takes per step (64 apart): 1, 2, 4 (0: 4) */
+    int32_t max_active;    /* bound on the active contacts of one evaluation (the record pool, 48 B each;
+                              96 B with ATAC_EVAL_PRECISE): 0 = one per point up to 4M points, a quarter of
+                              the points beyond; > 0 that many; < 0 one per point (needed by
+                              ATAC_EVAL_ALL_ACTIVE). An evaluation with more contacts gives NaN outputs and
+                              n_active -1, and atac_batch_workspace reports the overflow */
 } atac_tuning;
 
 /* flags of atac_batch_eval / atac_eval_plain */
@@ -124,7 +129,10 @@ int atac_points_permutation(atac_points p, int32_t segment, int64_t* sorted_to_o
 int atac_batch_create(const atac_grid* grids, int32_t n_grids, const atac_points* points, int32_t n_point_sets,
                       const atac_instance* inst, int32_t n_inst, const atac_tuning* tuning, atac_batch* out);
 int atac_batch_destroy(atac_batch b);
-/* New poses / parameters for the same (grid, points, segment) structure. */
+/* New poses / parameters for the same (grid, points, segment) structure. The
+ * adjoint records of the last evaluation are dropped (atac_batch_vjp then returns
+ * ATAC_ERR_STATE until the next evaluation). stream NULL: a synchronous upload
+ * ordered after the work already queued on the batch's own stream. */
 int atac_batch_set_instances(atac_batch b, const atac_instance* inst, int32_t n_inst, void* stream);
 /* Re-plans tiles and launch shape (used by the autotuner). */
 int atac_batch_retune(atac_batch b, const atac_tuning* tuning);
@@ -143,10 +151,18 @@ int atac_batch_eval(atac_batch b, uint32_t flags, const atac_outputs* out, void*
 int atac_batch_vjp(atac_batch b, const double* g_fm, const atac_gradients* out, void* stream);
 
 /* Captures [reset, detect, slow, contact, reduce] (and the adjoint when g_fm != NULL)
- * into a HIP graph bound to these outputs; atac_batch_graph_launch replays it. */
+ * into a HIP graph bound to these outputs; atac_batch_graph_launch replays it.
+ * Capturing runs nothing: atac_batch_vjp returns ATAC_ERR_STATE until the graph
+ * (or an eager evaluation) has been launched. */
 int atac_batch_graph_capture(atac_batch b, uint32_t flags, const atac_outputs* out, const double* g_fm,
                              const atac_gradients* grads);
 int atac_batch_graph_launch(atac_batch b, void* stream);
+
+/* The record pool of the batch and the last evaluation's use of it (synchronous):
+ * capacity (records), used (contacts the last evaluation found; may exceed the
+ * capacity) and overflowed (1: its outputs are NaN, its adjoint refused; plan the
+ * batch again with a larger max_active). Any output may be NULL. */
+int atac_batch_workspace(atac_batch b, int64_t* capacity, int64_t* used, int32_t* overflowed);
 
 /* Per-stage kernel timing with HIP events recorded on the launch stream around
  * each kernel of atac_batch_eval / atac_batch_vjp (not inside captured graphs).
@@ -158,6 +174,14 @@ int atac_batch_timing(atac_batch b, int32_t enable, double* ms_sum, int32_t* lau
 /* Active contacts of one instance from the last evaluation (host out): original
  * point indices (within the instance's segment), ascending. Synchronous. */
 int atac_batch_active_ids(atac_batch b, int32_t inst, int64_t* ids, int64_t capacity, int64_t* count);
+
+/* The plain path with the outputs of an evaluation followed by its adjoint: the
+ * wrench and n_active (out; Km, Dm, dfm_dp6 are not used) and, for the cotangent
+ * g_fm (device, 12 per instance), the gradients of atac_batch_vjp, with one launch
+ * per instance, one lane per point and FP64 atomics. The per-instance baseline of
+ * the batched evaluation + adjoint. */
+int atac_eval_plain_vjp(atac_batch b, uint32_t flags, const double* g_fm, const atac_outputs* out,
+                        const atac_gradients* grads, void* stream);
 
 /* Distinct grid values read by the detection stencils of all instances (summed
  * over grids). Synchronous diagnostic used for the algorithmic byte count. */

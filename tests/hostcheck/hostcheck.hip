@@ -15,9 +15,16 @@ using namespace atac;
 // same per-cell / per-block functions the set-up kernels run.
 struct HostGrid {
     std::vector<float4> cells;
-    std::vector<float2> blk;
+    std::vector<uint8_t> cls;   // the classification for the instance's col_th
     GridDev G;
 };
+
+// the instance's classification table (what the batch builds on the device)
+static void host_cls(HostGrid& hg, InstDev& in) {
+    hg.cls.resize(cls_bytes(hg.G));
+    for (uint32_t e = 0; e < (uint32_t)hg.cls.size(); ++e) hg.cls[e] = cls_fill(hg.G, in.col_th, e);
+    in.cls = hg.cls.data();
+}
 
 static void host_grid(const float* vals, const int32_t n[3], const double minb[3], const double maxb[3],
                       const double d[3], HostGrid& hg) {
@@ -30,7 +37,6 @@ static void host_grid(const float* vals, const int32_t n[3], const double minb[3
         G.maxb[a] = maxb[a];
         G.d[a] = d[a];
         G.inv_d[a] = 1.0 / d[a];
-        G.nb[a] = (n[a] - 1) / kBlk + 1;
     }
     set_fd_reach(G);
     {
@@ -38,11 +44,7 @@ static void host_grid(const float* vals, const int32_t n[3], const double minb[3
         hg.cells.resize(2 * ci * cj * ck);
         for (int64_t c = 0; c < ci * cj * ck; ++c)
             pack_cell(G, (int)(c % ci), (int)((c / ci) % cj), (int)(c / (ci * cj)), &hg.cells[2 * c]);
-        const int64_t ne = blk_outside(G);
-        hg.blk.resize(ne + 1);
-        for (int64_t e = 0; e <= ne; ++e) hg.blk[e] = blk_fill(G, (uint32_t)e);
         G.cells = hg.cells.data();
-        G.blk = hg.blk.data();
         G.fast = 1;
     }
 }
@@ -140,6 +142,7 @@ int hc_eval(const float* vals, const int32_t n[3], const double minb[3], const d
     InstDev in;
     std::memset(&in, 0, sizeof in);
     compose_instance(*inst, G, px, py, pz, npts, in);
+    host_cls(hg, in);
     if (flags & ATAC_EVAL_PRECISE)
         eval_t<double>(G, in, flags, fm, p6, Km, Dm, n_active, active, slow, g_fm, vjp);
     else
@@ -155,6 +158,7 @@ int hc_detect(const float* vals, const int32_t n[3], const double minb[3], const
     InstDev in;
     std::memset(&in, 0, sizeof in);
     compose_instance(*inst, hg.G, px, py, pz, npts, in);
+    host_cls(hg, in);
     for (int64_t o = 0; o < npts; ++o) out[o] = detect_value(hg.G, in, px[o], py[o], pz[o]);
     return 0;
 }
@@ -172,6 +176,7 @@ int hc_detect_decisions(const float* vals, const int32_t n[3], const double minb
     InstDev in;
     std::memset(&in, 0, sizeof in);
     compose_instance(*inst, G, px, py, pz, npts, in);
+    host_cls(hg, in);
     // the detection kernel's passes: 4 points at a time through the block bounds
     // (detect_candidates), then each candidate finished (candidate_active),
     // ragged tail included
@@ -204,14 +209,7 @@ int hc_detect_decisions(const float* vals, const int32_t n[3], const double minb
         if (!G.fast) continue;
         double cc[3];
         grid_coords(in, px[o], py[o], pz[o], cc);
-        if (!(cc[0] >= 0.0 && cc[0] <= in.cmax[0] && cc[1] >= 0.0 && cc[1] <= in.cmax[1] && cc[2] >= 0.0 &&
-              cc[2] <= in.cmax[2]))
-            continue;
-        int i = (int)cc[0], j = (int)cc[1], k = (int)cc[2];
-        i = i < G.n[0] - 1 ? i : G.n[0] - 1;
-        j = j < G.n[1] - 1 ? j : G.n[1] - 1;
-        k = k < G.n[2] - 1 ? k : G.n[2] - 1;
-        settled[o] = block_decision(G.blk[blk_entry(G, i / kBlk, j / kBlk, k / kBlk)], in) >= 0 ? 1 : 0;
+        settled[o] = in.cls[cls_offset(G, in, px[o], py[o], pz[o])] != kClsOpen ? 1 : 0;
     }
     return 0;
 }

@@ -214,3 +214,24 @@ def test_autograd_function(setup):
         sd = np.linalg.norm(g[i]) * max(r["scale_Dm"], 1e-30)
         assert rel_err(np.concatenate([gp1[i], gp2[i]]), Dc, sd) <= TOL
     assert torch.isfinite(kn.grad).all() and torch.isfinite(kt.grad).all()
+
+
+def test_plain_path_adjoint_matches_batched(setup):
+    """The plain per-instance baseline (one launch per force object, FP64 atomics)
+    gives the batched path's wrench and adjoint (same contact arithmetic, another
+    summation order)."""
+    w, ref, b = setup
+    g = torch.tensor(np.random.default_rng(4).normal(size=(b.n, 12)), device="cuda")
+    out = b.eval()
+    grads = b.vjp(g)
+    pout, pgrads = b.eval_plain_vjp(g)
+    torch.cuda.synchronize()
+    assert torch.equal(out["n_active"], pout["n_active"])
+    gn = g.cpu().numpy()
+    for i, r in enumerate(ref):
+        if not r["n_active"]:
+            continue
+        assert rel_err(pout["fm"][i].cpu().numpy(), out["fm"][i].cpu().numpy(), r["scale_fm"]) <= 1e-6
+        s = np.linalg.norm(gn[i]) * r["scale_Km"]
+        for k in ("g_q1", "g_q2"):
+            assert rel_err(pgrads[k][i].cpu().numpy(), grads[k][i].cpu().numpy(), s) <= 1e-6, k

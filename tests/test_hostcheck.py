@@ -192,3 +192,30 @@ def test_bounded_detection_adversarial():
     for vals in cases:
         g = W.SDFGrid(vals, np.array([lo] * 3), np.array([hi] * 3), np.array([dxyz] * 3))
         _decisions_agree(g, pts, ins)
+
+
+def test_bounded_detection_box_inside_and_beyond_grid():
+    """max_box below the grid's last node (the box face cuts interior cells, whose
+    table entries defer to the exact test) and beyond it (points past the last node
+    are inside and take the clamped last cell): the table's decisions equal the
+    full interpolation's, for points in and around the box."""
+    rng = np.random.default_rng(9)
+    n, lo = 11, -1.0
+    dx = 0.2
+    col_th = 0.01
+    zs = np.zeros(6)
+    ins = make_instance(0, 0, 0, np.eye(4), zs, np.eye(4), zs, col_th, 1e4, 0.0, 0.0, 0.0)
+    xs = lo + dx * np.arange(n)
+    X, Y, Z = np.meshgrid(xs, xs, xs, indexing="ij")
+    field = (0.35 * X + 0.2 * Y - 0.3 * Z + 0.05 + 0.02 * rng.normal(size=X.shape)).transpose(2, 1, 0)
+    vals = np.ascontiguousarray(field, dtype=np.float32)
+    for hi in (lo + 6.3 * dx, lo + (n - 1) * dx, lo + (n - 1) * dx + 0.37):
+        p = rng.uniform(lo - 0.3, hi + 0.3, (6000, 3))
+        on_face = rng.uniform(lo, hi, (600, 3))
+        on_face[:300, 0] = hi
+        on_face[300:, 2] = lo
+        cell_faces = lo + dx * rng.integers(0, n, (400, 3)) + rng.uniform(0, 1, (400, 3)) * np.array([0, 1, 1]) * dx
+        pts = np.concatenate([p, on_face, cell_faces]).astype(np.float32)
+        g = W.SDFGrid(vals, np.array([lo] * 3), np.array([hi] * 3), np.array([dx] * 3))
+        settled = _decisions_agree(g, pts, ins)
+        assert 0.0 < settled.mean() < 1.0

@@ -40,8 +40,9 @@ def test_library_reports_version_and_errors_without_gpu():
 
 
 def test_grid_size_limits_refused_before_allocation():
-    """Grids beyond the detection's 32-bit cell offsets (2^27 nodes) or its 24-bit
-    block-table products are refused before any value is read or allocated."""
+    """Grids beyond the detection's 32-bit cell offsets (2^27 nodes; the padded
+    cell-bound table then stays below 2^30 entries) are refused before any value
+    is read or allocated."""
     from atac import _lib
     L = _lib.lib()
     one = (ctypes.c_float * 1)(0.0)
@@ -49,7 +50,7 @@ def test_grid_size_limits_refused_before_allocation():
     box1 = (ctypes.c_double * 3)(1.0, 1.0, 1.0)
     d = (ctypes.c_double * 3)(1e-3, 1e-3, 1e-3)
     out = ctypes.c_void_p()
-    for dims, msg in (((1024, 1024, 128), b"2^27"), ((1, 1 << 24, 1), b"2^24"), ((1, 4096, 16384), b"2^24")):
+    for dims, msg in (((1024, 1024, 128), b"2^27"), ((1, 1 << 27, 1), b"2^27"), ((2, 4096, 16384), b"2^27")):
         rc = L.atac_grid_create(one, *dims, box0, box1, d, ctypes.byref(out))
         assert rc == _lib.ATAC_ERR_INVALID, dims
         assert msg in L.atac_last_error(), (dims, L.atac_last_error())
