@@ -245,6 +245,29 @@ def place_by_bisection(grid: SDFGrid, pts, R, u, target_pen, s_hi=4.0, iters=40)
     return hi * u
 
 
+def place_by_bisection_many(grid: SDFGrid, pts, Rs, us, targets, s_hi=4.0, iters=40):
+    """place_by_bisection for many (R, u, target) at once: the same bisection per
+    pose, run in lockstep over one trilinear lookup per step (the same values)."""
+    m = len(Rs)
+    X = np.stack([pts @ R.T for R in Rs])                       # (m, n, 3), as place_by_bisection forms them
+    us = np.asarray(us, dtype=np.float64).reshape(m, 3)
+    targets = np.asarray(targets, dtype=np.float64)
+
+    def pen(s):
+        x = X + s[:, None, None] * us[:, None, :]
+        return np.max(-trilinear_np(grid, x.reshape(-1, 3)).reshape(m, -1), axis=1)
+
+    lo, hi = np.zeros(m), np.full(m, s_hi)
+    done = pen(hi) > targets                                    # already past the target at s_hi
+    for _ in range(iters):
+        mid = 0.5 * (lo + hi)
+        over = pen(mid) > targets
+        lo = np.where(over, mid, lo)
+        hi = np.where(over, hi, mid)
+    hi = np.where(done, s_hi, hi)
+    return hi[:, None] * us
+
+
 # ------------------------------------------------------------------ workloads
 @dataclass
 class Instance:
@@ -291,7 +314,9 @@ def peg_in_hole(seed=1001, n_points=5000, n_poses=256, kt=KT) -> Workload:
                     {"seed": seed, "kn": KN, "kt": kt, "mu": MU, "damping": DAMPING, "col_th": COL_TH})
 
 
-def blobs(seed=1002, n_pairs=10, n_points=200_000, n_poses=64, grid_n=128, place_sample=4000) -> Workload:
+def blobs(seed=1002, n_pairs=10, n_points=200_000, n_poses=64, grid_n=128, place_sample=4000,
+          progress=None) -> Workload:
+    """progress: optional callable, called with the pair index once a pair is placed."""
     rng = np.random.default_rng(seed)
     grids, psets, inst = [], [], []
     for p in range(n_pairs):
@@ -303,13 +328,18 @@ def blobs(seed=1002, n_pairs=10, n_points=200_000, n_poses=64, grid_n=128, place
         psets.append(pts)
         diag = float(np.linalg.norm(g.max_box - g.min_box))
         sub = pts[rng.choice(len(pts), min(place_sample, len(pts)), replace=False)]
-        for _ in range(n_poses):
-            R = random_rotation(rng)
+        Rs, us, targets, phis = [], [], [], []
+        for _ in range(n_poses):   # the random draws in their order; the placement uses none
+            Rs.append(random_rotation(rng))
             u = rng.normal(size=3)
-            u /= np.linalg.norm(u)
-            target = rng.uniform(0.0, 0.03) * diag
-            t = place_by_bisection(g, sub, R, u, target)
-            inst.append(Instance(p, p, pose(R, t), rng.normal(0, 0.1, 6), np.eye(4), np.zeros(6)))
+            us.append(u / np.linalg.norm(u))
+            targets.append(rng.uniform(0.0, 0.03) * diag)
+            phis.append(rng.normal(0, 0.1, 6))
+        ts = place_by_bisection_many(g, sub, Rs, us, targets)
+        for R, t, phi in zip(Rs, ts, phis):
+            inst.append(Instance(p, p, pose(R, t), phi, np.eye(4), np.zeros(6)))
+        if progress is not None:
+            progress(p)
     return Workload(f"blobs_{grid_n}^3_{n_points}_x{n_pairs * n_poses}", grids, psets, inst, {"seed": seed})
 
 

@@ -9,6 +9,7 @@
 #include "Body/Body.h"
 #include "Body/BodyBVHObj.h"
 #include "Body/BodySDFObj.h"
+#include "Body/ContactPoints.h"
 #include "Simulation/BatchedBDF1.h"
 #include "Force/ForceGeneralBVHContact.h"
 #include "Force/ForceGeneralSDFContact.h"
@@ -86,6 +87,29 @@ sdfgen::Array3<dtype> grid_of(const arr& values) {
 
 PYBIND11_MODULE(_redmax, m) {
     m.doc() = "GPU-backed SDF and BVH contact forces with the reference simulator's C++ interface";
+
+    m.def(
+        "precompute_contact_points",
+        [](const arr& V, py::array_t<int64_t, py::array::c_style | py::array::forcecast> F, bool adaptive_sample) {
+            const std::vector<Vector3> v = points_of(V);
+            if (F.ndim() != 2 || F.shape(1) != 3) throw std::invalid_argument("faces must have shape (m, 3)");
+            std::vector<std::array<int, 3>> f((size_t)F.shape(0));
+            auto r = F.unchecked<2>();
+            for (py::ssize_t t = 0; t < F.shape(0); ++t)
+                for (int k = 0; k < 3; ++k) {
+                    if (r(t, k) < 0 || r(t, k) >= (int64_t)v.size()) throw std::invalid_argument("face index out of range");
+                    f[(size_t)t][(size_t)k] = (int)r(t, k);
+                }
+            int rate = 1;
+            const std::vector<Vector3> p = precompute_contact_points(v, f, adaptive_sample, &rate);
+            py::array_t<double> out({(py::ssize_t)p.size(), (py::ssize_t)3});
+            auto w = out.mutable_unchecked<2>();
+            for (size_t i = 0; i < p.size(); ++i)
+                for (int k = 0; k < 3; ++k) w((py::ssize_t)i, k) = p[i](k);
+            return py::make_tuple(out, rate);
+        },
+        py::arg("V"), py::arg("F"), py::arg("adaptive_sample") = false,
+        "BodyMeshObj::precompute_contact_points (BodyMeshObj.cpp:184-238): (points (n, 3), sample_rate).");
 
     py::class_<Body>(m, "Body")
         .def(py::init([](std::string name, const arr& pts, int index0) {

@@ -3,7 +3,7 @@
 // maximal-coordinate fm, Km, Dm and reduced-coordinate fr, Kr, Dr) and the per-force
 // timing report.
 #pragma once
-#include <iomanip>
+#include <cstdio>
 #include <iostream>
 #include <map>
 #include <string>
@@ -18,24 +18,24 @@ class Force {
 public:
     Simulation* _sim;
 
+    // Accumulated durations per named phase, in clock() ticks (microseconds),
+    // printed as one table row per phase in milliseconds.
     class Timer {
-        std::map<std::string, long long> _time;
+        std::map<std::string, long long> totals_;
 
     public:
-        void add(std::string name, long long duration) {
-            auto it = _time.find(name);
-            if (it == _time.end()) _time[name] = duration;
-            else it->second += duration;
-        }
+        void add(std::string name, long long duration) { totals_[name] += duration; }
         void print_time_report() {
-            for (auto it = _time.begin(); it != _time.end(); it++)
-                std::cerr << "|" << std::setw(20) << std::left << it->first << "|" << std::setw(7) << std::right
-                          << it->second / 1000 << "(ms) |" << std::endl;
+            for (const auto& kv : totals_) {
+                char row[96];
+                std::snprintf(row, sizeof row, "|%-20s|%7lld(ms) |", kv.first.c_str(), kv.second / 1000);
+                std::cerr << row << std::endl;
+            }
         }
-        void clear() { _time.clear(); }
+        void clear() { totals_.clear(); }
         long long get(const std::string& name) const {
-            auto it = _time.find(name);
-            return it == _time.end() ? 0 : it->second;
+            const auto it = totals_.find(name);
+            return it == totals_.end() ? 0 : it->second;
         }
     } _time;
 

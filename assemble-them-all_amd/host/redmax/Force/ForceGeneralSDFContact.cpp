@@ -102,6 +102,7 @@ void ForceGeneralSDFContact::evaluate(uint32_t flags, double* fm, double* Km, do
         atac_instance in;
         fill_instance(&in);
         check(atac_batch_create(&grid, 1, &D.points, 1, &in, 1, nullptr, &D.batch), "batch");
+        check(atac_batch_timing(D.batch, 1, nullptr, nullptr), "timing");   // stage times for _time
     }
     atac_instance in;
     fill_instance(&in);
@@ -109,6 +110,10 @@ void ForceGeneralSDFContact::evaluate(uint32_t flags, double* fm, double* Km, do
     int32_t n_active = 0;
     check(atac_batch_eval_host(D.batch, flags, fm, Km, Dm, p6, &n_active), "eval");
     _last_active = n_active;
+    double ms[6];
+    int32_t launches[6];
+    check(atac_batch_timing(D.batch, 1, ms, launches), "timing");
+    _last_detect_us = (long long)((ms[0] + ms[1]) * 1e3);   // detection and the slow-path samples
     if (n_active > 0) {   // CollisionDetection.cpp:143-146
         _contact_body->add_contact_body(_SDF_body->_name);
         _SDF_body->add_contact_body(_contact_body->_name);
@@ -166,8 +171,8 @@ void ForceGeneralSDFContact::computeForce(VectorX& fm, VectorX& fr, bool verbose
     if (verbose)
         std::cerr << "General SDF Contact (GPU): " << _last_active << " contacts, contact body = " << _contact_body->_name
                   << ", SDF body = " << _SDF_body->_name << std::endl;
-    _time.add("collision detection", 0);
-    _time.add("compute force", t1 - t0);
+    _time.add("collision detection", _last_detect_us);
+    _time.add("compute force", t1 - t0 - _last_detect_us);
 }
 
 void ForceGeneralSDFContact::computeForceWithDerivative(VectorX& fm, VectorX& fr, MatrixX& Km, MatrixX& Dm, MatrixX& Kr,
@@ -181,8 +186,8 @@ void ForceGeneralSDFContact::computeForceWithDerivative(VectorX& fm, VectorX& fr
     const long long t1 = now_us();
     if (verbose)
         std::cerr << "General SDF Contact (GPU): " << _last_active << " contacts" << std::endl;
-    _time.add("collision detection", 0);
-    _time.add("compute force", t1 - t0);
+    _time.add("collision detection", _last_detect_us);
+    _time.add("compute force", t1 - t0 - _last_detect_us);
 }
 
 void ForceGeneralSDFContact::computeForceWithDerivative(VectorX& fm, VectorX& fr, MatrixX& Km, MatrixX& Dm, MatrixX& Kr,
@@ -201,8 +206,8 @@ void ForceGeneralSDFContact::computeForceWithDerivative(VectorX& fm, VectorX& fr
         }
     const long long t1 = now_us();
     if (verbose) std::cerr << "General SDF Contact (GPU): " << _last_active << " contacts" << std::endl;
-    _time.add("collision detection", 0);
-    _time.add("compute force", t1 - t0);
+    _time.add("collision detection", _last_detect_us);
+    _time.add("compute force", t1 - t0 - _last_detect_us);
 }
 
 void ForceGeneralSDFContact::computeForce(std::vector<Contact>& contacts, VectorX& fm, bool verbose) {

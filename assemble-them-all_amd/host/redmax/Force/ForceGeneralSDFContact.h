@@ -63,6 +63,7 @@ private:
     struct Device;
     std::unique_ptr<Device> _dev;
     int _last_active = 0;
+    long long _last_detect_us = 0;   // GPU time of the last evaluation's detection (HIP events)
     void evaluate(uint32_t flags, double* fm, double* Km, double* Dm, double* p6);
     void evaluate_list(const std::vector<Contact>& contacts, uint32_t flags, double* fm, double* Km, double* Dm,
                        double* p6);

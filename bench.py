@@ -66,52 +66,98 @@ def traffic_from_profiles(kernel="detect_kernel"):
     return d.get("hbm_bytes_per_launch")
 
 
-def cpu_baseline(w, gpu_fm, budget_s, threads):
+def host_cpu():
+    """The host's CPU model (lscpu) and the cores this process may use."""
+    model = None
+    try:
+        import subprocess
+        for ln in subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout.splitlines():
+            if ln.startswith("Model name:"):
+                model = ln.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return {"model": model, "nproc": len(os.sched_getaffinity(0))}
+
+
+def cpu_baseline(w, gpu, g, budget_s, threads, budget_1t_s):
     """FP64 oracle (reference-mode restatement, computeForceWithDerivative per force
-    object) on the first instances of the workload, until about budget_s of CPU time."""
+    object: the reference's way to the gradient is Km/Dm, then the contraction) on the
+    first instances of the workload, until about budget_s of CPU time on `threads`
+    threads and budget_1t_s on one thread. Errors of the GPU step's outputs against
+    it on every instance evaluated: the wrench and the adjoint (g.Km, g.Dm), each
+    relative to the per-contact scale (the sum over contacts of each contact's
+    contribution norm) and the wrench also to its own norm."""
     from oracle import oracle as O
     from atac import workloads as W
-    grids = [O.GridRef.from_grid(g) for g in w.grids]
-    keep = []
+    grids = [O.GridRef.from_grid(g_) for g_ in w.grids]
+
+    def oracle_insts(lo, hi):
+        out = []
+        for ins in w.instances[lo:hi]:
+            pts = W.round_points_f32(w.point_sets[ins.points])
+            out.append(O.make_instance(ins.grid, pts, ins.E1, ins.phi1, ins.E2, ins.phi2, ins.col_th, ins.kn, ins.kt,
+                                       ins.mu, ins.damping, ins.scale, ins.contact_scale))
+        return out
+
     t_total, q_total, n_done = 0.0, 0, 0
-    errs_sum, errs_net = [], []
+    errs = {"fm_sum": [], "fm_net": [], "vjp_q": [], "vjp_phi": []}
+    band = 0
     chunk = 32
     while n_done < len(w.instances) and t_total < budget_s:
-        insts = w.instances[n_done:n_done + chunk]
-        oi = []
-        for ins in insts:
-            pts = W.round_points_f32(w.point_sets[ins.points])
-            keep.append(pts)
-            oi.append(O.make_instance(ins.grid, pts, ins.E1, ins.phi1, ins.E2, ins.phi2, ins.col_th, ins.kn, ins.kt,
-                                      ins.mu, ins.damping, ins.scale, ins.contact_scale))
+        oi = oracle_insts(n_done, n_done + chunk)
         t0 = time.perf_counter()
         r = O.eval_batch(grids, oi, O.FLAG_DERIV, threads=threads)
         t_total += time.perf_counter() - t0
-        q_total += sum(len(w.point_sets[i.points]) for i in insts)
-        for k in range(len(insts)):
+        for k in range(len(oi)):
+            i = n_done + k
+            q_total += oi[k].n_points
+            if int(gpu["n_active"][i]) != int(r["n_active"][k]):
+                band += 1          # the active sets differ within the |d| < 1e-6 band: not compared
+                continue
+            if not r["n_active"][k]:
+                continue
             ref = r["fm"][k]
-            d = np.linalg.norm(gpu_fm[n_done + k] - ref)
-            nref = np.linalg.norm(ref)
-            if nref > 0:
-                errs_net.append(d / nref)
-        n_done += len(insts)
-    # per-contact-normalised error on a few instances (needs the per-contact scale)
-    for k in range(min(n_done, 64)):
-        ins = w.instances[k]
-        pts = W.round_points_f32(w.point_sets[ins.points])
-        res = O.eval_instance(grids, O.make_instance(ins.grid, pts, ins.E1, ins.phi1, ins.E2, ins.phi2, ins.col_th,
-                                                     ins.kn, ins.kt, ins.mu, ins.damping))
-        if res["n_active"]:
-            errs_sum.append(np.linalg.norm(gpu_fm[k] - res["fm"]) / res["scale_fm"])
+            errs["fm_sum"].append(np.linalg.norm(gpu["fm"][i] - ref) / r["scale_fm"][k])
+            if np.linalg.norm(ref) > 0:
+                errs["fm_net"].append(np.linalg.norm(gpu["fm"][i] - ref) / np.linalg.norm(ref))
+            gn = np.linalg.norm(g[i])
+            gq = np.concatenate([gpu["g_q1"][i], gpu["g_q2"][i]])
+            gp = np.concatenate([gpu["g_phi1"][i], gpu["g_phi2"][i]])
+            errs["vjp_q"].append(np.linalg.norm(gq - g[i] @ r["Km"][k]) / (gn * r["scale_Km"][k]))
+            errs["vjp_phi"].append(np.linalg.norm(gp - g[i] @ r["Dm"][k]) / (gn * max(r["scale_Dm"][k], 1e-300)))
+        n_done += len(oi)
+    # one thread, the reference's own setting (the planners pin OMP_NUM_THREADS=1)
+    t1, q1, n1 = 0.0, 0, 0
+    while n1 < len(w.instances) and t1 < budget_1t_s:
+        oi = oracle_insts(n1, n1 + 4)
+        t0 = time.perf_counter()
+        O.eval_batch(grids, oi, O.FLAG_DERIV, threads=1)
+        t1 += time.perf_counter() - t0
+        q1 += sum(x.n_points for x in oi)
+        n1 += len(oi)
+    mx = lambda e: float(max(e)) if e else None
+    dist = lambda e: ({"max": float(np.max(e)), "p99_9": float(np.percentile(e, 99.9)),
+                       "median": float(np.median(e)), "above_1e-5": int(np.sum(np.asarray(e) > 1e-5))} if e else None)
     return {
         "value": q_total / t_total, "unit": "queries/s", "cores": threads, "kind": "port",
         "sample": f"first {n_done} of {len(w.instances)} force objects ({q_total} queries), FP64 oracle, "
                   f"detection + computeForceWithDerivative (FD stencils), OpenMP over force objects",
+        "one_thread": {"value": q1 / t1, "unit": "queries/s", "cores": 1,
+                       "sample": f"first {n1} force objects ({q1} queries)"},
+        "host": host_cpu(),
     }, {
-        "max_rel_err_vs_sum_of_contact_norms": float(max(errs_sum)) if errs_sum else None,
-        "max_rel_err_vs_net": float(max(errs_net)) if errs_net else None,
-        "median_rel_err_vs_net": float(np.median(errs_net)) if errs_net else None,
-        "instances_checked": n_done,
+        "max_rel_err_wrench_vs_sum_of_contact_norms": mx(errs["fm_sum"]),
+        "max_rel_err_wrench_vs_net": mx(errs["fm_net"]),
+        "median_rel_err_wrench_vs_net": float(np.median(errs["fm_net"])) if errs["fm_net"] else None,
+        "max_rel_err_vjp_pose_vs_g_Km": mx(errs["vjp_q"]),
+        "max_rel_err_vjp_twist_vs_g_Dm": mx(errs["vjp_phi"]),
+        # per-instance distributions (the Jacobian jumps at the friction branch switch: an
+        # instance with a contact within FP32 rounding of it can differ there while the
+        # wrench agrees)
+        "wrench_vs_sum_of_contact_norms": dist(errs["fm_sum"]),
+        "vjp_pose_vs_g_Km": dist(errs["vjp_q"]),
+        "vjp_twist_vs_g_Dm": dist(errs["vjp_phi"]),
+        "instances_checked": n_done - band, "instances_with_band_points_skipped": band,
     }
 
 
@@ -350,29 +396,36 @@ def main():
     copy_gbs = 2 * src.numel() * 4 / (c0.elapsed_time(c1) / 10 * 1e-3) / 1e9
     del src, dst
     log(f"device copy: {copy_gbs:.0f} GB/s")
+    # the whole step's algorithmic bytes: every point read once, every touched grid
+    # value once, per instance its poses, twists and parameters (408 B) and g_fm (96 B)
+    # read and its wrench (96 B) and gradients (208 B) written, and the adjoint's
+    # checkpoint of d and grad phi (16 B) per active contact
+    step_alg = 12 * n_q + 4 * touched + (408 + 96 + 96 + 208) * b.n + 16 * n_active
+    step_gbs = step_alg / (ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
                 "kernel": "detect_kernel", "algorithmic_bytes_per_launch": int(det_bytes),
                 "touched_grid_values": touched, "launch_ms": round(det_ms, 4),
-                "measured_copy_GBps": round(copy_gbs, 1), "frac_of_measured_copy": round(achieved / copy_gbs, 4)}
+                "measured_copy_GBps": round(copy_gbs, 1), "frac_of_measured_copy": round(achieved / copy_gbs, 4),
+                "step_algorithmic_bytes": int(step_alg), "step_achieved": round(step_gbs, 1),
+                "step_frac": round(step_gbs / HBM_PEAK_GBS, 4)}
 
     plain = None
     if not args.no_plain:
-        po = b.alloc_outputs(deriv=True)
-        b.eval_plain(deriv=True, out=po)
+        po, pg = b.eval_plain_vjp(g)
         torch.cuda.synchronize()
         reps = 3
         s2 = torch.cuda.Event(enable_timing=True)
         e2 = torch.cuda.Event(enable_timing=True)
         s2.record()
         for _ in range(reps):
-            b.eval_plain(deriv=True, out=po)
+            b.eval_plain_vjp(g, out=po, grads=pg)
         e2.record()
         torch.cuda.synchronize()
         pms = s2.elapsed_time(e2) / reps
         plain = {"ms_per_step": round(pms, 3), "queries_per_s": n_q / pms * 1e3,
-                 "what": "one launch per force object, one lane per point, inline contact math with Km/Dm "
-                         "(the adjoint by contraction), FP64 atomics",
+                 "what": "the step's outputs (wrench and adjoint) by the plain path: one launch per force object, "
+                         "one lane per point, the contact math and its adjoint inline, FP64 atomics",
                  "speedup_batched_vs_plain": round(pms / ms, 1)}
         log(f"plain path {pms:.2f} ms/step ({pms / ms:.1f}x slower)")
 
@@ -380,8 +433,12 @@ def main():
     err = None
     if not args.no_cpu:
         threads = min(16, len(os.sched_getaffinity(0)))
-        cpu, err = cpu_baseline(w, out["fm"].cpu().numpy(), args.cpu_budget, threads)
-        log(f"cpu baseline {cpu['value']:.3e} queries/s on {threads} threads; err {err}")
+        b.graph_launch()                      # the graph's outputs: the step's wrench and adjoint
+        torch.cuda.synchronize()
+        gpu = {k: v.cpu().numpy() for k, v in list(out.items()) + list(grads.items())}
+        cpu, err = cpu_baseline(w, gpu, g.cpu().numpy(), args.cpu_budget, threads, args.cpu_budget / 3)
+        log(f"cpu baseline {cpu['value']:.3e} queries/s on {threads} threads, {cpu['one_thread']['value']:.3e} "
+            f"on 1; err {err}")
 
     sdf_build = bdf1 = bvh = None
     if not args.no_extras:
@@ -398,8 +455,8 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": w.name, "instances": b.n, "queries": n_q, "grids": f"{len(w.grids)} x "
                    f"{'x'.join(str(v) for v in w.grids[0].n)} FP32", "active_fraction": round(n_active / n_q, 5),
-                   "step": "detect + slow-path records + contact wrench (records built inline) + reduce "
-                           "+ adjoint (VJP) + reduce, HIP graph replay",
+                   "step": "detect (cell classification, active list) + slow-path records + contact wrench "
+                           "(records built) + reduce + adjoint (VJP) + reduce, HIP graph replay",
                    "tile_points": info["tile_points"], "detect_block": info["detect_block"],
                    "points_per_lane": info["points_per_lane"],
                    "persistent_blocks": info["persistent_blocks"], "n_cu": n_cu, "parallelism": "none"},
