@@ -449,8 +449,9 @@ static int wave_span_of(int tile_points, int block, int ppl) {
 
 static int plan(atac_batch b, const atac_tuning* t) {
     atac_tuning tu = t ? *t : atac_tuning{};
-    if (tu.tile_points <= 0) tu.tile_points = 2048;
-    if (tu.detect_block <= 0) tu.detect_block = 128;
+    if (tu.tile_points <= 0) tu.tile_points = 4096;
+    if (tu.detect_block <= 0) tu.detect_block = 256;
+    if (tu.order == 0) tu.order = 1;
     if (tu.points_per_lane <= 0) tu.points_per_lane = 4;
     if (tu.tile_points > 32768 || tu.tile_points % 64 != 0)
         return fail(ATAC_ERR_INVALID, "tile_points must be a multiple of 64 and <= 32768");
@@ -459,7 +460,7 @@ static int plan(atac_batch b, const atac_tuning* t) {
         return fail(ATAC_ERR_INVALID, "detect_block must be 64, 128, 256, 512 or 1024");
     if (tu.points_per_lane != 1 && tu.points_per_lane != 2 && tu.points_per_lane != 4)
         return fail(ATAC_ERR_INVALID, "points_per_lane must be 1, 2 or 4");
-    if (tu.blocks_per_cu < 0 || tu.order < 0 || tu.order > 2) return fail(ATAC_ERR_INVALID, "bad tuning");
+    if (tu.blocks_per_cu < 0 || tu.order < 0 || tu.order > 3) return fail(ATAC_ERR_INVALID, "bad tuning");
     const int B = tu.detect_block;
     const int span = wave_span_of(tu.tile_points, B, tu.points_per_lane);
     const size_t lds = detect_lds_per_wave(span, tu.points_per_lane) * (B / 64);
@@ -505,9 +506,9 @@ static int plan(atac_batch b, const atac_tuning* t) {
     // work items: at most one partial item per tile beyond the full ones of the pool
     b->chunk_cap = std::min<int64_t>(chunks, b->rec_cap / kChunk + b->tiles.size());
     b->n_tiles = (int32_t)b->tiles.size();
-    // tile queues: order 0 = grouped by grid and cut into one queue per XCD with
-    // balanced point counts; 1 = one queue in instance order; 2 = one queue grouped by grid
-    b->nq = tu.order == 0 ? kXcds : 1;
+    // tile queues: 1 = one queue in instance order; 2 = one queue grouped by grid;
+    // 3 = grouped by grid and cut into one queue per XCD with balanced point counts
+    b->nq = tu.order == 3 ? kXcds : 1;
     std::vector<int32_t> order(b->n_tiles);
     std::iota(order.begin(), order.end(), 0);
     if (tu.order != 1) {

@@ -1,16 +1,21 @@
 // gfx950 kernels of the point-vs-SDF contact path.
 //
 // Batched path:
-//   detect_kernel  persistent; tiles of Morton-sorted FP32 contact points pulled
-//                  from per-XCD queues. Streams each tile (several points per
-//                  lane, nontemporal vector loads), maps every point into grid
-//                  index space with an FP64 affine map, gathers the 8 FP32
-//                  corners, compacts the points with d < 0 in LDS and writes one
-//                  record per active point (sample of phi, grad phi, Hessian).
-//   slow_kernel    the FP64 FD-stencil samples of the few records whose stencil
-//                  leaves their cell.
-//   contact_kernel work items of kChunk records, one wavefront each: the wrench
-//                  (optionally the p6 column and Km/Dm tangent columns) summed
+//   detect_kernel  tiles of Morton-sorted FP32 contact points, taken from one
+//                  queue in instance order (or grid-grouped / per-XCD queues).
+//                  Streams each tile (several points per lane, nontemporal
+//                  loads), maps every point into the grid's padded cell space
+//                  with an FP64 affine map and reads one byte of the grid's cell
+//                  classification table (inactive / active / open). Open cells
+//                  go through an LDS candidate ring and the exact test; the
+//                  active points are appended to the tile's slice of the
+//                  active list, with one 64-bit atomic per tile reserving
+//                  record slots and work items of the bounded record pool.
+//   slow_kernel    the FP64 FD-stencil samples of the few active points whose
+//                  stencil leaves their cell.
+//   contact_kernel work items of kChunk active points, one wavefront each: builds
+//                  the records (sample of phi, grad phi, Hessian) and sums the
+//                  wrench (optionally the p6 column and Km/Dm tangent columns)
 //                  into the item's partial row.
 //   vjp_kernel     the adjoint of the same records.
 //   reduce_kernel  per-instance sums of the partial rows in a fixed order.
@@ -70,9 +75,10 @@ struct LaneAcc {
 };
 
 // ------------------------------------------------------------------ tile assignment
-// Tiles are split into nq contiguous queues of balanced point counts. Workgroup
-// b belongs to queue b % nq (the XCD it is dispatched to, with round-robin
-// dispatch over the XCDs, so a queue's tiles share that XCD's L2) and takes the
+// Tiles are split into nq contiguous queues of balanced point counts (nq is 1,
+// or 8 for per-XCD queues). Workgroup b belongs to queue b % nq (with nq = 8 the
+// XCD it is dispatched to, with round-robin dispatch over the XCDs, so a queue's
+// tiles share that XCD's L2) and takes the
 // queue's tiles r, r + m, r + 2m, ... where r = b / nq is its rank among the m
 // workgroups of the queue. The assignment is static: no shared counter, and
 // every workgroup's tile sequence is known from the start.

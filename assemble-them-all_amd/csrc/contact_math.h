@@ -749,15 +749,18 @@ ATAC_HD void sdf_cell64(const GridDev& G, const Corners& c, const double f[3], d
 }
 
 // ------------------------------------------------------------------ contact math
-// Quotient a / b of the contact math: FP64 divides; FP32 on the GPU multiplies by
-// the hardware reciprocal (v_rcp_f32, 1 ulp) instead of the ten-instruction IEEE
-// division sequence.
-ATAC_HD inline double qdiv(double a, double b) { return a / b; }
-ATAC_HD inline float qdiv(float a, float b) {
+// Quotient a / b of the contact math: FP64 divides. FP32 multiplies by a reciprocal
+// instead of running the ten-instruction IEEE division sequence: on the GPU the
+// hardware one (v_rcp_f32, within 1 ulp of 1/b), in the host build the correctly
+// rounded 1.0f / b, so host checks of this code see the same two roundings
+// (reciprocal, then product; at most about 2 ulp from a / b in total) and differ from
+// the GPU only by v_rcp_f32's last bit.
+ATAC_HD double qdiv(double a, double b) { return a / b; }
+ATAC_HD float qdiv(float a, float b) {
 #if defined(__HIP_DEVICE_COMPILE__)
     return a * __builtin_amdgcn_rcpf(b);
 #else
-    return a / b;
+    return a * (1.0f / b);
 #endif
 }
 

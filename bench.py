@@ -284,8 +284,10 @@ def main():
                     help="skip the mesh -> SDF build and BDF1 stepping measurements (profiling runs)")
     ap.add_argument("--tile-points", type=int, default=0)
     ap.add_argument("--detect-block", type=int, default=0)
-    ap.add_argument("--order", type=int, default=0)
+    ap.add_argument("--order", type=int, default=0, help="tile queues: 0 default, 1 instance order, 2 grid-grouped, 3 per XCD")
     ap.add_argument("--points-per-lane", type=int, default=0)
+    ap.add_argument("--sort-cell", type=float, default=0.0,
+                    help="Morton sort granularity of the point sets (0: extent/1024; 1e30: generated order)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -303,7 +305,7 @@ def main():
     t0 = time.perf_counter()
     tuning = {"tile_points": args.tile_points, "detect_block": args.detect_block, "order": args.order,
               "points_per_lane": args.points_per_lane}
-    b = batch_from_workload(w, tuning=tuning)
+    b = batch_from_workload(w, tuning=tuning, sort_cell=args.sort_cell)
     info = b.info()
     log(f"batch ready in {time.perf_counter() - t0:.1f}s: {info}")
     n_q = info["n_points"]

@@ -53,12 +53,12 @@ typedef struct {
 } atac_instance;
 
 typedef struct {
-    int32_t tile_points;   /* points per tile (0: 2048); a multiple of 64, at most 32768 */
-    int32_t detect_block;  /* threads per detection workgroup: 64, 128, 256, 512, 1024 (0: 128) */
+    int32_t tile_points;   /* points per tile (0: 4096); a multiple of 64, at most 32768 */
+    int32_t detect_block;  /* threads per detection workgroup: 64, 128, 256, 512, 1024 (0: 256) */
     int32_t blocks_per_cu; /* detection: 0 = one workgroup per tile (default); n > 0 = n persistent
                               workgroups per CU (at most the occupancy limit) walking the tiles */
-    int32_t order;         /* work queues: 0 one per XCD, items grouped by grid (default);
-                              1 one queue in instance order; 2 one queue grouped by grid */
+    int32_t order;         /* tile queues: 0 default (1); 1 one queue in instance order;
+                              2 one queue grouped by grid; 3 one queue per XCD, grouped by grid */
     int32_t points_per_lane; /* points a detection lane takes per step (64 apart): 1, 2, 4 (0: 4) */
     int32_t max_active;    /* bound on the active contacts of one evaluation (the record pool, 48 B each;
                               96 B with ATAC_EVAL_PRECISE): 0 = one per point up to 4M points, a quarter of

@@ -31,11 +31,10 @@ def band_mask(w, ins, d_band=1e-6):
     return np.abs(d) < d_band
 
 
-def friction_margin(w, ins, ids):
-    """Relative distance of each contact from the friction branch switch (static if
-    mu*|fc1| >= kt*|tdot| - 1e-8, ForceGeneralSDFContact.cpp:202): the Jacobian
-    jumps there, so where FP32 rounding (~1e-7) can move a contact across it the
-    FP32 and FP64 arithmetic may differ in Km/Dm while the wrench agrees."""
+def friction_terms(w, ins, ids):
+    """(mu*|fc1|, kt*|tdot|) of each contact from the oracle's BodySDFObj::collision
+    values: the friction is static where the first is >= the second - 1e-8
+    (ForceGeneralSDFContact.cpp:202), dynamic otherwise."""
     g = O.GridRef.from_grid(w.grids[ins.grid])
     xi = W.round_points_f32(w.point_sets[ins.points])[ids]
     R1, p1 = ins.E1[:3, :3], ins.E1[:3, 3]
@@ -47,7 +46,14 @@ def friction_margin(w, ins, ids):
     lam = -kn * c["d"] + ins.damping * c["ddot"] * c["d"]
     n1 = c["n"] @ R1
     fcn = np.abs(lam) * np.sqrt(np.sum(np.cross(xi, n1) ** 2, axis=1) + np.sum(n1 ** 2, axis=1))
-    a, b = ins.mu * fcn, kt * np.linalg.norm(c["tdot"], axis=1)
+    return ins.mu * fcn, kt * np.linalg.norm(c["tdot"], axis=1)
+
+
+def friction_margin(w, ins, ids):
+    """Relative distance of each contact from the friction branch switch: the Jacobian
+    jumps there, so where FP32 rounding (~1e-7) can move a contact across it the
+    FP32 and FP64 arithmetic may differ in Km/Dm while the wrench agrees."""
+    a, b = friction_terms(w, ins, ids)
     return np.abs(a - b + 1e-8) / np.maximum(np.maximum(a, b), 1e-300)
 
 

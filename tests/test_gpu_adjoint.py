@@ -8,7 +8,7 @@ import torch
 
 from atac import workloads as W
 from atac.batch import batch_from_workload
-from helpers import branch_stable_contacts, oracle_workload, rel_err
+from helpers import branch_stable_contacts, friction_terms, oracle_workload, rel_err
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -69,6 +69,54 @@ def multilinear_workload(n_poses=8, n_points=4000, seed=21, mu=W.MU, kt=W.KT):
         E2 = W.pose(W.rotation_exp(rng.normal(0, 0.02, 3)), rng.normal(0, 0.01, 3))
         inst.append(W.Instance(0, 0, W.pose(R, t), rng.normal(0, 0.1, 6), E2, rng.normal(0, 0.1, 6), mu=mu, kt=kt))
     return W.Workload("multilinear_ground", [grid], [pts], inst)
+
+
+@pytest.mark.parametrize("branch", ["static", "dynamic"])
+def test_fp32_friction_branch_parity(gpu, branch):
+    """The FP32 contact arithmetic of each friction branch against the oracle. On the GPU
+    the FP32 quotients multiply by the hardware reciprocal (v_rcp_f32), which the host
+    build of the same code does not run, so this pins that path. The twists are set so
+    that nearly every contact is in one branch: static (sliding scaled by 1e-4) or
+    dynamic (a fast tangential slide, where the friction scale mu*|fc|/(kt*|tdot|) and its
+    derivative terms enter Km and Dm). Wrench, Km and Dm (the latter two through the
+    evaluation's Km/Dm outputs and through the adjoint of random cotangents) within 1e-5
+    of the per-contact scale."""
+    w = multilinear_workload(n_poses=8, seed=31)
+    for ins in w.instances:
+        if branch == "static":
+            ins.phi1 = ins.phi1 * 1e-4
+            ins.phi2 = ins.phi2 * 1e-4
+        else:
+            ins.phi1 = ins.phi1 + np.array([0.0, 0.0, 0.0, 2.0, -1.5, 0.0])
+    ref = oracle_workload(w)
+    n_static = n_all = 0
+    for ins, r in zip(w.instances, ref):
+        a, bb = friction_terms(w, ins, np.nonzero(r["active"])[0])
+        n_static += int(np.sum(a >= bb - 1e-8))
+        n_all += len(a)
+    frac = n_static / max(n_all, 1)
+    print(f"  {branch}: {n_all} contacts, static fraction {frac:.4f}", flush=True)
+    assert n_all > 1000
+    assert frac >= 0.95 if branch == "static" else frac <= 0.05
+    b = batch_from_workload(w)
+    g = np.random.default_rng(5).normal(size=(b.n, 12))
+    out = b.eval(deriv=True)
+    b.eval(checkpoint=True)
+    grads = b.vjp(torch.tensor(g, device="cuda"))
+    torch.cuda.synchronize()
+    o = {k: v.cpu().numpy() for k, v in out.items()}
+    gr = {k: v.cpu().numpy() for k, v in grads.items()}
+    worst = 0.0
+    for i, r in enumerate(ref):
+        assert int(o["n_active"][i]) == r["n_active"]
+        sk, sd, gn = r["scale_Km"], max(r["scale_Dm"], 1e-300), np.linalg.norm(g[i])
+        e = [rel_err(o["fm"][i], r["fm"], r["scale_fm"]), rel_err(o["Km"][i], r["Km"], sk),
+             rel_err(o["Dm"][i], r["Dm"], sd),
+             rel_err(np.concatenate([gr["g_q1"][i], gr["g_q2"][i]]), g[i] @ r["Km"], gn * sk),
+             rel_err(np.concatenate([gr["g_phi1"][i], gr["g_phi2"][i]]), g[i] @ r["Dm"], gn * sd)]
+        worst = max(worst, max(e))
+    print(f"  {branch}: worst relative error {worst:.3g} (fm, Km, Dm, adjoint)", flush=True)
+    assert worst <= TOL
 
 
 def _fd_check(w, ref, h, tol, n_check=4, seed=12):
