@@ -10,7 +10,10 @@
 #include "Body/BodyBVHObj.h"
 #include "Body/BodySDFObj.h"
 #include "Body/ContactPoints.h"
+#include "Body/MeshObj.h"
+#include "Joint/Joint.h"
 #include "Simulation/BatchedBDF1.h"
+#include "Simulation/Simulation.h"
 #include "Force/ForceGeneralBVHContact.h"
 #include "Force/ForceGeneralSDFContact.h"
 
@@ -75,6 +78,38 @@ Vector6 v6(const arr& a) {
     return v;
 }
 
+VectorX vec_in(const arr& a) {
+    VectorX v((int)a.size());
+    for (py::ssize_t i = 0; i < a.size(); ++i) v((int)i) = a.data()[i];
+    return v;
+}
+
+py::array_t<double> points_out(const std::vector<Vector3>& V, bool rows_are_points) {
+    const py::ssize_t n = (py::ssize_t)V.size();
+    py::array_t<double> out(rows_are_points ? std::vector<py::ssize_t>{n, 3} : std::vector<py::ssize_t>{3, n});
+    auto w = out.mutable_unchecked<2>();
+    for (py::ssize_t i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            if (rows_are_points) w(i, a) = V[(size_t)i](a);
+            else w(a, i) = V[(size_t)i](a);
+        }
+    return out;
+}
+
+py::array_t<int64_t> faces_out(const std::vector<std::array<int, 3>>& F) {
+    py::array_t<int64_t> out(std::vector<py::ssize_t>{3, (py::ssize_t)F.size()});
+    auto w = out.mutable_unchecked<2>();
+    for (size_t t = 0; t < F.size(); ++t)
+        for (int a = 0; a < 3; ++a) w(a, (py::ssize_t)t) = F[t][a];
+    return out;
+}
+
+py::array_t<double> row_major(const double* p, std::vector<py::ssize_t> shape) {
+    py::array_t<double> out(shape);
+    std::copy(p, p + out.size(), out.mutable_data());
+    return out;
+}
+
 sdfgen::Array3<dtype> grid_of(const arr& values) {
     if (values.ndim() != 3) throw std::invalid_argument("grid values must have shape (nk, nj, ni)");
     sdfgen::Array3<dtype> g;
@@ -87,6 +122,14 @@ sdfgen::Array3<dtype> grid_of(const arr& values) {
 
 PYBIND11_MODULE(_redmax, m) {
     m.doc() = "GPU-backed SDF and BVH contact forces with the reference simulator's C++ interface";
+    // throw_error throws the message as a std::string: raise it as RuntimeError(message)
+    py::register_exception_translator([](std::exception_ptr p) {
+        try {
+            if (p) std::rethrow_exception(p);
+        } catch (const std::string& msg) {
+            PyErr_SetString(PyExc_RuntimeError, msg.c_str());
+        }
+    });
 
     m.def(
         "precompute_contact_points",
@@ -352,4 +395,180 @@ PYBIND11_MODULE(_redmax, m) {
         .def("newton_iterations", &BatchedBDF1::newton_iterations)
         .def_property_readonly("n_candidates", &BatchedBDF1::n_candidates)
         .def_property_readonly("n_evaluations", &BatchedBDF1::n_evaluations);
+
+    m.def("load_obj", [](const std::string& filename, const arr& scale) {
+              TriMesh t = load_obj(filename, vec3(scale));
+              return py::make_tuple(points_out(t.V, true), faces_out(t.F));
+          }, py::arg("filename"), py::arg("scale") = std::vector<double>{1.0, 1.0, 1.0},
+          "BodyMeshObj::load_mesh: (vertices (n, 3), faces (3, m))");
+    m.def("process_mesh", [](const arr& V, const py::array_t<int64_t, py::array::c_style | py::array::forcecast>& F,
+                             double density) {
+              TriMesh t;
+              t.V = points_of(V);
+              if (F.ndim() != 2 || F.shape(1) != 3) throw std::invalid_argument("faces must have shape (m, 3)");
+              for (py::ssize_t i = 0; i < F.shape(0); ++i)
+                  t.F.push_back({(int)F.at(i, 0), (int)F.at(i, 1), (int)F.at(i, 2)});
+              dtype vol;
+              Vector3 com;
+              Matrix3 I;
+              compute_mass_property(t, vol, com, I);
+              MassFrame mf = process_mesh(t, density);
+              py::dict d;
+              d["volume"] = vol;
+              d["com"] = fixed_out(com);
+              d["I_unit_mass"] = fixed_out(I);
+              d["mass"] = mf.mass;
+              d["inertia"] = fixed_out(mf.inertia);
+              d["E_oi"] = fixed_out(mf.E_oi);
+              d["V_body"] = points_out(t.V, true);
+              return d;
+          }, py::arg("V"), py::arg("F"), py::arg("density") = 1.0,
+          "BodyMeshObj::compute_mass_property + process_mesh (volume, centre of mass, principal frame)");
+    m.def("so3_derivatives", [](const arr& r) {
+              if (r.size() != 3) throw std::invalid_argument("r has 3 components");
+              So3Derivs d;
+              so3_derivatives(r.data(), d);
+              py::dict o;
+              o["R"] = row_major(d.R, {3, 3});
+              o["Jr"] = row_major(d.Jr, {3, 3});
+              o["dR"] = row_major(&d.dR[0][0], {3, 3, 3});
+              o["dJr"] = row_major(&d.dJr[0][0], {3, 3, 3});
+              o["d2R"] = row_major(&d.d2R[0][0][0], {3, 3, 3, 3});
+              o["d2Jr"] = row_major(&d.d2Jr[0][0][0], {3, 3, 3, 3});
+              return o;
+          }, py::arg("r"), "exp([r]x), the right Jacobian and their first and second derivatives in r");
+
+    py::class_<Simulation>(m, "Simulation",
+                           "Scenes of mesh / SDF bodies on fixed, translational or free3d-exp joints with "
+                           "general_SDF_contact forces, stepped by BDF1 (redmax_py.Simulation subset)")
+        .def(py::init([](const std::string& xml, const std::string& asset_folder, bool verbose, bool precise) {
+                 return new Simulation(xml, asset_folder, verbose, precise);
+             }),
+             py::arg("xml_string"), py::arg("asset_folder"), py::arg("verbose") = false, py::arg("precise") = false)
+        .def(py::init([](const std::string& path, bool verbose) { return Simulation::from_file(path, verbose).release(); }),
+             py::arg("xml_file_path"), py::arg("verbose") = false)
+        .def_readonly("ndof_r", &Simulation::_ndof_r)
+        .def_readonly("ndof_m", &Simulation::_ndof_m)
+        .def_property_readonly("timestep", [](const Simulation& s) { return s._options->_h; })
+        .def_property_readonly("gravity", [](const Simulation& s) { return fixed_out(s._options->_gravity); })
+        .def_property_readonly("integrator", [](const Simulation& s) { return s._options->_integrator; })
+        .def("init", &Simulation::init)
+        .def("get_q", [](const Simulation& s) { return vec_out(s.get_q()); })
+        .def("get_qdot", [](const Simulation& s) { return vec_out(s.get_qdot()); })
+        .def("get_q_init", [](const Simulation& s) { return vec_out(s.get_q_init()); })
+        .def("get_qdot_init", [](const Simulation& s) { return vec_out(s.get_qdot_init()); })
+        .def("set_state_init", [](Simulation& s, const arr& q, const arr& qd) { s.set_state_init(vec_in(q), vec_in(qd)); },
+             py::arg("q_init"), py::arg("qdot_init"))
+        .def("set_q_init", [](Simulation& s, const arr& q) { s.set_q_init(vec_in(q)); }, py::arg("q_init"))
+        .def("set_qdot_init", [](Simulation& s, const arr& q) { s.set_qdot_init(vec_in(q)); }, py::arg("qdot_init"))
+        .def("set_state", [](Simulation& s, const arr& q, const arr& qd) { s.set_state(vec_in(q), vec_in(qd)); s.update_robot(); },
+             py::arg("q"), py::arg("qdot"))
+        .def("set_q", [](Simulation& s, const arr& q) { s.set_q(vec_in(q)); s.update_robot(); }, py::arg("q"))
+        .def("set_qdot", [](Simulation& s, const arr& q) { s.set_qdot(vec_in(q)); s.update_robot(); }, py::arg("qdot"))
+        .def("update_robot", &Simulation::update_robot, py::arg("design_gradient") = false)
+        .def("get_q_his", [](const Simulation& s) {
+                 std::vector<py::array_t<double>> out;
+                 for (const auto& q : s.get_q_his()) out.push_back(vec_out(q));
+                 return out;
+             })
+        .def("get_qdot_his", [](const Simulation& s) {
+                 std::vector<py::array_t<double>> out;
+                 for (const auto& q : s.get_qdot_his()) out.push_back(vec_out(q));
+                 return out;
+             })
+        .def("get_joint_q", [](const Simulation& s, const std::string& n) { return vec_out(s.get_joint_q(n)); }, py::arg("name"))
+        .def("get_joint_qdot", [](const Simulation& s, const std::string& n) { return vec_out(s.get_joint_qdot(n)); },
+             py::arg("name"))
+        .def("set_joint_q", [](Simulation& s, const std::string& n, const arr& q) { s.set_joint_q(n, vec_in(q)); },
+             py::arg("name"), py::arg("q"))
+        .def("zero_joint_q", &Simulation::zero_joint_q, py::arg("name"))
+        .def("zero_joint_qdot", &Simulation::zero_joint_qdot, py::arg("name"))
+        .def("get_body_mass", &Simulation::get_body_mass, py::arg("name"))
+        .def("get_body_E0i", [](const Simulation& s, const std::string& n) { return fixed_out(s.get_body_E0i(n)); },
+             py::arg("name"))
+        .def("get_body_Ei0", [](const Simulation& s, const std::string& n) { return fixed_out(s.get_body_Ei0(n)); },
+             py::arg("name"))
+        .def("get_body_vertices", [](const Simulation& s, const std::string& n, bool world) {
+                 return points_out(s.get_body_vertices(n, world), false);
+             }, py::arg("name"), py::arg("world_frame") = false)
+        .def("get_body_faces", [](const Simulation& s, const std::string& n) { return faces_out(s.get_body_faces(n)); },
+             py::arg("name"))
+        .def("get_body_contact_points", [](const Simulation& s, const std::string& n) {
+                 return points_out(s.get_body_contact_points(n), true);
+             }, py::arg("name"))
+        .def("get_body_inertia", [](const Simulation& s, const std::string& n) {
+                 for (const auto& b : s._bodies)
+                     if (b.name == n) return fixed_out(b.mass.inertia);
+                 throw std::invalid_argument("no body named " + n);
+             }, py::arg("name"))
+        .def("get_body_Eji", [](const Simulation& s, const std::string& n) {
+                 for (const auto& b : s._bodies)
+                     if (b.name == n) return fixed_out(b.E_ji);
+                 throw std::invalid_argument("no body named " + n);
+             }, py::arg("name"))
+        .def("get_body_sdf", [](const Simulation& s, const std::string& n) {
+                 for (const auto& b : s._bodies)
+                     if (b.name == n) {
+                         if (!b.sdf) throw std::invalid_argument(n + " is not an SDF body");
+                         const BodySDFObj& g = *b.sdf;
+                         py::array_t<float> v(std::vector<py::ssize_t>{g._SDF.nk, g._SDF.nj, g._SDF.ni});
+                         float* w = v.mutable_data();
+                         for (size_t e = 0; e < g._SDF.a.size(); ++e) w[e] = (float)g._SDF.a[e];
+                         py::dict d;
+                         d["values"] = v;
+                         d["min_box"] = fixed_out(g._min_box);
+                         d["max_box"] = fixed_out(g._max_box);
+                         d["dxyz"] = std::vector<double>{g._dx, g._dy, g._dz};
+                         d["col_th"] = g._col_th;
+                         return d;
+                     }
+                 throw std::invalid_argument("no body named " + n);
+             }, py::arg("name"), "the body's SDF grid (values (nk, nj, ni) as the GPU holds them, box, spacing, col_th)")
+        .def("set_body_external_force", [](Simulation& s, const std::string& n, const arr& f) {
+                 s.set_body_external_force(n, v6(f));
+             }, py::arg("name"), py::arg("force"))
+        .def("get_body_distance", &Simulation::get_body_distance, py::arg("name_from"), py::arg("name_to"))
+        .def("body_in_contact", &Simulation::body_in_contact, py::arg("name_a"), py::arg("name_b"), py::arg("eps") = 1e-5)
+        .def("get_contact_bodies", &Simulation::get_contact_bodies, py::arg("name"))
+        .def("clear_contact_bodies", &Simulation::clear_contact_bodies, py::arg("name"))
+        .def("clear_all_saved_sdfs", &Simulation::clear_all_saved_sdfs)
+        .def("set_contact_scale", &Simulation::set_contact_scale, py::arg("scale"))
+        .def("reset", &Simulation::reset, py::arg("backward_flag") = false, py::arg("backward_design_params_flag") = true)
+        .def("forward", &Simulation::forward, py::arg("num_steps"), py::arg("verbose") = false,
+             py::arg("test_derivatives") = false, py::call_guard<py::gil_scoped_release>())
+        .def("is_converged", &Simulation::is_converged)
+        .def("computeMatrices", [](Simulation& s, bool derivs) {
+                 py::dict d;
+                 MatrixX M, K, D;
+                 VectorX f;
+                 if (derivs) {
+                     std::vector<MatrixX> dM;
+                     s.computeMatrices(M, f, dM, K, D);
+                     std::vector<py::array_t<double>> dMs;
+                     for (const auto& x : dM) dMs.push_back(mat_out(x));
+                     d["K"] = mat_out(K);
+                     d["D"] = mat_out(D);
+                     d["dM_dq"] = dMs;
+                 } else {
+                     s.computeMatrices(M, f);
+                 }
+                 d["M"] = mat_out(M);
+                 d["f"] = vec_out(f);
+                 return d;
+             }, py::arg("derivatives") = false, "M, f (and K = df/dq, D = df/dqdot, dM/dq) at the current state")
+        .def("stats", [](const Simulation& s) {
+                 const auto& st = s.stats();
+                 py::dict d;
+                 d["steps"] = st.steps;
+                 d["evaluations"] = st.evaluations;
+                 d["derivative_evaluations"] = st.derivative_evaluations;
+                 d["newton_iterations"] = st.newton_iterations;
+                 d["pairs_evaluated"] = st.pairs_evaluated;
+                 d["pairs_culled"] = st.pairs_culled;
+                 d["batches_built"] = st.batches_built;
+                 d["eval_ms"] = st.eval_ms;
+                 d["step_ms"] = st.step_ms;
+                 return d;
+             })
+        .def("reset_stats", &Simulation::reset_stats);
 }
