@@ -17,6 +17,9 @@ Workloads (seeds are 1000 + workload number):
                    (cylindrical-hole boxes, square-hole boxes, blobs), pose = nominal
                    assembled pose + s*u, s ~U[0.01,0.1], u one of 26 unit directions.
   4 differentiated 1,024 instances (512 peg/hole + 512 blob), 20k points each.
+  5 multi_part     a scene: a fixed base plate and 19 free parts (superquadrics and
+                   square-hole blocks) resting on it, ~100k contact points and ~256^3
+                   grid nodes each, every ordered pair a contact (multi_part_scene).
 Force parameters for all of them: kn=1e4, damping=10, mu=0.3, kt=1e3, col_th=0.01.
 """
 from __future__ import annotations
@@ -468,3 +471,173 @@ def torus(R=1.0, r=0.35, nu=24, nv=12):
             F.append((a, b, c))
             F.append((a, c, d))
     return np.array(V), np.array(F, dtype=np.int64)
+
+
+# ------------------------------------------------------------------ multi-part scene (workload 5)
+
+def superquadric_mesh(axes, eps, n_lat=64, n_lon=128):
+    """Closed mesh of the superquadric |x/a|^(2/e2) + |y/b|^(2/e2))^(e2/e1) + |z/c|^(2/e1) = 1:
+    rings of the (eta, omega) parametrisation plus two pole vertices, outward triangles.
+    (n_lat - 1) * n_lon + 2 vertices."""
+    a, b, c = axes
+    e1, e2 = eps
+
+    def spow(v, e):
+        return np.sign(v) * np.abs(v) ** e
+
+    eta = -np.pi / 2 + np.pi * np.arange(1, n_lat) / n_lat
+    om = -np.pi + 2 * np.pi * np.arange(n_lon) / n_lon
+    E, W = np.meshgrid(eta, om, indexing="ij")
+    x = a * spow(np.cos(E), e1) * spow(np.cos(W), e2)
+    y = b * spow(np.cos(E), e1) * spow(np.sin(W), e2)
+    z = c * spow(np.sin(E), e1)
+    V = np.concatenate([np.stack([x, y, z], axis=-1).reshape(-1, 3), [[0, 0, -c], [0, 0, c]]])
+    south, north = len(V) - 2, len(V) - 1
+    F = []
+    ring = lambda i, j: i * n_lon + (j % n_lon)
+    for i in range(n_lat - 2):
+        for j in range(n_lon):
+            p, q, r, s = ring(i, j), ring(i, j + 1), ring(i + 1, j + 1), ring(i + 1, j)
+            F += [[p, q, r], [p, r, s]]
+    for j in range(n_lon):
+        F.append([south, ring(0, j + 1), ring(0, j)])
+        F.append([north, ring(n_lat - 2, j), ring(n_lat - 2, j + 1)])
+    return V.astype(np.float64), np.array(F, dtype=np.int64)
+
+
+def frame_mesh(outer, hole, height, step):
+    """Closed mesh of a square block with a square through-hole along z: |x|, |y| <= outer,
+    not both |x|, |y| < hole, |z| <= height/2. Every face is split on one lattice of
+    spacing `step` (outer, hole and height/2 multiples of it), so faces meet vertex to
+    vertex and vertices are shared; outward triangles."""
+    na, nb, nh = int(round(outer / step)), int(round(hole / step)), int(round(height / 2 / step))
+    index = {}
+    V = []
+
+    def vid(p):
+        if p not in index:
+            index[p] = len(V)
+            V.append(p)
+        return index[p]
+
+    F = []
+
+    def rect(axis, level, u0, u1, v0, v1, outward_positive):
+        """Lattice rectangle on the plane coord[axis] = level (lattice units), spanning
+        [u0, u1] x [v0, v1] in the two other axes (cyclic order), normal +axis or -axis."""
+        u, v = (axis + 1) % 3, (axis + 2) % 3
+        for i in range(u0, u1):
+            for j in range(v0, v1):
+                def c(di, dj):
+                    p = [0, 0, 0]
+                    p[axis], p[u], p[v] = level, i + di, j + dj
+                    return vid(tuple(p))
+                q = [c(0, 0), c(1, 0), c(1, 1), c(0, 1)]
+                if outward_positive:
+                    F.extend([[q[0], q[1], q[2]], [q[0], q[2], q[3]]])
+                else:
+                    F.extend([[q[0], q[2], q[1]], [q[0], q[3], q[2]]])
+
+    # outer walls x = +-outer (over y, z) and y = +-outer (over z, x)
+    rect(0, na, -na, na, -nh, nh, True)
+    rect(0, -na, -na, na, -nh, nh, False)
+    rect(1, na, -nh, nh, -na, na, True)
+    rect(1, -na, -nh, nh, -na, na, False)
+    # hole walls, facing the hole
+    rect(0, nb, -nb, nb, -nh, nh, False)
+    rect(0, -nb, -nb, nb, -nh, nh, True)
+    rect(1, nb, -nh, nh, -nb, nb, False)
+    rect(1, -nb, -nh, nh, -nb, nb, True)
+    # top and bottom annuli as four rectangles each (axis 2: u = x, v = y)
+    for level, up in ((nh, True), (-nh, False)):
+        rect(2, level, -na, na, nb, na, up)
+        rect(2, level, -na, na, -na, -nb, up)
+        rect(2, level, nb, na, -nb, nb, up)
+        rect(2, level, -na, -nb, -nb, nb, up)
+    return np.array(V, dtype=np.float64) * step, np.array(F, dtype=np.int64)
+
+
+def multi_part_scene(folder, seed=1005, n_parts=20, points_per_part=100_000, grid_res=250, kn=1e4, kt=1e3, mu=0.3,
+                     damping=10.0, col_th=0.01, timestep=1e-3, gravity=9.8):
+    """Workload 5: a fixed base plate and n_parts - 1 free parts (free3d-exp joints) resting
+    on it in a grid, superquadrics and square-hole blocks alternating; each part's mesh
+    has about points_per_part vertices (its contact points) and its SDF grid about
+    grid_res + 3 nodes per axis. Every ordered pair of parts gets a general_SDF_contact
+    (the AABB broad phase keeps the ones that can touch). Writes part<k>.obj into
+    `folder` and returns (xml string, part names)."""
+    import os
+
+    rng = np.random.default_rng(seed)
+    names = [f"part{k}" for k in range(n_parts)]
+    n_free = n_parts - 1
+    cols = int(np.ceil(np.sqrt(n_free * 1.25)))
+    rows = int(np.ceil(n_free / cols))
+    pitch = 2.6
+    plate = (cols * pitch + 1.0, rows * pitch + 1.0, 0.5)
+    # base plate: a box with the requested vertex count
+    side = np.sqrt(points_per_part / (2 * (plate[0] * plate[1] + plate[0] * plate[2] + plate[1] * plate[2])))
+    step = 1.0 / max(side, 1e-9)
+    nx, ny, nz = [max(1, int(round(L / step))) for L in plate]
+    from itertools import product
+    index, V, F = {}, [], []
+
+    def vid(p):
+        if p not in index:
+            index[p] = len(V)
+            V.append(p)
+        return index[p]
+
+    n = (nx, ny, nz)
+    for axis in range(3):
+        u, v = (axis + 1) % 3, (axis + 2) % 3
+        for side_, pos in ((0, False), (n[axis], True)):
+            for i, j in product(range(n[u]), range(n[v])):
+                def c(di, dj):
+                    p = [0, 0, 0]
+                    p[axis], p[u], p[v] = side_, i + di, j + dj
+                    return vid(tuple(p))
+                q = [c(0, 0), c(1, 0), c(1, 1), c(0, 1)]
+                F.extend([[q[0], q[1], q[2]], [q[0], q[2], q[3]]] if pos else [[q[0], q[2], q[1]], [q[0], q[3], q[2]]])
+    Vp = (np.array(V, dtype=np.float64) / np.array(n) - 0.5) * np.array(plate)
+    _write_obj(os.path.join(folder, "part0.obj"), Vp, np.array(F))
+    links = [_scene_link("part0", "fixed", "part0.obj", (0.0, 0.0, -plate[2] / 2), col_th, grid_res)]
+    top = 0.0
+    for k in range(1, n_parts):
+        r, c = divmod(k - 1, cols)
+        cx = (c - (cols - 1) / 2) * pitch
+        cy = (r - (rows - 1) / 2) * pitch
+        if k % 2 == 1:
+            axes = rng.uniform(0.55, 1.0, 3)
+            eps = rng.uniform(0.3, 1.5, 2)
+            n_lat = int(np.sqrt(points_per_part / 2))
+            Vk, Fk = superquadric_mesh(axes, eps, n_lat=n_lat, n_lon=2 * n_lat)
+        else:
+            outer, hole, height = 1.0, 0.5, 0.8
+            area = 8 * outer * height + 8 * hole * height + 2 * (4 * outer * outer - 4 * hole * hole)
+            s = np.sqrt(area / points_per_part)
+            s = 0.1 / max(1, int(round(0.1 / s)))         # outer, hole, height/2 multiples of s
+            Vk, Fk = frame_mesh(outer, hole, height, s)
+        _write_obj(os.path.join(folder, f"part{k}.obj"), Vk, Fk)
+        # resting on the plate, its lowest vertex 0.2 mm past the contact threshold col_th
+        z = top - Vk[:, 2].min() - (col_th + 2e-4)
+        links.append(_scene_link(f"part{k}", "free3d-exp", f"part{k}.obj", (cx, cy, z), col_th, grid_res))
+    contacts = "".join(f'<general_SDF_contact general_body="{a}" SDF_body="{b}"/>'
+                       for a in names for b in names if a != b)
+    xml = (f'<redmax model="multi_part"><option integrator="BDF1" timestep="{timestep}" gravity="0 0 {-gravity}"/>'
+           f'<default><general_SDF_contact kn="{kn}" kt="{kt}" mu="{mu}" damping="{damping}"/></default>'
+           + "".join(links) + f"<contact>{contacts}</contact></redmax>")
+    return xml, names
+
+
+def _scene_link(name, joint, filename, pos, col_th, grid_res):
+    p = " ".join("%.17g" % float(v) for v in pos)
+    return (f'<robot><link name="{name}"><joint name="{name}" type="{joint}" pos="{p}" '
+            f'quat="1 0 0 0" frame="WORLD" damping="0"/><body name="{name}" type="SDF" filename="{filename}" '
+            f'pos="0 0 0" quat="1 0 0 0" scale="1 1 1" transform_type="OBJ_TO_JOINT" density="1" dx="1000" '
+            f'res="{grid_res}" col_th="{col_th}"/></link></robot>')
+
+
+def _write_obj(path, V, F):
+    with open(path, "w") as f:
+        f.write("".join("v %.17g %.17g %.17g\n" % tuple(v) for v in V))
+        f.write("".join("f %d %d %d\n" % tuple(t) for t in (np.asarray(F) + 1)))

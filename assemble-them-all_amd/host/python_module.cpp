@@ -533,6 +533,7 @@ PYBIND11_MODULE(_redmax, m) {
         .def("clear_contact_bodies", &Simulation::clear_contact_bodies, py::arg("name"))
         .def("clear_all_saved_sdfs", &Simulation::clear_all_saved_sdfs)
         .def("set_contact_scale", &Simulation::set_contact_scale, py::arg("scale"))
+        .def("set_broad_phase", &Simulation::set_broad_phase, py::arg("on"))
         .def("reset", &Simulation::reset, py::arg("backward_flag") = false, py::arg("backward_design_params_flag") = true)
         .def("forward", &Simulation::forward, py::arg("num_steps"), py::arg("verbose") = false,
              py::arg("test_derivatives") = false, py::call_guard<py::gil_scoped_release>())

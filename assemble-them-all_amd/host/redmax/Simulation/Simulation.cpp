@@ -724,6 +724,10 @@ std::vector<int> Simulation::broad_phase() const {
     for (size_t c = 0; c < _contacts.size(); ++c) {
         const SimBody& g = _bodies[_contacts[c].general_body];
         const SimBody& s = _bodies[_contacts[c].sdf_body];
+        if (!_broad_phase) {
+            keep.push_back((int)c);
+            continue;
+        }
         if (g.body->_contact_points.empty()) continue;
         double l1[3], h1[3], l2[3], h2[3];
         world_box(g.body->_E_0i, g.pts_lo, g.pts_hi, l1, h1);
@@ -855,6 +859,18 @@ void Simulation::computeMatrices(MatrixX& M, VectorX& f) {
 }
 
 void Simulation::computeMatrices(MatrixX& M, VectorX& f, std::vector<MatrixX>& dM_dq, MatrixX& K, MatrixX& D) {
+    std::vector<std::vector<MatrixX>> dMj;
+    assemble(M, f, K, D, dMj);
+    dM_dq.assign(_ndof_r, MatrixX::Zero(_ndof_r, _ndof_r));
+    for (size_t ji = 0; ji < _joints.size(); ++ji) {
+        const Joint& j = *_joints[ji];
+        for (int k = 0; k < j._ndof; ++k)
+            for (int a = 0; a < j._ndof; ++a)
+                for (int c = 0; c < j._ndof; ++c) dM_dq[j._index[k]](j._index[a], j._index[c]) = dMj[ji][k](a, c);
+    }
+}
+
+void Simulation::assemble(MatrixX& M, VectorX& f, MatrixX& K, MatrixX& D, std::vector<std::vector<MatrixX>>& dMj) {
     update_robot();
     VectorX fm = VectorX::Zero(_ndof_m);
     MatrixX Km = MatrixX::Zero(_ndof_m, _ndof_m), Dm = MatrixX::Zero(_ndof_m, _ndof_m);
@@ -866,7 +882,7 @@ void Simulation::computeMatrices(MatrixX& M, VectorX& f, std::vector<MatrixX>& d
     M = MatrixX::Zero(_ndof_r, _ndof_r);
     K = Kr;
     D = Dr;
-    dM_dq.assign(_ndof_r, MatrixX());
+    dMj.assign(_joints.size(), std::vector<MatrixX>());
     const int nb = (int)_bodies.size();
     std::vector<BodyJac> Js(nb);
     std::vector<std::vector<double>> dJq(nb);   // dJ_dq(k) qdot as columns: 6 x n
@@ -906,17 +922,18 @@ void Simulation::computeMatrices(MatrixX& M, VectorX& f, std::vector<MatrixX>& d
                 D(o + a, o + c) += dq;
             }
         }
+        const size_t ji = (size_t)j._id;
+        dMj[ji].assign(n, MatrixX::Zero(n, n));
         for (int k = 0; k < n; ++k) {
-            // dM_dq(k) = dJ_k^T Mm J + (dJ_k^T Mm J)^T
-            MatrixX dM = MatrixX::Zero(_ndof_r, _ndof_r);
+            // dM_dq(k) = dJ_k^T Mm J + (dJ_k^T Mm J)^T (non-zero only in the joint's block)
+            MatrixX& dM = dMj[ji][k];
             for (int a = 0; a < n; ++a)
                 for (int c = 0; c < n; ++c) {
                     double s = 0.0;
                     for (int r = 0; r < 6; ++r) s += J.dJ[k][n * r + a] * Mm[r] * J.J[n * r + c];
-                    dM(o + a, o + c) += s;
-                    dM(o + c, o + a) += s;
+                    dM(a, c) += s;
+                    dM(c, a) += s;
                 }
-            dM_dq[o + k] = std::move(dM);
             // Kqvv col k = -dJ_k^T (Mm Jdot qdot) - J^T Mm (dJdot_k qdot); + dJ_k^T fm
             double dJdq[6];
             for (int r = 0; r < 6; ++r) {
@@ -932,8 +949,6 @@ void Simulation::computeMatrices(MatrixX& M, VectorX& f, std::vector<MatrixX>& d
             }
         }
     }
-    for (int i = 0; i < _ndof_r; ++i)
-        if (dM_dq[i].rows() == 0) dM_dq[i] = MatrixX::Zero(_ndof_r, _ndof_r);
     // J^T Km J + J^T Dm dJdq_qdot, J^T Dm J over the body pairs with non-zero blocks
     std::set<std::pair<int, int>> blocks;
     for (int bi = 0; bi < nb; ++bi) blocks.insert({bi, bi});
@@ -997,8 +1012,8 @@ void Simulation::evaluate_g_with_derivatives(const VectorX& q1, VectorX& g, Matr
     set_state(q1, qdot1);
     MatrixX M, K, D;
     VectorX f;
-    std::vector<MatrixX> dM;
-    computeMatrices(M, f, dM, K, D);
+    std::vector<std::vector<MatrixX>> dMj;
+    assemble(M, f, K, D, dMj);
     VectorX dq(_ndof_r);
     for (int i = 0; i < _ndof_r; ++i) dq(i) = q1(i) - _q0(i) - _h * _qdot0(i);
     g = VectorX(_ndof_r);
@@ -1009,12 +1024,15 @@ void Simulation::evaluate_g_with_derivatives(const VectorX& q1, VectorX& g, Matr
         g(i) = s - _h * _h * f(i);
         for (int c = 0; c < _ndof_r; ++c) H(i, c) = M(i, c) - _h * _h * K(i, c) - _h * D(i, c);
     }
-    for (int k = 0; k < _ndof_r; ++k)
-        for (int i = 0; i < _ndof_r; ++i) {
-            double s = 0.0;
-            for (int c = 0; c < _ndof_r; ++c) s += dM[k](i, c) * dq(c);
-            H(i, k) += s;
-        }
+    for (size_t ji = 0; ji < _joints.size(); ++ji) {
+        const Joint& j = *_joints[ji];
+        for (int k = 0; k < j._ndof; ++k)
+            for (int a = 0; a < j._ndof; ++a) {
+                double s = 0.0;
+                for (int c = 0; c < j._ndof; ++c) s += dMj[ji][k](a, c) * dq(j._index[c]);
+                H(j._index[a], j._index[k]) += s;
+            }
+    }
 }
 
 // Simulation::newton (Simulation.cpp:999-1079): full Newton steps with a halving line

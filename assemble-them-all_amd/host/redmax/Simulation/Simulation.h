@@ -114,6 +114,8 @@ public:
     void clear_all_saved_sdfs();
     // Robot::set_contact_scale (Robot.cpp:462-472)
     void set_contact_scale(dtype scale);
+    // the AABB broad phase (on by default); off, every contact pair is evaluated
+    void set_broad_phase(bool on) { _broad_phase = on; }
 
     // Simulation::reset / forward (Simulation.cpp:894-993); BDF1 only
     void reset(bool backward_flag = false, bool backward_design_params_flag = true);
@@ -141,6 +143,7 @@ private:
     dtype _h = 0.0;
     bool _newton_converge = true;
     bool _precise = false;
+    bool _broad_phase = true;
     Stats _stats;
 
     // contact evaluation on the GPU
@@ -165,6 +168,8 @@ private:
     // Km, Dm (maximal coordinates)
     void contact_forces(bool derivs, VectorX& fm, MatrixX* Km, MatrixX* Dm);
     void body_forces(bool derivs, VectorX& fm, MatrixX* Km, MatrixX* Dm) const;
+    // M, f, K, D with dM/dq as per-joint blocks (dMj[joint][k] n x n, the rest is zero)
+    void assemble(MatrixX& M, VectorX& f, MatrixX& K, MatrixX& D, std::vector<std::vector<MatrixX>>& dMj);
     void evaluate_g(const VectorX& q1, VectorX& g);
     void evaluate_g_with_derivatives(const VectorX& q1, VectorX& g, MatrixX& H);
     void newton(VectorX& x);

@@ -228,6 +228,73 @@ def bdf1_measurement(cpu):
     return res
 
 
+def multipart_measurement(cpu, steps):
+    """The multi-part step loop (workload 5): a fixed plate and 19 free parts (superquadrics
+    and square-hole blocks on free3d-exp joints) resting on it under gravity, ~100k
+    contact points and ~253^3 grid nodes each, all 380 ordered pairs declared and the
+    AABB broad phase choosing the ones each evaluation runs; BDF1 at dt = 1e-3 with the
+    FP64 Newton on the host (FP32 contact arithmetic). `steps` timed steps after 5
+    untimed ones; the 2,000-step loop's time is extrapolated from them."""
+    import tempfile
+    from atac import _redmax as R
+    from atac import workloads as W
+    with tempfile.TemporaryDirectory() as d:
+        t0 = time.perf_counter()
+        xml, names = W.multi_part_scene(d)
+        t_gen = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        sim = R.Simulation(xml, d)
+        t_build = time.perf_counter() - t0
+    n_points = int(sum(len(sim.get_body_contact_points(m)) for m in names))
+    sim.reset()
+    sim.forward(5)
+    sim.reset_stats()
+    t0 = time.perf_counter()
+    sim.forward(steps)
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    st = sim.stats()
+    ev = max(st["evaluations"], 1)
+    res = {"parts": len(names), "contact_points": n_points, "pairs_declared": len(names) * (len(names) - 1),
+           "grid": "x".join(str(v) for v in sim.get_body_sdf(names[1])["values"].shape[::-1]) + " (part1)",
+           "steps_timed": steps, "ms_per_step": round(ms, 3),
+           "newton_iterations_per_step": round(st["newton_iterations"] / steps, 2),
+           "evaluations_per_step": round(st["evaluations"] / steps, 2),
+           "pairs_evaluated_per_evaluation": round(st["pairs_evaluated"] / ev, 1),
+           "pairs_culled_per_evaluation": round(st["pairs_culled"] / ev, 1),
+           "contact_ms_per_step": round(st["eval_ms"] / steps, 3),
+           "converged_last_step": bool(sim.is_converged()),
+           "loop_2000_steps_s": round(ms * 2000 / 1e3, 1),
+           "scene_generation_s": round(t_gen, 1), "scene_build_s": round(t_build, 1)}
+    if cpu:
+        # the FP64 oracle over the pairs in contact (each free part with the plate, both
+        # ways) at the final state, on all host threads, per evaluation with and without
+        # derivatives; a step's CPU time counts the evaluations the GPU step made
+        from oracle import oracle as O
+        grids, gidx = [], {}
+        insts = []
+        q = {m: W.round_points_f32(np.asarray(sim.get_body_contact_points(m))) for m in names}
+        for a_, b_ in [(m, names[0]) for m in names[1:]] + [(names[0], m) for m in names[1:]]:
+            if b_ not in gidx:
+                gd = sim.get_body_sdf(b_)
+                gidx[b_] = len(grids)
+                grids.append(O.GridRef(gd["values"], gd["min_box"].ravel(), gd["max_box"].ravel(), gd["dxyz"]))
+            insts.append(O.make_instance(gidx[b_], q[a_], sim.get_body_E0i(a_), np.zeros(6), sim.get_body_E0i(b_),
+                                         np.zeros(6), 0.01, W.KN, W.KT, W.MU, W.DAMPING))
+        threads = host_cpu()["nproc"]
+        t0 = time.perf_counter()
+        O.eval_batch(grids, insts, O.FLAG_DERIV, threads)
+        t_d = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        O.eval_batch(grids, insts, 0, threads)
+        t_f = time.perf_counter() - t0
+        cpu_step = (st["derivative_evaluations"] * t_d + (st["evaluations"] - st["derivative_evaluations"]) * t_f) \
+            / steps * 1e3
+        res["cpu_contact_ms_per_step"] = round(cpu_step, 1)
+        res["cpu_kind"] = (f"port (FP64 oracle) over the {len(insts)} plate pairs, {threads} threads, per "
+                           f"evaluation {t_d * 1e3:.0f} ms with derivatives, {t_f * 1e3:.0f} ms without")
+    return res
+
+
 def bvh_measurement():
     """BVH contact variant on a 4,096-triangle torus: exact signed distance +
     pseudonormal queries per second (100k points around the mesh, host arrays in and
@@ -286,6 +353,7 @@ def main():
     ap.add_argument("--detect-block", type=int, default=0)
     ap.add_argument("--order", type=int, default=0, help="tile queues: 0 default, 1 instance order, 2 grid-grouped, 3 per XCD")
     ap.add_argument("--points-per-lane", type=int, default=0)
+    ap.add_argument("--multipart-steps", type=int, default=100, help="timed steps of the multi-part step loop")
     ap.add_argument("--sort-cell", type=float, default=0.0,
                     help="Morton sort granularity of the point sets (0: extent/1024; 1e30: generated order)")
     args = ap.parse_args()
@@ -442,8 +510,10 @@ def main():
         log(f"cpu baseline {cpu['value']:.3e} queries/s on {threads} threads, {cpu['one_thread']['value']:.3e} "
             f"on 1; err {err}")
 
-    sdf_build = bdf1 = bvh = None
+    sdf_build = bdf1 = bvh = multipart = None
     if not args.no_extras:
+        multipart = multipart_measurement(not args.no_cpu, args.multipart_steps)
+        log(f"multi-part step loop: {multipart}")
         sdf_build = sdf_build_measurement(not args.no_cpu)
         log(f"sdf build: {sdf_build}")
         bdf1 = bdf1_measurement(not args.no_cpu)
@@ -477,6 +547,7 @@ def main():
         "sdf_build": sdf_build,
         "bdf1_translational": bdf1,
         "bvh_contact": bvh,
+        "multi_part_loop": multipart,
     }
     print(json.dumps(line), flush=True)
 
