@@ -212,7 +212,43 @@ PYBIND11_MODULE(_redmax, m) {
                     },
                     py::arg("name"), py::arg("V"), py::arg("F"), py::arg("dx"), py::arg("res"), py::arg("col_th"),
                     py::arg("contact_points"), py::arg("index0") = 0)
+        .def(py::init([](std::string filename, const arr& R, const arr& p, double dx, int res, double col_th,
+                         std::string transform_type, double density, const arr& scale, bool adaptive_sample,
+                         bool load_sdf, bool save_sdf) {
+                 if (R.size() != 9) throw std::invalid_argument("R is 3 x 3");
+                 Matrix3 Rm;
+                 for (int r = 0; r < 3; ++r)
+                     for (int c = 0; c < 3; ++c) Rm(r, c) = R.data()[3 * r + c];
+                 BodyMeshObj::TransformType tt;
+                 if (transform_type == "BODY_TO_JOINT") tt = BodyMeshObj::BODY_TO_JOINT;
+                 else if (transform_type == "OBJ_TO_JOINT") tt = BodyMeshObj::OBJ_TO_JOINT;
+                 else throw std::invalid_argument("transform_type: BODY_TO_JOINT or OBJ_TO_JOINT without a joint");
+                 return new BodySDFObj(nullptr, nullptr, filename, Rm, vec3(p), dx, res, col_th, tt, density, vec3(scale),
+                                       adaptive_sample, load_sdf, save_sdf);
+             }),
+             py::arg("filename"), py::arg("R"), py::arg("p"), py::arg("dx"), py::arg("res"), py::arg("col_th"),
+             py::arg("transform_type") = "BODY_TO_JOINT", py::arg("density") = 1.0,
+             py::arg("scale") = std::vector<double>{1.0, 1.0, 1.0}, py::arg("adaptive_sample") = false,
+             py::arg("load_sdf") = false, py::arg("save_sdf") = false,
+             "BodySDFObj(sim, joint, filename, R, p, dx, res, col_th, transform_type, density, scale, "
+             "adaptive_sample, load_sdf, save_sdf) for a body without a joint")
         .def("save_sdf", &BodySDFObj::save_SDF_file)
+        .def("clear_saved_SDF", &BodySDFObj::clear_saved_SDF)
+        .def("surface_contact_pt", [](BodySDFObj& b, const arr& xw) { return fixed_out(b.surface_contact_pt(vec3(xw))); })
+        .def("test_collision_derivatives", &BodySDFObj::test_collision_derivatives)
+        .def("test_collision_derivatives_runtime", [](BodySDFObj& b, const arr& xw, const arr& xw_dot) {
+                 return b.test_collision_derivatives_runtime(vec3(xw), vec3(xw_dot));
+             })
+        .def("get_vertices", [](const BodySDFObj& b) { return points_out(b._V, true); })
+        .def("get_faces", [](const BodySDFObj& b) { return faces_out(b._F); })
+        .def_property_readonly("mass", [](const BodySDFObj& b) { return b._mass; })
+        .def_property_readonly("inertia", [](const BodySDFObj& b) { return fixed_out(b._Inertia); })
+        .def_property_readonly("E_ji", [](const BodySDFObj& b) { return fixed_out(b._E_ji); })
+        .def_property_readonly("E_oi", [](const BodySDFObj& b) { return fixed_out(b._E_oi); })
+        .def("get_AABB", [](const BodySDFObj& b) {
+                 auto box = b.get_AABB();
+                 return py::make_tuple(fixed_out(box.first), fixed_out(box.second));
+             })
         .def_readwrite("col_th", &BodySDFObj::_col_th)
         .def_property_readonly("min_box", [](const BodySDFObj& b) { return fixed_out(b._min_box); })
         .def_property_readonly("max_box", [](const BodySDFObj& b) { return fixed_out(b._max_box); })
@@ -498,12 +534,12 @@ PYBIND11_MODULE(_redmax, m) {
              }, py::arg("name"))
         .def("get_body_inertia", [](const Simulation& s, const std::string& n) {
                  for (const auto& b : s._bodies)
-                     if (b.name == n) return fixed_out(b.mass.inertia);
+                     if (b.name == n) return fixed_out(b.body->_Inertia);
                  throw std::invalid_argument("no body named " + n);
              }, py::arg("name"))
         .def("get_body_Eji", [](const Simulation& s, const std::string& n) {
                  for (const auto& b : s._bodies)
-                     if (b.name == n) return fixed_out(b.E_ji);
+                     if (b.name == n) return fixed_out(b.body->_E_ji);
                  throw std::invalid_argument("no body named " + n);
              }, py::arg("name"))
         .def("get_body_sdf", [](const Simulation& s, const std::string& n) {

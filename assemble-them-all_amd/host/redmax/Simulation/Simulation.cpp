@@ -59,14 +59,6 @@ Matrix3 quat2mat(const std::string& s) {
     return R;
 }
 
-Matrix4 make_se3(const Matrix3& R, const Vector3& p) {
-    Matrix4 E = Matrix4::Identity();
-    for (int i = 0; i < 3; ++i) {
-        for (int j = 0; j < 3; ++j) E(i, j) = R(i, j);
-        E(i, 3) = p(i);
-    }
-    return E;
-}
 Matrix4 mul4(const Matrix4& A, const Matrix4& B) {
     Matrix4 C;
     for (int i = 0; i < 4; ++i)
@@ -125,7 +117,7 @@ BodyJac body_jacobian(const Simulation::SimBody& b, bool derivs) {
     const Joint& j = *b.joint;
     o.n = j._ndof;
     double A[36];
-    adjoint(b.E_ij, A);
+    adjoint(b.body->_E_ij, A);
     auto apply = [&](const MatrixX& S, std::vector<double>& out) {
         out.assign(6 * o.n, 0.0);
         for (int r = 0; r < 6; ++r)
@@ -291,53 +283,10 @@ void Simulation::parse(const std::string& xml_string) {
             SimBody b;
             b.name = bn->attr("name") ? *bn->attr("name") : "";
             b.joint = joint.get();
-            b.filename = filename;
-            b.mesh = load_obj(filename, scale);
-            b.mass = process_mesh(b.mesh, density);
-            // E_ji by transform type (BodyMeshObj.cpp:22-38)
-            const Matrix4& E_oi = b.mass.E_oi;
-            if (ttype == "BODY_TO_JOINT") {
-                b.E_ji = make_se3(bR, bpos);
-            } else if (ttype == "OBJ_TO_WORLD") {
-                b.E_ji = mul4(mul4(einv(joint->_E_pj_0), make_se3(bR, bpos)), E_oi);
-            } else {
-                Matrix3 R_ji;
-                Vector3 p_ji;
-                for (int i = 0; i < 3; ++i) {
-                    for (int j = 0; j < 3; ++j) {
-                        double s = 0.0;
-                        for (int k = 0; k < 3; ++k) s += bR(i, k) * E_oi(k, j);
-                        R_ji(i, j) = s;
-                    }
-                    double s = 0.0;
-                    for (int k = 0; k < 3; ++k) s += bR(i, k) * E_oi(k, 3);
-                    p_ji(i) = s + bpos(i);
-                }
-                b.E_ji = make_se3(R_ji, p_ji);
-            }
-            b.E_ij = einv(b.E_ji);
-            for (int a = 0; a < 3; ++a) {
-                b.bbox_lo(a) = INFINITY;
-                b.bbox_hi(a) = -INFINITY;
-            }
-            for (const auto& v : b.mesh.V)
-                for (int a = 0; a < 3; ++a) {
-                    b.bbox_lo(a) = std::min(b.bbox_lo(a), v(a));
-                    b.bbox_hi(a) = std::max(b.bbox_hi(a), v(a));
-                }
-            std::vector<Vector3> pts = precompute_contact_points(b.mesh.V, b.mesh.F, adaptive);
-            for (int a = 0; a < 3; ++a) {
-                b.pts_lo(a) = INFINITY;
-                b.pts_hi(a) = -INFINITY;
-            }
-            for (const auto& v : pts)
-                for (int a = 0; a < 3; ++a) {
-                    b.pts_lo(a) = std::min(b.pts_lo(a), v(a));
-                    b.pts_hi(a) = std::max(b.pts_hi(a), v(a));
-                }
-            const int index0 = 6 * (int)_bodies.size();
+            const BodyMeshObj::TransformType tt = ttype == "BODY_TO_JOINT" ? BodyMeshObj::BODY_TO_JOINT
+                                                  : ttype == "OBJ_TO_WORLD" ? BodyMeshObj::OBJ_TO_WOLRD
+                                                                            : BodyMeshObj::OBJ_TO_JOINT;
             if (btype == "SDF") {
-                // BodySDFObj (BodySDFObj.cpp:6-36): the grid from the .sdf cache or built
                 dtype dx = 0.05, col_th = 0.01;
                 int res = 20;
                 if (const auto* a = bn->attr("dx")) dx = as_float(*a);
@@ -345,19 +294,25 @@ void Simulation::parse(const std::string& xml_string) {
                 if (const auto* a = bn->attr("col_th")) col_th = as_float(*a);
                 const bool load_sdf = bn->attr("load_sdf") ? as_bool(*bn->attr("load_sdf")) : false;
                 const bool save_sdf = bn->attr("save_sdf") ? as_bool(*bn->attr("save_sdf")) : false;
-                b.sdf_file = filename.size() >= 4 ? filename.substr(0, filename.size() - 4) + ".sdf" : filename + ".sdf";
-                sdfgen::Array3<dtype> grid;
-                Vector3 mn, mx;
-                dtype gdx = 0, gdy = 0, gdz = 0;
-                bool loaded = load_sdf && BodySDFObj::load_SDF_file(b.sdf_file, grid, mn, mx, gdx, gdy, gdz);
-                if (!loaded) BodySDFObj::precompute_SDF(b.mesh.V, b.mesh.F, dx, res, grid, mn, mx, gdx, gdy, gdz);
-                auto* s = new BodySDFObj(this, b.name, grid, mn, mx, gdx, gdy, gdz, col_th, pts, index0);
+                auto* s = new BodySDFObj(this, joint.get(), filename, bR, bpos, dx, res, col_th, tt, density, scale,
+                                         adaptive, load_sdf, save_sdf);
                 b.body.reset(s);
                 b.sdf = s;
-                if (!loaded && save_sdf) s->save_SDF_file(b.sdf_file);
             } else {
-                b.body.reset(new Body(this, b.name, pts, index0));
+                b.body.reset(new BodyMeshObj(this, joint.get(), filename, bR, bpos, tt, density, scale, adaptive));
             }
+            b.body->_name = b.name;
+            const int index0 = 6 * (int)_bodies.size();
+            for (int k = 0; k < 6; ++k) b.body->_index[k] = index0 + k;
+            for (int a = 0; a < 3; ++a) {
+                b.pts_lo(a) = INFINITY;
+                b.pts_hi(a) = -INFINITY;
+            }
+            for (const auto& v : b.body->_contact_points)
+                for (int a = 0; a < 3; ++a) {
+                    b.pts_lo(a) = std::min(b.pts_lo(a), v(a));
+                    b.pts_hi(a) = std::max(b.pts_hi(a), v(a));
+                }
             _bodies.push_back(std::move(b));
             _joints.push_back(std::move(joint));
         }
@@ -489,9 +444,9 @@ void Simulation::set_qdot_init(const VectorX& qdot) {
 void Simulation::update_robot(bool) {
     for (auto& j : _joints) j->update();
     for (auto& b : _bodies) {
-        b.body->_E_0i = mul4(b.joint->_E_0j, b.E_ji);
+        b.body->_E_0i = mul4(b.joint->_E_0j, b.body->_E_ji);
         double A[36];
-        adjoint(b.E_ij, A);
+        adjoint(b.body->_E_ij, A);
         for (int r = 0; r < 6; ++r) {
             double s = 0.0;
             for (int k = 0; k < 6; ++k) s += A[6 * r + k] * b.joint->_phi(k);
@@ -527,7 +482,7 @@ void Simulation::zero_joint_qdot(const std::string& name) {
 dtype Simulation::get_body_mass(const std::string& name) const {
     const int i = body_index(name);
     if (i < 0) throw_error("Simulation: no body named " + name);
-    return _bodies[i].mass.mass;
+    return _bodies[i].body->_mass;
 }
 
 Matrix4 Simulation::get_body_E0i(const std::string& name) const {
@@ -541,7 +496,7 @@ Matrix4 Simulation::get_body_Ei0(const std::string& name) const { return einv(ge
 std::vector<Vector3> Simulation::get_body_vertices(const std::string& name, bool world_frame) const {
     const int i = body_index(name);
     if (i < 0) throw_error("Simulation: no body named " + name);
-    std::vector<Vector3> V = _bodies[i].mesh.V;
+    std::vector<Vector3> V = _bodies[i].body->_V;
     if (world_frame) {
         const Matrix4& E = _bodies[i].body->_E_0i;
         for (auto& v : V) {
@@ -555,7 +510,7 @@ std::vector<Vector3> Simulation::get_body_vertices(const std::string& name, bool
 std::vector<std::array<int, 3>> Simulation::get_body_faces(const std::string& name) const {
     const int i = body_index(name);
     if (i < 0) throw_error("Simulation: no body named " + name);
-    return _bodies[i].mesh.F;
+    return _bodies[i].body->_F;
 }
 
 std::vector<Vector3> Simulation::get_body_contact_points(const std::string& name) const {
@@ -567,7 +522,7 @@ std::vector<Vector3> Simulation::get_body_contact_points(const std::string& name
 void Simulation::set_body_external_force(const std::string& name, const Vector6& force) {
     const int i = body_index(name);
     if (i < 0) throw_error("Simulation: no body named " + name);
-    _bodies[i].f_external = force;
+    _bodies[i].body->set_external_force(force);
 }
 
 dtype Simulation::get_body_distance(const std::string& name_from, const std::string& name_to) {
@@ -610,7 +565,7 @@ void Simulation::clear_all_contact_bodies() {
 
 void Simulation::clear_all_saved_sdfs() {
     for (const auto& b : _bodies)
-        if (b.sdf && !b.sdf_file.empty()) std::remove(b.sdf_file.c_str());
+        if (b.sdf) b.sdf->clear_saved_SDF();
 }
 
 void Simulation::set_contact_scale(dtype scale) {
@@ -631,8 +586,8 @@ void Simulation::body_forces(bool derivs, VectorX& fm, MatrixX* Km, MatrixX* Dm)
         const double v[3] = {b.body->_phi(3), b.body->_phi(4), b.body->_phi(5)};
         double Iw[3], mv[3];
         for (int k = 0; k < 3; ++k) {
-            Iw[k] = b.mass.inertia(k) * w[k];
-            mv[k] = b.mass.inertia(3 + k) * v[k];
+            Iw[k] = b.body->_Inertia(k) * w[k];
+            mv[k] = b.body->_Inertia(3 + k) * v[k];
         }
         // ad(phi)^T (I phi) = [Iw x w + mv x v; mv x w]
         double c1[3], c2[3], c3[3];
@@ -642,17 +597,17 @@ void Simulation::body_forces(bool derivs, VectorX& fm, MatrixX* Km, MatrixX* Dm)
         auto RT = [&](const double x[3], double out[3]) {
             for (int i = 0; i < 3; ++i) out[i] = E(0, i) * x[0] + E(1, i) * x[1] + E(2, i) * x[2];
         };
-        const double mg[3] = {b.mass.mass * _options->_gravity(0), b.mass.mass * _options->_gravity(1),
-                              b.mass.mass * _options->_gravity(2)};
-        const double te[3] = {b.f_external(0), b.f_external(1), b.f_external(2)};
-        const double fe[3] = {b.f_external(3), b.f_external(4), b.f_external(5)};
+        const double mg[3] = {b.body->_mass * _options->_gravity(0), b.body->_mass * _options->_gravity(1),
+                              b.body->_mass * _options->_gravity(2)};
+        const double te[3] = {b.body->_f_external(0), b.body->_f_external(1), b.body->_f_external(2)};
+        const double fe[3] = {b.body->_f_external(3), b.body->_f_external(4), b.body->_f_external(5)};
         double g_b[3], t_b[3], f_b[3];
         RT(mg, g_b);
         RT(te, t_b);
         RT(fe, f_b);
         for (int k = 0; k < 3; ++k) {
-            fm(o + k) += c1[k] + c2[k] + b.mass.inertia(k) * t_b[k];
-            fm(o + 3 + k) += c3[k] + g_b[k] + b.mass.inertia(3 + k) * f_b[k];
+            fm(o + k) += c1[k] + c2[k] + b.body->_Inertia(k) * t_b[k];
+            fm(o + 3 + k) += c3[k] + g_b[k] + b.body->_Inertia(3 + k) * f_b[k];
         }
         if (!derivs) continue;
         MatrixX& K = *Km;
@@ -665,10 +620,10 @@ void Simulation::body_forces(bool derivs, VectorX& fm, MatrixX* Km, MatrixX* Dm)
         // external: rows scaled by the inertia entries
         skew(t_b, S);
         for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) K(o + i, o + j) += S[3 * i + j] * b.mass.inertia(i);
+            for (int j = 0; j < 3; ++j) K(o + i, o + j) += S[3 * i + j] * b.body->_Inertia(i);
         skew(f_b, S);
         for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) K(o + 3 + i, o + j) += S[3 * i + j] * b.mass.inertia(3 + i);
+            for (int j = 0; j < 3; ++j) K(o + 3 + i, o + j) += S[3 * i + j] * b.body->_Inertia(3 + i);
         // Coriolis: ad(phi)^T diag(I) + [[ [Iw]x, [mv]x ], [ [mv]x, 0 ]]
         double adT[36] = {0};
         double Sw[9], Sv[9];
@@ -681,7 +636,7 @@ void Simulation::body_forces(bool derivs, VectorX& fm, MatrixX* Km, MatrixX* Dm)
                 adT[6 * (3 + i) + 3 + j] = -Sw[3 * i + j];
             }
         for (int i = 0; i < 6; ++i)
-            for (int k = 0; k < 6; ++k) D(o + i, o + k) += adT[6 * i + k] * b.mass.inertia(k);
+            for (int k = 0; k < 6; ++k) D(o + i, o + k) += adT[6 * i + k] * b.body->_Inertia(k);
         double SI[9], SM[9];
         skew(Iw, SI);
         skew(mv, SM);
@@ -843,12 +798,12 @@ void Simulation::computeMatrices(MatrixX& M, VectorX& f) {
         for (int r = 0; r < 6; ++r) {
             double s = 0.0;
             for (int c = 0; c < n; ++c) s += J.Jd[n * r + c] * j._qdot(c);
-            v[r] = b.mass.inertia(r) * s;
+            v[r] = b.body->_Inertia(r) * s;
         }
         for (int a = 0; a < n; ++a) {
             for (int c = 0; c < n; ++c) {
                 double s = 0.0;
-                for (int r = 0; r < 6; ++r) s += J.J[n * r + a] * b.mass.inertia(r) * J.J[n * r + c];
+                for (int r = 0; r < 6; ++r) s += J.J[n * r + a] * b.body->_Inertia(r) * J.J[n * r + c];
                 M(o + a, o + c) = s;
             }
             double s = 0.0;
@@ -900,7 +855,7 @@ void Simulation::assemble(MatrixX& M, VectorX& f, MatrixX& K, MatrixX& D, std::v
                 for (int c = 0; c < n; ++c) s += J.dJ[k][n * r + c] * j._qdot(c);
                 dJq[bi][n * r + k] = s;
             }
-        const double* Mm = b.mass.inertia.data();
+        const double* Mm = b.body->_Inertia.data();
         double MJdq[6];   // Mm Jdot qdot
         for (int r = 0; r < 6; ++r) {
             double s = 0.0;
@@ -1095,7 +1050,7 @@ void Simulation::reset(bool backward_flag, bool) {
     set_state(_q_init, _qdot_init);
     clear_all_contact_bodies();
     update_robot();
-    for (auto& b : _bodies) b.f_external.setZero();
+    for (auto& b : _bodies) b.body->reset_external_force();
     _q_his.clear();
     _qdot_his.clear();
     _q_his.push_back(_q_init);

@@ -17,7 +17,6 @@
 #include <vector>
 
 #include "Body/BodySDFObj.h"
-#include "Body/MeshObj.h"
 #include "Joint/Joint.h"
 #include "atac.h"
 
@@ -33,20 +32,13 @@ public:
         std::string _unit = "cm-g";
     };
 
-    // A mesh or SDF body on its joint (Body / BodyMeshObj / BodySDFObj fields)
+    // A mesh or SDF body on its joint
     struct SimBody {
         std::string name;
         Joint* joint = nullptr;
-        std::string filename;
-        TriMesh mesh;                 // vertices in the body frame
-        MassFrame mass;
-        Matrix4 E_ji, E_ij;           // body in its joint's frame
-        Vector6 f_external;           // [torque; force], world frame (Body::set_external_force)
-        std::unique_ptr<Body> body;   // _E_0i, _phi, _index, contact points and history
-        BodySDFObj* sdf = nullptr;    // the same object, for SDF bodies
-        Vector3 bbox_lo, bbox_hi;     // of the vertices, body frame (precompute_bounding_box)
-        Vector3 pts_lo, pts_hi;       // of the contact points, body frame
-        std::string sdf_file;         // <obj>.sdf when load_sdf / save_sdf
+        std::unique_ptr<BodyMeshObj> body;   // a BodySDFObj for SDF bodies
+        BodySDFObj* sdf = nullptr;           // the same object, for SDF bodies
+        Vector3 pts_lo, pts_hi;              // box of the contact points, body frame
     };
 
     // ForceGeneralSDFContact between two bodies (indices into _bodies)

@@ -10,6 +10,7 @@ import pytest
 
 import scene_meshes as SM
 from atac import _redmax as R
+from atac import workloads as W
 from oracle import oracle as O
 from sim_reference import RefScene
 
@@ -165,3 +166,44 @@ def test_sdf_cache_round_trip(assets):
     np.testing.assert_array_equal(g2["min_box"], six(g1["min_box"]))
     sim2.clear_all_saved_sdfs()
     assert not os.path.exists(path)
+
+
+def test_body_sdf_obj_reference_constructor_and_methods(assets):
+    """BodySDFObj(filename, R, p, dx, res, col_th, transform_type, density, ...)
+    (BodySDFObj.h:18-27): the mesh body in its principal frame, its grid built from the
+    body-frame mesh; surface_contact_pt (BodySDFObj.cpp:215-237) against the oracle's
+    collision (xi2 = x - d e) and on a flat field; the collision-derivative self-test;
+    clear_saved_SDF."""
+    path = os.path.join(assets, "box.obj")
+    p = np.array([0.1, 0.2, 0.3])
+    b = R.BodySDFObj(path, np.eye(3), p, 0.05, 20, 0.01, transform_type="OBJ_TO_JOINT", density=2.0, save_sdf=True)
+    m = 1.0 * 0.8 * 0.6 * 2.0
+    assert abs(b.mass - m) < 1e-12
+    np.testing.assert_allclose(b.inertia.ravel()[:3], m / 12 * np.array([0.64 + 0.36, 1.0 + 0.36, 1.0 + 0.64]),
+                               rtol=1e-12)
+    # OBJ_TO_JOINT with R = I: E_ji = [R_oi, p_oi + p]
+    np.testing.assert_allclose(b.E_ji[:3, :3], b.E_oi[:3, :3], atol=1e-15)
+    np.testing.assert_allclose(b.E_ji[:3, 3], b.E_oi[:3, 3] + p, atol=1e-15)
+    g = R.BodySDFObj.from_mesh("same", b.get_vertices(), b.get_faces().T.copy(), 0.05, 20, 0.01, np.zeros((1, 3)))
+    np.testing.assert_array_equal(b.values(), g.values())
+    np.testing.assert_array_equal(b.min_box, g.min_box)
+    assert os.path.exists(os.path.join(assets, "box.sdf"))
+    b.clear_saved_SDF()
+    assert not os.path.exists(os.path.join(assets, "box.sdf"))
+    # surface_contact_pt at points near the surface, in a posed body
+    rng = np.random.default_rng(3)
+    E = W.pose(W.random_rotation(rng), rng.normal(size=3))
+    b.E_0i = E
+    G = O.GridRef(b.values().astype(np.float32), b.min_box.ravel(), b.max_box.ravel(), b.dxyz)
+    xb = rng.uniform(-0.55, 0.55, size=(20, 3)) * np.array([1.0, 0.8, 0.6])
+    xw = xb @ E[:3, :3].T + E[:3, 3]
+    ref = O.collision(G, E, np.zeros(6), 0.01, xw, np.zeros_like(xw))
+    for k in range(len(xw)):
+        np.testing.assert_allclose(b.surface_contact_pt(xw[k]).ravel(), ref["xi2"][k], atol=1e-12)
+    # a flat field: no gradient, so the contact point is the query point itself
+    flat = R.BodySDFObj("flat", -np.ones((4, 4, 4)), [0, 0, 0], [3, 3, 3], [1, 1, 1], 0.0, np.zeros((1, 3)))
+    np.testing.assert_allclose(flat.surface_contact_pt([1.2, 1.7, 0.4]).ravel(), [1.2, 1.7, 0.4], atol=1e-15)
+    # the collision-derivative self-test (random pose, twist and interior point)
+    err = b.test_collision_derivatives()
+    print(f"  test_collision_derivatives: worst relative error {err:.3g}", flush=True)
+    assert err < 1e-5
