@@ -125,6 +125,9 @@ struct atac_batch_s {
     int32_t* h_nact = nullptr;
     double* h_gfm = nullptr;      // 12 per instance
     double* h_grad = nullptr;     // 26 per instance
+    // pinned host staging of the host-buffer entry points' results: the device-to-host
+    // copies run as DMA without the runtime's pageable staging, then one wait
+    char* pin = nullptr;          // 312 doubles per instance | n_active per instance | overflow flag
     // optional per-stage timing: event pairs recorded around each kernel launch
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -564,7 +567,9 @@ static int plan(atac_batch b, const atac_tuning* t) {
         for (int m = 0; m < 3; ++m) {
             int nc = 0;
             ATAC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nc, contact_kernel_ptr(m, pr == 1), 256, 0));
-            b->grid_contact[pr][m] = (int)std::max<int64_t>(1, std::min<int64_t>(items, (int64_t)std::max(nc, 1) * b->n_cu));
+            // the Km/Dm mode takes a work item per workgroup, the others one per wavefront
+            const int64_t wanted = m == 2 ? std::max<int64_t>(1, b->chunk_cap) : items;
+            b->grid_contact[pr][m] = (int)std::max<int64_t>(1, std::min<int64_t>(wanted, (int64_t)std::max(nc, 1) * b->n_cu));
         }
         int nv = 0;
         ATAC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nv, vjp_kernel_ptr(pr == 1), 256, 0));
@@ -623,6 +628,7 @@ int atac_batch_destroy(atac_batch b) {
     dfree(b->d_inst); dfree(b->d_grids);
     for (auto& ct : b->cls) dfree(ct.d);
     dfree(b->h_out); dfree(b->h_nact); dfree(b->h_gfm); dfree(b->h_grad);
+    if (b->pin) (void)hipHostFree(b->pin);
     if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
     delete b;
     return ATAC_OK;
@@ -1044,6 +1050,14 @@ static int ensure_staging(atac_batch b) {
     if (!rc && !b->h_nact) rc = dalloc(&b->h_nact, n);
     if (!rc && !b->h_gfm) rc = dalloc(&b->h_gfm, n * 12);
     if (!rc && !b->h_grad) rc = dalloc(&b->h_grad, n * kVjpOut);
+    if (!rc && !b->pin) {
+        const hipError_t e = hipHostMalloc((void**)&b->pin, n * (312 * sizeof(double) + sizeof(int32_t)) + 16,
+                                           hipHostMallocDefault);
+        if (e != hipSuccess) {
+            b->pin = nullptr;
+            rc = fail(ATAC_ERR_NOMEM, std::string("pinned staging: ") + hipGetErrorString(e));
+        }
+    }
     return rc;
 }
 
@@ -1064,14 +1078,21 @@ int atac_batch_eval_host(atac_batch b, uint32_t flags, double* fm, double* Km, d
     hipStream_t st = b->cap_stream;
     rc = enqueue_eval(b, flags, &o, st);
     if (rc) return rc;
-    if (fm) ATAC_HIP(hipMemcpyAsync(fm, o.fm, sizeof(double) * 12 * n, hipMemcpyDeviceToHost, st));
-    if (dfm_dp6) ATAC_HIP(hipMemcpyAsync(dfm_dp6, o.dfm_dp6, sizeof(double) * 12 * n, hipMemcpyDeviceToHost, st));
-    if (Km) ATAC_HIP(hipMemcpyAsync(Km, o.Km, sizeof(double) * 144 * n, hipMemcpyDeviceToHost, st));
-    if (Dm) ATAC_HIP(hipMemcpyAsync(Dm, o.Dm, sizeof(double) * 144 * n, hipMemcpyDeviceToHost, st));
-    if (n_active) ATAC_HIP(hipMemcpyAsync(n_active, o.n_active, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
-    int32_t ovf = 0;
-    ATAC_HIP(hipMemcpyAsync(&ovf, b->d_counters + CNT_OVERFLOW, sizeof ovf, hipMemcpyDeviceToHost, st));
+    // the device staging's layout (fm | p6 | Km | Dm) copied in at most three pieces
+    double* P = reinterpret_cast<double*>(b->pin);
+    int32_t* Pn = reinterpret_cast<int32_t*>(b->pin + n * 312 * sizeof(double));
+    ATAC_HIP(hipMemcpyAsync(P, base, sizeof(double) * (dfm_dp6 ? 24 : 12) * n, hipMemcpyDeviceToHost, st));
+    if (Km || Dm)
+        ATAC_HIP(hipMemcpyAsync(P + 24 * n, base + 24 * n, sizeof(double) * 288 * n, hipMemcpyDeviceToHost, st));
+    ATAC_HIP(hipMemcpyAsync(Pn, o.n_active, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
+    ATAC_HIP(hipMemcpyAsync(Pn + n, b->d_counters + CNT_OVERFLOW, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     ATAC_HIP(hipStreamSynchronize(st));
+    if (fm) std::memcpy(fm, P, sizeof(double) * 12 * n);
+    if (dfm_dp6) std::memcpy(dfm_dp6, P + 12 * n, sizeof(double) * 12 * n);
+    if (Km) std::memcpy(Km, P + 24 * n, sizeof(double) * 144 * n);
+    if (Dm) std::memcpy(Dm, P + 168 * n, sizeof(double) * 144 * n);
+    if (n_active) std::memcpy(n_active, Pn, sizeof(int32_t) * n);
+    const int32_t ovf = Pn[n];
     if (ovf)
         return fail(ATAC_ERR_NOMEM, "atac_batch_eval_host: more active contacts than the record pool holds (" +
                                         std::to_string(b->rec_cap) + "); plan the batch with a larger max_active");
@@ -1091,12 +1112,14 @@ int atac_batch_vjp_host(atac_batch b, const double* g_fm, double* g_q1, double* 
     atac_gradients g{G, G + 6 * n, G + 12 * n, G + 18 * n, G + 24 * n, G + 25 * n};
     rc = enqueue_vjp(b, b->h_gfm, &g, st);
     if (rc) return rc;
-    double* outs[6] = {g_q1, g_q2, g_phi1, g_phi2, g_kn, g_kt};
-    double* srcs[6] = {g.g_q1, g.g_q2, g.g_phi1, g.g_phi2, g.g_kn, g.g_kt};
-    const size_t w[6] = {6, 6, 6, 6, 1, 1};
-    for (int k = 0; k < 6; ++k)
-        if (outs[k]) ATAC_HIP(hipMemcpyAsync(outs[k], srcs[k], sizeof(double) * w[k] * n, hipMemcpyDeviceToHost, st));
+    // the gradients are contiguous in the staging (q1 | q2 | phi1 | phi2 | kn | kt): one copy
+    double* P = reinterpret_cast<double*>(b->pin);
+    ATAC_HIP(hipMemcpyAsync(P, G, sizeof(double) * 26 * n, hipMemcpyDeviceToHost, st));
     ATAC_HIP(hipStreamSynchronize(st));
+    double* outs[6] = {g_q1, g_q2, g_phi1, g_phi2, g_kn, g_kt};
+    const size_t begin[6] = {0, 6, 12, 18, 24, 25}, w[6] = {6, 6, 6, 6, 1, 1};
+    for (int k = 0; k < 6; ++k)
+        if (outs[k]) std::memcpy(outs[k], P + begin[k] * n, sizeof(double) * w[k] * n);
     return ATAC_OK;
 }
 

@@ -409,12 +409,17 @@ __global__ __launch_bounds__(256) void contact_kernel(ContactArgs a) {
     constexpr bool DERIV = MODE >= 2;
     constexpr int NF = P6 ? 24 : 12;
     constexpr int NACC = NF + (DERIV ? 288 : 0);
-    constexpr int U = kChunk / 64;
+    // records per lane: a wavefront takes a whole work item (kChunk records, 4 per lane),
+    // except with the Km/Dm columns, where the 24 tangent columns and their wave sums
+    // dominate and a workgroup takes an item, its wave w records [64 w, 64 w + 64): four
+    // times the wavefronts for the same work, which small batches need to fill the GPU
+    constexpr int U = DERIV ? 1 : kChunk / 64;
     __shared__ double s_km[DERIV ? 4 : 1][DERIV ? 288 : 1];
     __shared__ double s_red[4][64 * NF];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int total = min(a.counters[CNT_CHUNKS], a.chunk_cap);
-    for (int c = wave_id(); c < total; c += wave_count()) {
+    const int j0 = DERIV ? 64 * w : 0;
+    for (int c = DERIV ? (int)blockIdx.x : wave_id(); c < total; c += DERIV ? (int)gridDim.x : wave_count()) {
         const ChunkDev ch = as_const(a.chunks)[c];
         const InstDev in = as_const(a.inst)[ch.inst];
         const GridDev G = as_const(a.grids)[in.grid];
@@ -431,7 +436,7 @@ __global__ __launch_bounds__(256) void contact_kernel(ContactArgs a) {
         uint32_t e[U];
         float qx[U], qy[U], qz[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) e[u] = (u * 64 + lane < n) ? act[u * 64 + lane] : kActSlow;
+        for (int u = 0; u < U; ++u) e[u] = (j0 + u * 64 + lane < n) ? act[j0 + u * 64 + lane] : kActSlow;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const uint32_t off = e[u] & ~kActSlow;
@@ -441,7 +446,7 @@ __global__ __launch_bounds__(256) void contact_kernel(ContactArgs a) {
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int j = u * 64 + lane;
+            const int j = j0 + u * 64 + lane;
             const bool have = j < n;
             ContactFwd<R> s;
             if (have) {
@@ -491,12 +496,27 @@ __global__ __launch_bounds__(256) void contact_kernel(ContactArgs a) {
         double lv[NF];
 #pragma unroll
         for (int k = 0; k < NF; ++k) lv[k] = acc[k].value();
-        const double col = lds_column_sums<NF>(&s_red[w][0], lv);
-        if (lane < NF) dst[lane] = col;
-        if (DERIV) {
-            __builtin_amdgcn_wave_barrier();
-            for (int k = lane; k < 288; k += 64) dst[NF + k] = s_km[DERIV ? w : 0][DERIV ? k : 0];
-            __builtin_amdgcn_wave_barrier();
+        if (!DERIV) {
+            const double col = lds_column_sums<NF>(&s_red[w][0], lv);
+            if (lane < NF) dst[lane] = col;
+        } else {
+            // the item's row. The wrench (and p6) sums in the order of the one-wavefront
+            // modes: per lane the records 64 apart in turn (wave order), then the lanes'
+            // column sums; the Km/Dm sums in wave order.
+#pragma unroll
+            for (int k = 0; k < NF; ++k) s_red[w][lane * NF + k] = lv[k];
+            __syncthreads();
+            if (w == 0) {
+#pragma unroll
+                for (int k = 0; k < NF; ++k)
+                    lv[k] = ((s_red[0][lane * NF + k] + s_red[1][lane * NF + k]) + s_red[2][lane * NF + k]) +
+                            s_red[3][lane * NF + k];
+                const double col = lds_column_sums<NF>(&s_red[0][0], lv);
+                if (lane < NF) dst[lane] = col;
+            }
+            for (int k = threadIdx.x; k < 288; k += 256)
+                dst[NF + k] = ((s_km[0][k] + s_km[DERIV ? 1 : 0][k]) + s_km[DERIV ? 2 : 0][k]) + s_km[DERIV ? 3 : 0][k];
+            __syncthreads();
         }
     }
 }

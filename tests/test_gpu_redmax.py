@@ -46,8 +46,11 @@ def test_force_with_derivative_matches_oracle(scene):
             assert rel_err(fm, r["fm"], r["scale_fm"]) <= TOL
             assert rel_err(Km, r["Km"], r["scale_Km"]) <= TOL
             assert rel_err(Dm, r["Dm"], r["scale_Dm"]) <= TOL
+        # the wrench alone: the same contacts and summation order; the two evaluations are
+        # separate kernels whose FP32 contact arithmetic may round differently (the compiler
+        # fuses multiply-adds per kernel), so they agree to FP32 rounding of the per-contact scale
         fm2 = f.computeForce(12)
-        assert np.allclose(fm2, fm, rtol=0, atol=1e-9 * max(1.0, r["scale_fm"]))
+        assert rel_err(fm2, fm, max(1.0, r["scale_fm"])) <= 1e-7
 
 
 def test_scatter_into_global_arrays(scene):
