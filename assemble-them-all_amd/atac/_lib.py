@@ -31,7 +31,7 @@ EXPORTED = [
     "atac_batch_active_ids", "atac_batch_touched_values", "atac_eval_plain", "atac_eval_plain_vjp", "atac_batch_timing",
     "atac_batch_workspace",
     "atac_sdf_distance", "atac_sdf_collision",
-    "atac_batch_eval_host", "atac_batch_vjp_host", "atac_sdf_distance_host", "atac_sdf_collision_host",
+    "atac_batch_eval_host", "atac_batch_eval_instances_host", "atac_batch_vjp_host", "atac_sdf_distance_host", "atac_sdf_collision_host",
     "atac_sdf_from_mesh", "atac_sdf_mesh_grid",
     "atac_mesh_create", "atac_mesh_destroy", "atac_mesh_distance_normal", "atac_mesh_distance_normal_host",
     "atac_bvh_force_host", "atac_bvh_force_noalloc",
@@ -111,6 +111,7 @@ def lib():
     L.atac_sdf_distance.argtypes = [vp, dp, C.c_double, vp, i64, vp, vp]
     L.atac_sdf_collision.argtypes = [vp, dp, dp, C.c_double, u32, vp, vp, i64, vp, vp, vp]
     L.atac_batch_eval_host.argtypes = [vp, u32, vp, vp, vp, vp, vp]
+    L.atac_batch_eval_instances_host.argtypes = [vp, vp, i32, u32, vp, vp, vp, vp, vp]
     L.atac_batch_vjp_host.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
     L.atac_sdf_distance_host.argtypes = [vp, dp, C.c_double, vp, i64, vp]
     L.atac_sdf_collision_host.argtypes = [vp, dp, dp, C.c_double, u32, vp, vp, i64, vp, vp]

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -128,6 +129,16 @@ struct atac_batch_s {
     // pinned host staging of the host-buffer entry points' results: the device-to-host
     // copies run as DMA without the runtime's pageable staging, then one wait
     char* pin = nullptr;          // 312 doubles per instance | n_active per instance | overflow flag
+    InstDev* pin_inst = nullptr;  // pinned instance records for atac_batch_eval_instances_host
+    // graphs of atac_batch_eval_instances_host per flag combination, valid while no device
+    // buffer has been (re)allocated since their capture (buf_gen)
+    struct HostGraph {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t e = nullptr;
+        uint64_t gen = 0;
+    };
+    std::map<uint32_t, HostGraph> host_graphs;
+    uint64_t buf_gen = 1;
     // optional per-stage timing: event pairs recorded around each kernel launch
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -423,11 +434,14 @@ static int64_t part_width_of(int mode) {
 static int ensure_buffers(atac_batch b, int mode, bool precise) {
     int rc = ATAC_OK;
     if (b->n_slots > 0) {
+        const void* before[5] = {b->d_act, b->d_rec64, b->d_rec32, b->d_slow, b->d_slow_diag};
         if (!b->d_act) rc = dalloc(&b->d_act, (size_t)b->rec_cap);
         if (!rc && precise && !b->d_rec64) rc = dalloc(&b->d_rec64, (size_t)b->rec_cap);
         if (!rc && !precise && !b->d_rec32) rc = dalloc(&b->d_rec32, (size_t)b->rec_cap);
         if (!rc && !b->d_slow) rc = dalloc(&b->d_slow, (size_t)b->slow_cap);
         if (!rc && !precise && !b->d_slow_diag) rc = dalloc(&b->d_slow_diag, (size_t)b->slow_cap * 4);
+        const void* after[5] = {b->d_act, b->d_rec64, b->d_rec32, b->d_slow, b->d_slow_diag};
+        for (int k = 0; k < 5; ++k) b->buf_gen += before[k] != after[k];
     }
     // wide enough for the adjoint too, so a VJP after a captured evaluation does
     // not reallocate the rows (which would drop the captured graph)
@@ -440,6 +454,7 @@ static int ensure_buffers(atac_batch b, int mode, bool precise) {
         dfree(b->d_part);
         rc = dalloc(&b->d_part, (size_t)(b->chunk_cap * w));
         b->part_width = rc ? 0 : w;
+        ++b->buf_gen;
     }
     return rc;
 }
@@ -451,6 +466,7 @@ static int wave_span_of(int tile_points, int block, int ppl) {
 }
 
 static int plan(atac_batch b, const atac_tuning* t) {
+    ++b->buf_gen;   // the tile plan and its buffers are rebuilt: cached graphs are stale
     atac_tuning tu = t ? *t : atac_tuning{};
     if (tu.tile_points <= 0) tu.tile_points = 4096;
     if (tu.detect_block <= 0) tu.detect_block = 256;
@@ -629,6 +645,11 @@ int atac_batch_destroy(atac_batch b) {
     for (auto& ct : b->cls) dfree(ct.d);
     dfree(b->h_out); dfree(b->h_nact); dfree(b->h_gfm); dfree(b->h_grad);
     if (b->pin) (void)hipHostFree(b->pin);
+    if (b->pin_inst) (void)hipHostFree(b->pin_inst);
+    for (auto& kv : b->host_graphs) {
+        if (kv.second.e) (void)hipGraphExecDestroy(kv.second.e);
+        if (kv.second.g) (void)hipGraphDestroy(kv.second.g);
+    }
     if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
     delete b;
     return ATAC_OK;
@@ -1095,6 +1116,103 @@ int atac_batch_eval_host(atac_batch b, uint32_t flags, double* fm, double* Km, d
     const int32_t ovf = Pn[n];
     if (ovf)
         return fail(ATAC_ERR_NOMEM, "atac_batch_eval_host: more active contacts than the record pool holds (" +
+                                        std::to_string(b->rec_cap) + "); plan the batch with a larger max_active");
+    return ATAC_OK;
+}
+
+int atac_batch_eval_instances_host(atac_batch b, const atac_instance* inst, int32_t n_inst, uint32_t flags, double* fm,
+                                   double* Km, double* Dm, double* dfm_dp6, int32_t* n_active) {
+    if (!b || (n_inst && !inst)) return fail(ATAC_ERR_INVALID, "atac_batch_eval_instances_host: null argument");
+    if (n_inst != (int32_t)b->inst.size())
+        return fail(ATAC_ERR_INVALID, "atac_batch_eval_instances_host: instance count differs");
+    for (int i = 0; i < n_inst; ++i)
+        if (inst[i].grid != b->inst[i].grid || inst[i].points != b->inst[i].points || inst[i].segment != b->inst[i].segment)
+            return fail(ATAC_ERR_INVALID, "atac_batch_eval_instances_host: instance " + std::to_string(i) +
+                                              " changes its grid/points/segment");
+    if (flags & ~(ATAC_EVAL_DERIV | ATAC_EVAL_P6 | ATAC_EVAL_CHECKPOINT | ATAC_EVAL_PRECISE | ATAC_EVAL_ALL_ACTIVE))
+        return fail(ATAC_ERR_INVALID, "atac_batch_eval_instances_host: unknown flag");
+    const int mode = mode_of(flags);
+    if (dfm_dp6 && mode < 1) return fail(ATAC_ERR_INVALID, "dfm_dp6 requested without ATAC_EVAL_P6 or ATAC_EVAL_DERIV");
+    if ((Km || Dm) && mode < 2) return fail(ATAC_ERR_INVALID, "Km/Dm requested without ATAC_EVAL_DERIV");
+    int rc = check_instances(b, inst, n_inst);
+    if (rc) return rc;
+    for (int i = 0; i < n_inst; ++i) {
+        const int32_t tb = b->inst_dev[i].tile_begin, tc = b->inst_dev[i].tile_count;
+        compose_for(b, inst[i], b->inst_dev[i]);
+        b->inst_dev[i].tile_begin = tb;
+        b->inst_dev[i].tile_count = tc;
+    }
+    rc = ensure_cls(b);
+    if (rc) return rc;
+    b->inst.assign(inst, inst + n_inst);
+    b->rec_valid = false;
+    if (n_inst == 0) return ATAC_OK;
+    rc = ensure_staging(b);
+    if (!rc) rc = ensure_buffers(b, mode, (flags & ATAC_EVAL_PRECISE) != 0);
+    if (rc) return rc;
+    const size_t n = (size_t)n_inst;
+    if (!b->pin_inst) {
+        const hipError_t e = hipHostMalloc((void**)&b->pin_inst, sizeof(InstDev) * n, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            b->pin_inst = nullptr;
+            return fail(ATAC_ERR_NOMEM, std::string("pinned staging: ") + hipGetErrorString(e));
+        }
+    }
+    double* base = b->h_out;
+    double* P = reinterpret_cast<double*>(b->pin);
+    int32_t* Pn = reinterpret_cast<int32_t*>(b->pin + n * 312 * sizeof(double));
+    const uint32_t key = flags | ((Km || Dm) ? 1u << 8 : 0u) | (dfm_dp6 ? 1u << 9 : 0u);
+    atac_batch_s::HostGraph& hg = b->host_graphs[key];
+    hipStream_t st = b->cap_stream;
+    if (!hg.e || hg.gen != b->buf_gen) {
+        // capture: instance upload, evaluation, result copies (no stage timing inside)
+        if (hg.e) (void)hipGraphExecDestroy(hg.e);
+        if (hg.g) (void)hipGraphDestroy(hg.g);
+        hg = atac_batch_s::HostGraph{};
+        ATAC_HIP(hipStreamSynchronize(st));
+        const bool timing = b->timing, rc_cap = b->rec_captured, cp = b->cap_precise;
+        b->timing = false;
+        atac_outputs o{};
+        o.fm = base;
+        o.dfm_dp6 = dfm_dp6 ? base + 12 * n : nullptr;
+        o.Km = (Km || Dm) ? base + 24 * n : nullptr;
+        o.Dm = (Km || Dm) ? base + 168 * n : nullptr;
+        o.n_active = b->h_nact;
+        ATAC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hipError_t e = hipMemcpyAsync(b->d_inst, b->pin_inst, sizeof(InstDev) * n, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) rc = enqueue_eval(b, flags, &o, st);
+        if (e == hipSuccess && !rc) e = hipMemcpyAsync(P, base, sizeof(double) * (dfm_dp6 ? 24 : 12) * n, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess && !rc && (Km || Dm))
+            e = hipMemcpyAsync(P + 24 * n, base + 24 * n, sizeof(double) * 288 * n, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess && !rc) e = hipMemcpyAsync(Pn, b->h_nact, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess && !rc)
+            e = hipMemcpyAsync(Pn + n, b->d_counters + CNT_OVERFLOW, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+        hipGraph_t g = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(st, &g);
+        b->timing = timing;
+        b->rec_captured = rc_cap;   // this graph is the library's own: a user capture's state stays as it was
+        b->cap_precise = cp;
+        if (rc || e != hipSuccess || e2 != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            if (rc) return rc;
+            return fail(ATAC_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+        }
+        hg.g = g;
+        ATAC_HIP(hipGraphInstantiate(&hg.e, hg.g, nullptr, nullptr, 0));
+        hg.gen = b->buf_gen;
+    }
+    std::memcpy(b->pin_inst, b->inst_dev.data(), sizeof(InstDev) * n);   // the previous call has finished with it
+    ATAC_HIP(hipGraphLaunch(hg.e, st));
+    ATAC_HIP(hipStreamSynchronize(st));
+    b->rec_valid = true;   // the replay left its records for atac_batch_vjp
+    b->rec_precise = (flags & ATAC_EVAL_PRECISE) != 0;
+    if (fm) std::memcpy(fm, P, sizeof(double) * 12 * n);
+    if (dfm_dp6) std::memcpy(dfm_dp6, P + 12 * n, sizeof(double) * 12 * n);
+    if (Km) std::memcpy(Km, P + 24 * n, sizeof(double) * 144 * n);
+    if (Dm) std::memcpy(Dm, P + 168 * n, sizeof(double) * 144 * n);
+    if (n_active) std::memcpy(n_active, Pn, sizeof(int32_t) * n);
+    if (Pn[n])
+        return fail(ATAC_ERR_NOMEM, "atac_batch_eval_instances_host: more active contacts than the record pool holds (" +
                                         std::to_string(b->rec_cap) + "); plan the batch with a larger max_active");
     return ATAC_OK;
 }

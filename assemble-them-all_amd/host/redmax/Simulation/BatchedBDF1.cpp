@@ -188,11 +188,10 @@ void BatchedBDF1::set_poses(const std::vector<V3>& x, const std::vector<V3>& xdo
 
 void BatchedBDF1::evaluate(bool deriv) {
     if (!batch_) return;
-    check_rc(atac_batch_set_instances(batch_, inst_.data(), (int32_t)inst_.size(), nullptr), "atac_batch_set_instances");
     const uint32_t flags = (deriv ? ATAC_EVAL_DERIV : ATAC_EVAL_FORCE) | (precise_ ? ATAC_EVAL_PRECISE : 0u);
-    check_rc(atac_batch_eval_host(batch_, flags, fm_.data(), deriv ? Km_.data() : nullptr, deriv ? Dm_.data() : nullptr,
-                                  nullptr, nullptr),
-             "atac_batch_eval_host");
+    check_rc(atac_batch_eval_instances_host(batch_, inst_.data(), (int32_t)inst_.size(), flags, fm_.data(),
+                                            deriv ? Km_.data() : nullptr, deriv ? Dm_.data() : nullptr, nullptr, nullptr),
+             "atac_batch_eval_instances_host");
     ++n_evals_;
 }
 

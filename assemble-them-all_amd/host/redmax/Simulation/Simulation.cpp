@@ -744,18 +744,16 @@ void Simulation::contact_forces(bool derivs, VectorX& fm, MatrixX* Km, MatrixX* 
                  "atac_batch_create");
         ++_stats.batches_built;
         it = _batches.emplace(act, cb).first;
-    } else {
-        check_rc(atac_batch_set_instances(it->second.batch, inst.data(), (int32_t)inst.size(), nullptr),
-                 "atac_batch_set_instances");
     }
     it->second.last_use = ++_use_clock;
     const size_t n = act.size();
     std::vector<double> f(12 * n), K(derivs ? 144 * n : 0), D(derivs ? 144 * n : 0);
     std::vector<int32_t> na(n);
     uint32_t flags = (derivs ? ATAC_EVAL_DERIV : ATAC_EVAL_FORCE) | (_precise ? ATAC_EVAL_PRECISE : 0u);
-    check_rc(atac_batch_eval_host(it->second.batch, flags, f.data(), derivs ? K.data() : nullptr,
-                                  derivs ? D.data() : nullptr, nullptr, na.data()),
-             "atac_batch_eval_host");
+    check_rc(atac_batch_eval_instances_host(it->second.batch, inst.data(), (int32_t)inst.size(), flags, f.data(),
+                                            derivs ? K.data() : nullptr, derivs ? D.data() : nullptr, nullptr,
+                                            na.data()),
+             "atac_batch_eval_instances_host");
     ++_stats.evaluations;
     if (derivs) ++_stats.derivative_evaluations;
     for (size_t t = 0; t < n; ++t) {
