@@ -641,3 +641,71 @@ def _write_obj(path, V, F):
     with open(path, "w") as f:
         f.write("".join("v %.17g %.17g %.17g\n" % tuple(v) for v in V))
         f.write("".join("f %d %d %d\n" % tuple(t) for t in (np.asarray(F) + 1)))
+
+
+def subdivided_box_mesh(size, n, center=(0.0, 0.0, 0.0)):
+    """A box of the given edge lengths, each face split into n x n squares of two outward
+    triangles, vertices shared along edges: 6 n^2 + 2 vertices."""
+    size = np.asarray(size, dtype=np.float64)
+    index, V, F = {}, [], []
+
+    def vid(p):
+        if p not in index:
+            index[p] = len(V)
+            V.append(p)
+        return index[p]
+
+    for axis in range(3):
+        u, v = (axis + 1) % 3, (axis + 2) % 3
+        for side, up in ((0, False), (n, True)):
+            for i in range(n):
+                for j in range(n):
+                    def c(di, dj):
+                        p = [0, 0, 0]
+                        p[axis], p[u], p[v] = side, i + di, j + dj
+                        return vid(tuple(p))
+                    q = [c(0, 0), c(1, 0), c(1, 1), c(0, 1)]
+                    F.extend([[q[0], q[1], q[2]], [q[0], q[2], q[3]]] if up else [[q[0], q[2], q[1]], [q[0], q[3], q[2]]])
+    V = (np.array(V, dtype=np.float64) / n - 0.5) * size + np.asarray(center)
+    return V, np.array(F, dtype=np.int64)
+
+
+def peg_frame_scene(folder, peg_points=5000, kn=1e4, kt=1e3, mu=0.3, damping=10.0, col_th=0.01, rotation=True):
+    """The disassembly planner's two-part scene (examples/run_joint_plan.py:140-180): a square
+    peg (0.9 x 0.9 x 1.6, about peg_points mesh vertices) in the 1 x 1 hole of a fixed block
+    (2 x 2 x 0.8, square-hole frame), the peg on a free3d-exp joint (rotation=True) or a
+    translational one, gravity 1e-12 as the planner sets it, contacts both ways. Writes the
+    OBJ files into `folder`; returns the XML string."""
+    import os
+
+    n = max(1, int(round(np.sqrt((peg_points - 2) / 6.0))))
+    Vp, Fp = subdivided_box_mesh((0.9, 0.9, 1.6), n)
+    _write_obj(os.path.join(folder, "peg.obj"), Vp, Fp)
+    Vf, Ff = frame_mesh(1.0, 0.5, 0.8, 0.05)
+    _write_obj(os.path.join(folder, "block.obj"), Vf, Ff)
+    joint = "free3d-exp" if rotation else "translational"
+    body = ('<body name="{n}" type="SDF" filename="{n}.obj" pos="0 0 0" quat="1 0 0 0" scale="1 1 1" '
+            f'transform_type="OBJ_TO_JOINT" density="1" dx="0.05" col_th="{col_th}" mu="0"/>')
+    return (f'<redmax model="peg_frame"><option integrator="BDF1" timestep="1e-3" gravity="0. 0. 1e-12"/>'
+            f'<default><general_SDF_contact kn="{kn}" kt="{kt}" mu="{mu}" damping="{damping}"/></default>'
+            f'<robot><link name="peg"><joint name="peg" type="{joint}" axis="0. 0. 0." pos="0 0 0" quat="1 0 0 0" '
+            f'frame="WORLD" damping="0"/>{body.format(n="peg")}</link></robot>'
+            f'<robot><link name="block"><joint name="block" type="fixed" axis="0. 0. 0." pos="0 0 0" quat="1 0 0 0" '
+            f'frame="WORLD" damping="0"/>{body.format(n="block")}</link></robot>'
+            '<contact><general_SDF_contact general_body="peg" SDF_body="block"/>'
+            '<general_SDF_contact general_body="block" SDF_body="peg"/></contact></redmax>')
+
+
+# the planner's 12 actions for rotation (examples/run_joint_plan.py:593-606): [rotation; translation]
+ROTATION_ACTIONS = np.array([[0, 0, 0, 0, 0, 1], [0, 0, 0, 0, 0, -1], [0, 0, 0, 0, 1, 0], [0, 0, 0, 0, -1, 0],
+                             [0, 0, 0, 1, 0, 0], [0, 0, 0, -1, 0, 0], [0, 0, 1, 0, 0, 0], [0, 0, -1, 0, 0, 0],
+                             [0, 1, 0, 0, 0, 0], [0, -1, 0, 0, 0, 0], [1, 0, 0, 0, 0, 0], [-1, 0, 0, 0, 0, 0]],
+                            dtype=np.float64)
+
+
+def action_force(action, force_mag):
+    """PhysicsPlanner.apply_action (run_joint_plan.py:263-272): the unit action scaled by
+    force_mag, the rotational part three times as strong, as [torque; force]."""
+    a = np.asarray(action, dtype=np.float64)
+    a = a / np.linalg.norm(a) * force_mag
+    return np.concatenate([a[:3] * 3, a[3:]]) if len(a) == 6 else np.concatenate([np.zeros(3), a])

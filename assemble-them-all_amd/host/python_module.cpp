@@ -574,6 +574,49 @@ PYBIND11_MODULE(_redmax, m) {
         .def("forward", &Simulation::forward, py::arg("num_steps"), py::arg("verbose") = false,
              py::arg("test_derivatives") = false, py::call_guard<py::gil_scoped_release>())
         .def("is_converged", &Simulation::is_converged)
+        .def("body_names", [](const Simulation& s) {
+                 std::vector<std::string> n;
+                 for (const auto& b : s._bodies) n.push_back(b.name);
+                 return n;
+             })
+        .def("forward_candidates", [](Simulation& s, const arr& q, const arr& qdot, const arr& forces, int num_steps) {
+                 const py::ssize_t K = q.ndim() == 2 ? q.shape(0) : -1;
+                 const py::ssize_t nb = (py::ssize_t)s._bodies.size();
+                 if (K < 0 || q.shape(1) != s._ndof_r || qdot.ndim() != 2 || qdot.shape(0) != K ||
+                     qdot.shape(1) != s._ndof_r)
+                     throw std::invalid_argument("q and qdot must have shape (candidates, ndof_r)");
+                 if (forces.ndim() != 3 || forces.shape(0) != K || forces.shape(1) != nb || forces.shape(2) != 6)
+                     throw std::invalid_argument("forces must have shape (candidates, bodies, 6)");
+                 std::vector<VectorX> qs(K), qds(K);
+                 std::vector<std::vector<Vector6>> fs(K, std::vector<Vector6>(nb));
+                 for (py::ssize_t c = 0; c < K; ++c) {
+                     qs[c] = VectorX(s._ndof_r);
+                     qds[c] = VectorX(s._ndof_r);
+                     for (int i = 0; i < s._ndof_r; ++i) {
+                         qs[c](i) = q.data()[c * s._ndof_r + i];
+                         qds[c](i) = qdot.data()[c * s._ndof_r + i];
+                     }
+                     for (py::ssize_t b = 0; b < nb; ++b)
+                         for (int k = 0; k < 6; ++k) fs[c][b](k) = forces.data()[(c * nb + b) * 6 + k];
+                 }
+                 std::vector<int> conv;
+                 {
+                     py::gil_scoped_release rel;
+                     s.forward_candidates(qs, qds, fs, num_steps, &conv);
+                 }
+                 py::array_t<double> qo({K, (py::ssize_t)s._ndof_r}), qdo({K, (py::ssize_t)s._ndof_r});
+                 for (py::ssize_t c = 0; c < K; ++c)
+                     for (int i = 0; i < s._ndof_r; ++i) {
+                         qo.mutable_data()[c * s._ndof_r + i] = qs[c](i);
+                         qdo.mutable_data()[c * s._ndof_r + i] = qds[c](i);
+                     }
+                 py::dict d;
+                 d["q"] = qo;
+                 d["qdot"] = qdo;
+                 d["converged"] = conv;
+                 return d;
+             }, py::arg("q"), py::arg("qdot"), py::arg("forces"), py::arg("num_steps"),
+             "candidates of this scene (own state and external forces per body) stepped together")
         .def("computeMatrices", [](Simulation& s, bool derivs) {
                  py::dict d;
                  MatrixX M, K, D;

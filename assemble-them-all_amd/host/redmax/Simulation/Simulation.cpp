@@ -698,20 +698,13 @@ std::vector<int> Simulation::broad_phase() const {
 // keeps, in one batched GPU evaluation; the wrench, Km and Dm scattered at the two
 // bodies' maximal indices (ForceGeneralSDFContact.cpp:46-144) and the contact-body
 // history updated (CollisionDetection.cpp:143-146).
-void Simulation::contact_forces(bool derivs, VectorX& fm, MatrixX* Km, MatrixX* Dm) {
-    if (_contacts.empty()) return;
-    const auto t0 = Clock::now();
-    const std::vector<int> act = broad_phase();
-    _stats.pairs_culled += (long long)(_contacts.size() - act.size());
-    _stats.pairs_evaluated += (long long)act.size();
-    if (act.empty()) return;
-    std::vector<atac_instance> inst(act.size());
-    for (size_t t = 0; t < act.size(); ++t) {
-        const SdfContact& c = _contacts[act[t]];
+void Simulation::collect_instances(std::vector<int>& pairs, std::vector<atac_instance>& inst) const {
+    pairs = broad_phase();
+    for (int p : pairs) {
+        const SdfContact& c = _contacts[p];
         const SimBody& g = _bodies[c.general_body];
         const SimBody& s = _bodies[c.sdf_body];
-        atac_instance& in = inst[t];
-        in = atac_instance{};
+        atac_instance in{};
         in.grid = _grid_of_body[c.sdf_body];
         in.points = 0;
         in.segment = c.general_body;
@@ -728,8 +721,21 @@ void Simulation::contact_forces(bool derivs, VectorX& fm, MatrixX* Km, MatrixX* 
         in.damping = c.damping;
         in.scale = c.scale;
         in.contact_scale = g.body->_contact_scale;
+        inst.push_back(in);
     }
-    auto it = _batches.find(act);
+}
+
+void Simulation::evaluate_instances(const std::vector<int>& key, std::vector<atac_instance>& inst, bool derivs,
+                                    ContactEval& out) {
+    const auto t0 = Clock::now();
+    const size_t n = inst.size();
+    out.derivs = derivs;
+    out.f.assign(12 * n, 0.0);
+    out.K.assign(derivs ? 144 * n : 0, 0.0);
+    out.D.assign(derivs ? 144 * n : 0, 0.0);
+    out.na.assign(n, 0);
+    if (n == 0) return;
+    auto it = _batches.find(key);
     if (it == _batches.end()) {
         if (_batches.size() >= kMaxCachedBatches) {   // drop the least recently used
             auto old = _batches.begin();
@@ -739,40 +745,64 @@ void Simulation::contact_forces(bool derivs, VectorX& fm, MatrixX* Km, MatrixX* 
             _batches.erase(old);
         }
         CachedBatch cb;
-        check_rc(atac_batch_create(_grids.data(), (int32_t)_grids.size(), &_points, 1, inst.data(), (int32_t)inst.size(),
-                                   nullptr, &cb.batch),
+        check_rc(atac_batch_create(_grids.data(), (int32_t)_grids.size(), &_points, 1, inst.data(), (int32_t)n, nullptr,
+                                   &cb.batch),
                  "atac_batch_create");
         ++_stats.batches_built;
-        it = _batches.emplace(act, cb).first;
+        it = _batches.emplace(key, cb).first;
     }
     it->second.last_use = ++_use_clock;
-    const size_t n = act.size();
-    std::vector<double> f(12 * n), K(derivs ? 144 * n : 0), D(derivs ? 144 * n : 0);
-    std::vector<int32_t> na(n);
-    uint32_t flags = (derivs ? ATAC_EVAL_DERIV : ATAC_EVAL_FORCE) | (_precise ? ATAC_EVAL_PRECISE : 0u);
-    check_rc(atac_batch_eval_instances_host(it->second.batch, inst.data(), (int32_t)inst.size(), flags, f.data(),
-                                            derivs ? K.data() : nullptr, derivs ? D.data() : nullptr, nullptr,
-                                            na.data()),
+    const uint32_t flags = (derivs ? ATAC_EVAL_DERIV : ATAC_EVAL_FORCE) | (_precise ? ATAC_EVAL_PRECISE : 0u);
+    check_rc(atac_batch_eval_instances_host(it->second.batch, inst.data(), (int32_t)n, flags, out.f.data(),
+                                            derivs ? out.K.data() : nullptr, derivs ? out.D.data() : nullptr, nullptr,
+                                            out.na.data()),
              "atac_batch_eval_instances_host");
     ++_stats.evaluations;
     if (derivs) ++_stats.derivative_evaluations;
-    for (size_t t = 0; t < n; ++t) {
-        if (na[t] < 0) throw_error("Simulation: the contact evaluation overflowed its record pool");
-        const SdfContact& c = _contacts[act[t]];
+    _stats.eval_ms += ms_since(t0);
+}
+
+void Simulation::scatter_contacts(const ContactEval& ce, size_t st, VectorX& fm, MatrixX* Km, MatrixX* Dm) {
+    const std::vector<int>& pairs = ce.pairs[st];
+    for (size_t k = 0; k < pairs.size(); ++k) {
+        const size_t t = ce.first[st] + k;
+        if (ce.na[t] < 0) throw_error("Simulation: the contact evaluation overflowed its record pool");
+        const SdfContact& c = _contacts[pairs[k]];
         const int o[2] = {6 * c.general_body, 6 * c.sdf_body};
-        for (int r = 0; r < 12; ++r) fm(o[r / 6] + r % 6) += f[12 * t + r];
-        if (derivs)
+        for (int r = 0; r < 12; ++r) fm(o[r / 6] + r % 6) += ce.f[12 * t + r];
+        if (Km && ce.derivs)
             for (int r = 0; r < 12; ++r)
                 for (int q = 0; q < 12; ++q) {
-                    (*Km)(o[r / 6] + r % 6, o[q / 6] + q % 6) += K[144 * t + 12 * r + q];
-                    (*Dm)(o[r / 6] + r % 6, o[q / 6] + q % 6) += D[144 * t + 12 * r + q];
+                    (*Km)(o[r / 6] + r % 6, o[q / 6] + q % 6) += ce.K[144 * t + 12 * r + q];
+                    (*Dm)(o[r / 6] + r % 6, o[q / 6] + q % 6) += ce.D[144 * t + 12 * r + q];
                 }
-        if (na[t] > 0) {
+        if (ce.na[t] > 0) {
             _bodies[c.general_body].body->add_contact_body(_bodies[c.sdf_body].name);
             _bodies[c.sdf_body].body->add_contact_body(_bodies[c.general_body].name);
         }
     }
-    _stats.eval_ms += ms_since(t0);
+}
+
+// ForceGeneralSDFContact::computeForce(WithDerivative) of every pair the broad phase
+// keeps, in one batched GPU evaluation; the wrench, Km and Dm scattered at the two
+// bodies' maximal indices (ForceGeneralSDFContact.cpp:46-144) and the contact-body
+// history updated (CollisionDetection.cpp:143-146).
+void Simulation::contact_forces(bool derivs, VectorX& fm, MatrixX* Km, MatrixX* Dm) {
+    if (_contacts.empty()) return;
+    if (_override) {
+        scatter_contacts(*_override, _override_state, fm, Km, Dm);
+        return;
+    }
+    ContactEval ce;
+    ce.pairs.resize(1);
+    ce.first = {0};
+    std::vector<atac_instance> inst;
+    collect_instances(ce.pairs[0], inst);
+    _stats.pairs_culled += (long long)(_contacts.size() - ce.pairs[0].size());
+    _stats.pairs_evaluated += (long long)ce.pairs[0].size();
+    if (inst.empty()) return;
+    evaluate_instances(ce.pairs[0], inst, derivs, ce);
+    scatter_contacts(ce, 0, fm, Km, Dm);
 }
 
 // ------------------------------------------------------------------ dynamics
@@ -1053,6 +1083,171 @@ void Simulation::reset(bool backward_flag, bool) {
     _qdot_his.clear();
     _q_his.push_back(_q_init);
     _qdot_his.push_back(_qdot_init);
+}
+
+void Simulation::forward_candidates(std::vector<VectorX>& q, std::vector<VectorX>& qdot,
+                                    const std::vector<std::vector<Vector6>>& f_external, int num_steps,
+                                    std::vector<int>* converged) {
+    const size_t nc = q.size();
+    if (qdot.size() != nc || f_external.size() != nc)
+        throw_error("forward_candidates: q, qdot and f_external need one entry per candidate");
+    for (size_t c = 0; c < nc; ++c) {
+        if (q[c].size() != _ndof_r || qdot[c].size() != _ndof_r) throw_error("forward_candidates: state size != ndof_r");
+        if (f_external[c].size() != _bodies.size()) throw_error("forward_candidates: one external force per body");
+    }
+    if (_options->_integrator != "BDF1")
+        throw_error("Integrator " + _options->_integrator + " is not in this build (BDF1)");
+    // the scene's own state, restored at the end
+    const VectorX q_save = get_q(), qd_save = get_qdot();
+    std::vector<Vector6> f_save;
+    for (const auto& b : _bodies) f_save.push_back(b.body->_f_external);
+    const dtype h = _options->_h;
+    const int max_iter = std::max(20 * _ndof_r, 100);
+    // per candidate: the reference newton's state (Simulation.cpp:999-1079)
+    struct Cand {
+        VectorX q0, qd0, x, dx, g, gnew;
+        dtype gnorm = 0, alpha = 1;
+        int trial = 0, fail_strike = 0, iter = 0;
+        bool active = false, ls_done = false, ls_ok = false, conv = false;
+    };
+    std::vector<Cand> C(nc);
+    auto load = [&](size_t c, const VectorX& x) {
+        VectorX xd(_ndof_r);
+        for (int i = 0; i < _ndof_r; ++i) xd(i) = (x(i) - C[c].q0(i)) / h;
+        set_state(x, xd);
+        for (size_t b = 0; b < _bodies.size(); ++b) _bodies[b].body->_f_external = f_external[c][b];
+        update_robot();
+        _q0 = C[c].q0;
+        _qdot0 = C[c].qd0;
+        _h = h;
+    };
+    // one round: the states xs[k] of candidates cs[k] evaluated together; g (and H)
+    auto round = [&](const std::vector<size_t>& cs, const std::vector<VectorX>& xs, bool derivs,
+                     std::vector<VectorX>& g, std::vector<MatrixX>* H) {
+        ContactEval ce;
+        std::vector<atac_instance> inst;
+        std::vector<int> key;
+        ce.pairs.resize(cs.size());
+        for (size_t k = 0; k < cs.size(); ++k) {
+            load(cs[k], xs[k]);
+            ce.first.push_back(inst.size());
+            if (!_contacts.empty()) collect_instances(ce.pairs[k], inst);
+            _stats.pairs_culled += (long long)(_contacts.size() - ce.pairs[k].size());
+            _stats.pairs_evaluated += (long long)ce.pairs[k].size();
+            key.insert(key.end(), ce.pairs[k].begin(), ce.pairs[k].end());
+            key.push_back(-1);
+        }
+        evaluate_instances(key, inst, derivs, ce);
+        g.assign(cs.size(), VectorX());
+        if (H) H->assign(cs.size(), MatrixX());
+        for (size_t k = 0; k < cs.size(); ++k) {
+            load(cs[k], xs[k]);
+            _override = &ce;
+            _override_state = k;
+            try {
+                if (derivs) evaluate_g_with_derivatives(xs[k], g[k], (*H)[k]);
+                else evaluate_g(xs[k], g[k]);
+            } catch (...) {
+                _override = nullptr;
+                throw;
+            }
+            _override = nullptr;
+        }
+    };
+    for (int step = 0; step < num_steps; ++step) {
+        for (size_t c = 0; c < nc; ++c) {
+            C[c] = Cand();
+            C[c].q0 = q[c];
+            C[c].qd0 = qdot[c];
+            C[c].x = VectorX(_ndof_r);
+            for (int i = 0; i < _ndof_r; ++i) C[c].x(i) = q[c](i) + h * qdot[c](i);
+            C[c].active = _ndof_r > 0;
+        }
+        while (true) {
+            std::vector<size_t> act;
+            for (size_t c = 0; c < nc; ++c)
+                if (C[c].active) act.push_back(c);
+            if (act.empty()) break;
+            // derivative round: g, H at x; the Newton directions
+            std::vector<VectorX> xs, g;
+            std::vector<MatrixX> H;
+            for (size_t c : act) xs.push_back(C[c].x);
+            round(act, xs, true, g, &H);
+            for (size_t k = 0; k < act.size(); ++k) {
+                Cand& cd = C[act[k]];
+                cd.g = g[k];
+                VectorX mg(_ndof_r);
+                for (int i = 0; i < _ndof_r; ++i) mg(i) = -cd.g(i);
+                cd.dx = lu_solve(H[k], mg);
+                cd.gnorm = norm(cd.g);
+                cd.alpha = 1.0;
+                cd.trial = 0;
+                cd.ls_done = false;
+                cd.ls_ok = false;
+                ++_stats.newton_iterations;
+            }
+            // line-search rounds: halving alpha until |g| decreases, 20 trials at most
+            while (true) {
+                std::vector<size_t> ls;
+                for (size_t c : act)
+                    if (!C[c].ls_done) ls.push_back(c);
+                if (ls.empty()) break;
+                std::vector<VectorX> xt;
+                for (size_t c : ls) {
+                    VectorX x(_ndof_r);
+                    for (int i = 0; i < _ndof_r; ++i) x(i) = C[c].x(i) + C[c].alpha * C[c].dx(i);
+                    xt.push_back(x);
+                }
+                std::vector<VectorX> gn;
+                round(ls, xt, false, gn, nullptr);
+                for (size_t k = 0; k < ls.size(); ++k) {
+                    Cand& cd = C[ls[k]];
+                    cd.gnew = gn[k];
+                    if (norm(cd.gnew) < cd.gnorm) {
+                        cd.ls_ok = true;
+                        cd.ls_done = true;
+                    } else {
+                        cd.alpha *= 0.5;
+                        if (++cd.trial >= kMaxIterLS) cd.ls_done = true;
+                    }
+                }
+            }
+            for (size_t c : act) {
+                Cand& cd = C[c];
+                if (cd.ls_ok) {
+                    cd.fail_strike = 0;
+                } else if (++cd.fail_strike >= kMaxFailStrike) {
+                    cd.active = false;
+                    continue;
+                }
+                for (int i = 0; i < _ndof_r; ++i) cd.x(i) += cd.alpha * cd.dx(i);
+                if (norm(cd.gnew) < kNewtonTol) {
+                    cd.conv = true;
+                    cd.active = false;
+                    continue;
+                }
+                if (++cd.iter >= max_iter) cd.active = false;
+            }
+        }
+        // qdot1 = (q1 - q0)/h, then the joints' reparametrisation
+        for (size_t c = 0; c < nc; ++c) {
+            VectorX qd1(_ndof_r);
+            for (int i = 0; i < _ndof_r; ++i) qd1(i) = (C[c].x(i) - C[c].q0(i)) / h;
+            set_state(C[c].x, qd1);
+            reparam();
+            update_robot();
+            q[c] = get_q();
+            qdot[c] = get_qdot();
+        }
+        ++_stats.steps;
+    }
+    if (converged) {
+        converged->assign(nc, 0);
+        for (size_t c = 0; c < nc; ++c) (*converged)[c] = C[c].conv ? 1 : 0;
+    }
+    set_state(q_save, qd_save);
+    for (size_t b = 0; b < _bodies.size(); ++b) _bodies[b].body->_f_external = f_save[b];
+    update_robot();
 }
 
 void Simulation::forward(int num_steps, bool verbose, bool) {

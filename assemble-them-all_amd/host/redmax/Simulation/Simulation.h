@@ -114,6 +114,18 @@ public:
     void forward(int num_steps, bool verbose = false, bool test_derivatives = false);
     bool is_converged() const { return _newton_converge; }
 
+    // Candidates of a planner's search (examples/run_joint_plan.py:593-660 steps each
+    // action from a state with its own forward calls): each candidate is this scene
+    // with its own state q, qdot and external forces (one [torque; force] per body),
+    // stepped num_steps BDF1 steps. All candidates advance in lockstep, each with its
+    // own Newton decisions, so each ends as forward would leave it; every derivative
+    // round and every line-search round of all of them is one batched contact
+    // evaluation. The scene's own state is left unchanged. converged (may be null)
+    // receives each candidate's last-step Newton convergence.
+    void forward_candidates(std::vector<VectorX>& q, std::vector<VectorX>& qdot,
+                            const std::vector<std::vector<Vector6>>& f_external, int num_steps,
+                            std::vector<int>* converged = nullptr);
+
     // M qddot = f in reduced coordinates, with K = df/dq, D = df/dqdot and dM/dq
     // (Simulation::computeMatrices, Simulation.cpp:379-506) at the current state
     void computeMatrices(MatrixX& M, VectorX& f);
@@ -156,6 +168,25 @@ private:
     Joint* joint_by_name(const std::string& name);
     // the contact pairs that can touch at the current poses
     std::vector<int> broad_phase() const;
+    // one contact evaluation of several states: per state its kept pairs, and the
+    // batch outputs of all of them (12, 144, 144 per pair)
+    struct ContactEval {
+        std::vector<std::vector<int>> pairs;
+        std::vector<size_t> first;   // per state, its first pair's position in the outputs
+        std::vector<double> f, K, D;
+        std::vector<int32_t> na;
+        bool derivs = false;
+    };
+    // the instances of the kept pairs at the current state, appended
+    void collect_instances(std::vector<int>& pairs, std::vector<atac_instance>& inst) const;
+    // evaluate instances of the given pair-list key with a cached batch
+    void evaluate_instances(const std::vector<int>& key, std::vector<atac_instance>& inst, bool derivs, ContactEval& out);
+    // the forces of state s of `ce` added at the bodies' maximal indices (and the history)
+    void scatter_contacts(const ContactEval& ce, size_t s, VectorX& fm, MatrixX* Km, MatrixX* Dm);
+    // when set, contact_forces takes state _override_state of this evaluation instead
+    // of evaluating the current state
+    const ContactEval* _override = nullptr;
+    size_t _override_state = 0;
     // all contact forces at the current state: fm (6 per body), and with derivatives
     // Km, Dm (maximal coordinates)
     void contact_forces(bool derivs, VectorX& fm, MatrixX* Km, MatrixX* Dm);

@@ -295,6 +295,73 @@ def multipart_measurement(cpu, steps):
     return res
 
 
+def rotation_sweep_measurement(cpu, frame_skip=100):
+    """One expansion of the disassembly planner's rotation search (plan_rot,
+    examples/run_joint_plan.py:593-660): from a state, each of the 12 actions (a torque or a
+    force along each axis, force_mag 1e3, torques three times as strong) is stepped
+    frame_skip = 100 BDF1 steps of a peg on a free3d-exp joint in the square hole of a
+    fixed block (the planner's contact parameters: kn 1e6, kt 1e3, mu 0, damping 0,
+    dt 1e-3). Timed both ways: the 12 candidates in lockstep (forward_candidates, one
+    batched contact evaluation per Newton round of all of them) and one at a time
+    (reset, set the state and the force, forward), as the planner's loop does."""
+    import tempfile
+    from atac import _redmax as R
+    from atac import workloads as W
+    with tempfile.TemporaryDirectory() as d:
+        xml = W.peg_frame_scene(d, kn=1e6, kt=1e3, mu=0.0, damping=0.0)
+        sim = R.Simulation(xml, d)
+    names = sim.body_names()
+    K = len(W.ROTATION_ACTIONS)
+    forces = np.zeros((K, len(names), 6))
+    for c, a in enumerate(W.ROTATION_ACTIONS):
+        forces[c, names.index("peg")] = W.action_force(a, 1e3)
+    q0, qd0 = np.zeros((K, sim.ndof_r)), np.zeros((K, sim.ndof_r))
+    sim.forward_candidates(q0, qd0, forces, 2)           # builds the batches
+    sim.reset_stats()
+    t0 = time.perf_counter()
+    out = sim.forward_candidates(q0, qd0, forces, frame_skip)
+    ms_lock = (time.perf_counter() - t0) * 1e3
+    st_lock = sim.stats()
+    sim.reset_stats()
+    t0 = time.perf_counter()
+    moved = []
+    for c in range(K):
+        sim.set_state_init(q0[c], qd0[c])
+        sim.reset()
+        sim.set_body_external_force("peg", forces[c, names.index("peg")])
+        sim.forward(frame_skip)
+        moved.append(float(np.abs(sim.get_q() - out["q"][c]).max()))
+    ms_seq = (time.perf_counter() - t0) * 1e3
+    st_seq = sim.stats()
+    sim.set_body_external_force("peg", np.zeros(6))
+    res = {"candidates": K, "steps_per_candidate": frame_skip,
+           "contact_points": {m: len(sim.get_body_contact_points(m)) for m in names},
+           "lockstep_ms": round(ms_lock, 1), "one_at_a_time_ms": round(ms_seq, 1),
+           "speedup": round(ms_seq / ms_lock, 2),
+           "lockstep_evaluations": st_lock["evaluations"], "one_at_a_time_evaluations": st_seq["evaluations"],
+           "lockstep_contact_ms": round(st_lock["eval_ms"], 1), "one_at_a_time_contact_ms": round(st_seq["eval_ms"], 1),
+           "max_abs_q_difference": max(moved),
+           "converged": int(np.sum(out["converged"]))}
+    if cpu:
+        # the FP64 oracle for both pairs of one candidate at its final state, one thread,
+        # with derivatives, times the evaluations the one-at-a-time loop made
+        from oracle import oracle as O
+        sim.set_state(out["q"][6], out["qdot"][6])
+        grids, insts = [], []
+        for i, (a_, b_) in enumerate([("peg", "block"), ("block", "peg")]):
+            gd = sim.get_body_sdf(b_)
+            grids.append(O.GridRef(gd["values"], gd["min_box"].ravel(), gd["max_box"].ravel(), gd["dxyz"]))
+            insts.append(O.make_instance(i, W.round_points_f32(np.asarray(sim.get_body_contact_points(a_))),
+                                         sim.get_body_E0i(a_), np.zeros(6), sim.get_body_E0i(b_), np.zeros(6), 0.01,
+                                         1e6, 1e3, 0.0, 0.0))
+        t0 = time.perf_counter()
+        O.eval_batch(grids, insts, O.FLAG_DERIV, 1)
+        t_d = time.perf_counter() - t0
+        res["cpu_contact_ms_one_at_a_time"] = round(t_d * 1e3 * st_seq["evaluations"], 1)
+        res["cpu_kind"] = f"port (FP64 oracle), 1 thread, {t_d * 1e3:.1f} ms per evaluation with derivatives"
+    return res
+
+
 def bvh_measurement():
     """BVH contact variant on a 4,096-triangle torus: exact signed distance +
     pseudonormal queries per second (100k points around the mesh, host arrays in and
@@ -510,10 +577,12 @@ def main():
         log(f"cpu baseline {cpu['value']:.3e} queries/s on {threads} threads, {cpu['one_thread']['value']:.3e} "
             f"on 1; err {err}")
 
-    sdf_build = bdf1 = bvh = multipart = None
+    sdf_build = bdf1 = bvh = multipart = sweep = None
     if not args.no_extras:
         multipart = multipart_measurement(not args.no_cpu, args.multipart_steps)
         log(f"multi-part step loop: {multipart}")
+        sweep = rotation_sweep_measurement(not args.no_cpu)
+        log(f"rotation sweep: {sweep}")
         sdf_build = sdf_build_measurement(not args.no_cpu)
         log(f"sdf build: {sdf_build}")
         bdf1 = bdf1_measurement(not args.no_cpu)
@@ -548,6 +617,7 @@ def main():
         "bdf1_translational": bdf1,
         "bvh_contact": bvh,
         "multi_part_loop": multipart,
+        "rotation_sweep": sweep,
     }
     print(json.dumps(line), flush=True)
 

@@ -207,3 +207,36 @@ def test_body_sdf_obj_reference_constructor_and_methods(assets):
     err = b.test_collision_derivatives()
     print(f"  test_collision_derivatives: worst relative error {err:.3g}", flush=True)
     assert err < 1e-5
+
+
+def test_candidates_in_lockstep_match_single_runs(assets):
+    """The planner's candidate actions (run_joint_plan.py:593-606: a force or a torque
+    along each axis) stepped together: every candidate's trajectory equals, bit for bit,
+    the one reset + forward gives it alone; the rounds evaluate all candidates' contact
+    pairs in one batch each."""
+    sim = R.Simulation(_scene("free3d-exp"), assets)
+    names = sim.body_names()
+    K = 6
+    q0 = np.zeros((K, 6))
+    qd0 = np.zeros((K, 6))
+    forces = np.zeros((K, len(names), 6))
+    for c in range(K):
+        forces[c, names.index("box"), c] = 3.0 if c < 3 else 0.0        # torques
+        forces[c, names.index("box"), c] += 20.0 if c >= 3 else 0.0     # forces
+    sim.reset_stats()
+    out = sim.forward_candidates(q0, qd0, forces, 8)
+    batched = sim.stats()
+    single_evals = 0
+    for c in range(K):
+        sim.set_state_init(q0[c], qd0[c])
+        sim.reset()
+        sim.set_body_external_force("box", forces[c, names.index("box")])
+        sim.reset_stats()
+        sim.forward(8)
+        single_evals += sim.stats()["evaluations"]
+        np.testing.assert_array_equal(out["q"][c], sim.get_q())
+        np.testing.assert_array_equal(out["qdot"][c], sim.get_qdot())
+        assert out["converged"][c] == int(sim.is_converged())
+    print(f"  {K} candidates x 8 steps: {batched['evaluations']} batched evaluations "
+          f"({single_evals} one at a time)", flush=True)
+    assert batched["evaluations"] < single_evals

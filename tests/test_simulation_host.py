@@ -290,3 +290,26 @@ def test_translational_and_fixed_bodies_and_state_api(assets):
         s.get_joint_q("missing")
     with pytest.raises(RuntimeError):   # forward needs reset (Simulation.cpp:918-921)
         R.Simulation(xml, assets).forward(1)
+
+
+def test_candidates_step_as_forward_does(free_box):
+    """forward_candidates: each candidate ends exactly where reset + forward would leave it
+    (no contact here: mesh bodies), and the scene's own state is left as it was."""
+    s, q, qd, ext = free_box
+    rng = np.random.default_rng(5)
+    K = 4
+    qs = np.stack([q + rng.normal(0, 0.1, 6) for _ in range(K)])
+    qds = np.stack([qd + rng.normal(0, 0.5, 6) for _ in range(K)])
+    fs = rng.normal(0, 2.0, (K, 1, 6))
+    s.set_state(q, qd)
+    out = s.forward_candidates(qs, qds, fs, 7)
+    np.testing.assert_array_equal(s.get_q(), q)
+    np.testing.assert_array_equal(s.get_qdot(), qd)
+    for c in range(K):
+        s.set_state_init(qs[c], qds[c])
+        s.reset()
+        s.set_body_external_force("box", fs[c, 0])
+        s.forward(7)
+        np.testing.assert_array_equal(out["q"][c], s.get_q())
+        np.testing.assert_array_equal(out["qdot"][c], s.get_qdot())
+        assert out["converged"][c] == int(s.is_converged())
