@@ -30,6 +30,7 @@ EXPORTED = [
     "atac_batch_eval", "atac_batch_vjp", "atac_batch_graph_capture", "atac_batch_graph_launch",
     "atac_batch_active_ids", "atac_batch_touched_values", "atac_eval_plain", "atac_eval_plain_vjp", "atac_batch_timing",
     "atac_batch_workspace",
+    "atac_batch_live_points",
     "atac_sdf_distance", "atac_sdf_collision",
     "atac_batch_eval_host", "atac_batch_eval_instances_host", "atac_batch_vjp_host", "atac_sdf_distance_host", "atac_sdf_collision_host",
     "atac_sdf_from_mesh", "atac_sdf_mesh_grid",
@@ -57,7 +58,8 @@ class Instance(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("tile_points", C.c_int32), ("detect_block", C.c_int32), ("blocks_per_cu", C.c_int32),
-                ("order", C.c_int32), ("points_per_lane", C.c_int32), ("max_active", C.c_int32)]
+                ("order", C.c_int32), ("points_per_lane", C.c_int32), ("max_active", C.c_int32),
+                ("group_cull", C.c_int32)]
 
 
 class Outputs(C.Structure):
@@ -108,6 +110,7 @@ def lib():
     L.atac_eval_plain_vjp.argtypes = [vp, u32, vp, vp, vp, vp]
     L.atac_batch_timing.argtypes = [vp, i32, vp, vp]
     L.atac_batch_workspace.argtypes = [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i32)]
+    L.atac_batch_live_points.argtypes = [vp, C.POINTER(i64), C.POINTER(i64)]
     L.atac_sdf_distance.argtypes = [vp, dp, C.c_double, vp, i64, vp, vp]
     L.atac_sdf_collision.argtypes = [vp, dp, dp, C.c_double, u32, vp, vp, i64, vp, vp, vp]
     L.atac_batch_eval_host.argtypes = [vp, u32, vp, vp, vp, vp, vp]

@@ -19,7 +19,7 @@ import torch
 
 DEFAULT_CANDIDATES = [
     {"tile_points": tp, "detect_block": blk, "points_per_lane": ppl, "order": order}
-    for order in (1, 3)
+    for order in (1, 4)
     for tp in (1024, 2048, 4096, 8192)
     for blk in (128, 256, 512)
     for ppl in (2, 4)

@@ -167,6 +167,7 @@ class ContactBatch:
         t.order = int(tuning.get("order", 0))
         t.points_per_lane = int(tuning.get("points_per_lane", 0))
         t.max_active = int(tuning.get("max_active", 0))
+        t.group_cull = int(tuning.get("group_cull", 0))
         return t
 
     def close(self):
@@ -188,7 +189,8 @@ class ContactBatch:
         check(lib().atac_batch_info(self.handle, C.byref(n), C.byref(nt), C.byref(pb), C.byref(tu)))
         return {"n_points": int(n.value), "n_tiles": int(nt.value), "persistent_blocks": int(pb.value),
                 "tile_points": tu.tile_points, "detect_block": tu.detect_block, "blocks_per_cu": tu.blocks_per_cu,
-                "order": tu.order, "points_per_lane": tu.points_per_lane, "max_active": tu.max_active}
+                "order": tu.order, "points_per_lane": tu.points_per_lane, "max_active": tu.max_active,
+                "group_cull": tu.group_cull}
 
     def retune(self, tuning):
         self.generation += 1
@@ -309,6 +311,13 @@ class ContactBatch:
         cap, used, ovf = C.c_int64(), C.c_int64(), C.c_int32()
         check(lib().atac_batch_workspace(self.handle, C.byref(cap), C.byref(used), C.byref(ovf)))
         return {"capacity": int(cap.value), "used": int(used.value), "overflowed": bool(ovf.value)}
+
+    def live_points(self):
+        """(points the last evaluation's detection read after group culling, points of the batch).
+        Synchronous."""
+        live, n = C.c_int64(), C.c_int64()
+        check(lib().atac_batch_live_points(self.handle, C.byref(live), C.byref(n)))
+        return int(live.value), int(n.value)
 
     def touched_values(self):
         c = C.c_int64()

@@ -68,6 +68,8 @@ struct atac_points_s {
     std::vector<int64_t> dseg;  // device offset of each segment (multiples of 4: 16-byte aligned)
     int64_t dn = 0;             // device floats per coordinate (segments padded to multiples of 4)
     std::vector<int64_t> perm;  // sorted position -> original index within the segment
+    float* d_gbox = nullptr;    // per group of kGroup sorted points of a segment: lo xyz, 0, hi xyz, 0
+    std::vector<int64_t> gseg;  // first group of each segment
 };
 
 struct atac_batch_s {
@@ -76,6 +78,7 @@ struct atac_batch_s {
         int32_t grid;
         float cth;
         uint8_t* d;
+        int32_t* sat;   // summed-area table of its 2 x 2 x 2 blocks (group culling)
     };
     std::vector<ClsTable> cls;
     int device = 0;
@@ -99,6 +102,8 @@ struct atac_batch_s {
     int64_t slow_cap = 0;             // slow list entries
     int64_t chunk_cap = 0;            // work items if the pool were full
     int32_t* d_tile_active = nullptr;
+    uint32_t* d_tile_live = nullptr;  // per tile: the groups the culling pass keeps
+    bool last_culled = false;         // the last evaluation enqueued ran the culling pass
     int32_t* d_tile_chunk0 = nullptr;
     ChunkDev* d_chunks = nullptr;
     int32_t* d_counters = nullptr;    // CNT_WORDS
@@ -321,9 +326,43 @@ int atac_points_create(const double* xyz, int64_t n, const int64_t* seg_offsets,
             P->perm[b + i] = keys[i].second;
         }
     }
+    // the groups' boxes (NaN for a group with a non-finite coordinate: never dropped)
+    P->gseg.resize(seg.size());
+    int64_t ngr = 0;
+    for (size_t s = 0; s + 1 < seg.size(); ++s) {
+        P->gseg[s] = ngr;
+        ngr += (seg[s + 1] - seg[s] + kGroup - 1) / kGroup;
+    }
+    P->gseg.back() = ngr;
+    std::vector<float> gbox((size_t)ngr * 8, 0.0f);
+    for (size_t s = 0; s + 1 < seg.size(); ++s) {
+        const int64_t m = seg[s + 1] - seg[s], db = P->dseg[s];
+        for (int64_t g = 0; g * kGroup < m; ++g) {
+            float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            bool finite = true;
+            for (int64_t i = g * kGroup; i < std::min<int64_t>(m, (g + 1) * kGroup); ++i) {
+                const float c[3] = {hx[db + i], hy[db + i], hz[db + i]};
+                for (int a = 0; a < 3; ++a) {
+                    finite = finite && std::isfinite(c[a]);
+                    lo[a] = std::min(lo[a], c[a]);
+                    hi[a] = std::max(hi[a], c[a]);
+                }
+            }
+            float* o = &gbox[(size_t)(P->gseg[s] + g) * 8];
+            for (int a = 0; a < 3; ++a) {
+                o[a] = finite ? lo[a] : NAN;
+                o[4 + a] = finite ? hi[a] : NAN;
+            }
+        }
+    }
     int rc = dalloc(&P->d_x, dn);
     if (!rc) rc = dalloc(&P->d_y, dn);
     if (!rc) rc = dalloc(&P->d_z, dn);
+    if (!rc) rc = dalloc(&P->d_gbox, gbox.size());
+    if (!rc && !gbox.empty()) {
+        hipError_t e = hipMemcpy(P->d_gbox, gbox.data(), sizeof(float) * gbox.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = fail(ATAC_ERR_HIP, hipGetErrorString(e));
+    }
     if (!rc && dn > 0) {
         hipError_t e = hipMemcpy(P->d_x, hx.data(), sizeof(float) * dn, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(P->d_y, hy.data(), sizeof(float) * dn, hipMemcpyHostToDevice);
@@ -331,7 +370,7 @@ int atac_points_create(const double* xyz, int64_t n, const int64_t* seg_offsets,
         if (e != hipSuccess) rc = fail(ATAC_ERR_HIP, hipGetErrorString(e));
     }
     if (rc) {
-        dfree(P->d_x); dfree(P->d_y); dfree(P->d_z);
+        dfree(P->d_x); dfree(P->d_y); dfree(P->d_z); dfree(P->d_gbox);
         delete P;
         return rc;
     }
@@ -341,7 +380,7 @@ int atac_points_create(const double* xyz, int64_t n, const int64_t* seg_offsets,
 
 int atac_points_destroy(atac_points p) {
     if (!p) return ATAC_OK;
-    dfree(p->d_x); dfree(p->d_y); dfree(p->d_z);
+    dfree(p->d_x); dfree(p->d_y); dfree(p->d_z); dfree(p->d_gbox);
     delete p;
     return ATAC_OK;
 }
@@ -361,6 +400,7 @@ static void compose_for(atac_batch b, const atac_instance& s, InstDev& d) {
     const int64_t beg = P.dseg[s.segment];
     compose_instance(s, b->grids[s.grid]->dev, P.d_x + beg, P.d_y + beg, P.d_z + beg,
                      P.seg[s.segment + 1] - P.seg[s.segment], d);
+    d.gbox = P.d_gbox ? P.d_gbox + 8 * P.gseg[s.segment] : nullptr;
 }
 
 // A classification table for every (grid, col_th) the instances use, built on
@@ -375,23 +415,27 @@ static int ensure_cls(atac_batch b) {
         while (t < b->cls.size() && !(b->cls[t].grid == d.grid && __builtin_memcmp(&b->cls[t].cth, &cth, sizeof cth) == 0)) ++t;
         if (t == b->cls.size()) {
             const GridDev& G = b->grids[d.grid]->dev;
-            atac_batch_s::ClsTable ct{d.grid, cth, nullptr};
+            atac_batch_s::ClsTable ct{d.grid, cth, nullptr, nullptr};
             int rc = dalloc(&ct.d, (size_t)cls_bytes(G));
-            if (rc) return rc;
+            if (!rc) rc = dalloc(&ct.sat, sat_count(G));
+            if (rc) { dfree(ct.d); return rc; }
             hipError_t e = launch_cls_setup(G, d.col_th, ct.d, nullptr);
+            if (e == hipSuccess) e = launch_sat_setup(G, ct.d, ct.sat, nullptr);
             if (e == hipSuccess) e = hipDeviceSynchronize();
-            if (e != hipSuccess) { dfree(ct.d); return fail(ATAC_ERR_HIP, hipGetErrorString(e)); }
+            if (e != hipSuccess) { dfree(ct.d); dfree(ct.sat); return fail(ATAC_ERR_HIP, hipGetErrorString(e)); }
             b->cls.push_back(ct);
             used.push_back(0);
         }
         used[t] = 1;
         d.cls = b->cls[t].d;
+        d.csat = b->cls[t].sat;
     }
     bool idle = false;
     for (size_t t = b->cls.size(); t-- > 0;) {
         if (used[t]) continue;
         if (!idle) { ATAC_HIP(hipDeviceSynchronize()); idle = true; }   // an earlier evaluation may still read it
         dfree(b->cls[t].d);
+        dfree(b->cls[t].sat);
         b->cls.erase(b->cls.begin() + t);
     }
     return ATAC_OK;
@@ -413,7 +457,7 @@ constexpr int kXcds = 8;   // MI355X: 8 XCDs, each with its own L2
 
 static void release_plan(atac_batch b) {
     dfree(b->d_tiles); dfree(b->d_order); dfree(b->d_tq_begin); dfree(b->d_counters);
-    dfree(b->d_tile_active); dfree(b->d_tile_chunk0); dfree(b->d_chunks);
+    dfree(b->d_tile_active); dfree(b->d_tile_chunk0); dfree(b->d_chunks); dfree(b->d_tile_live);
     dfree(b->d_act); dfree(b->d_rec32); dfree(b->d_rec64); dfree(b->d_slow); dfree(b->d_slow_diag);
     dfree(b->d_part);
     b->part_width = 0;
@@ -459,10 +503,11 @@ static int ensure_buffers(atac_batch b, int mode, bool precise) {
     return rc;
 }
 
-// Points of a tile per wavefront of the detection kernel.
-static int wave_span_of(int tile_points, int block, int ppl) {
-    const int nw = block / 64, step = 64 * ppl;
-    return ((tile_points + nw - 1) / nw + step - 1) / step * step;
+// Output capacity per wavefront of the detection kernel: the wavefronts share
+// a tile's groups, at most ceil(groups / wavefronts) each.
+static int wave_span_of(int tile_points, int block) {
+    const int nw = block / 64, ng = (tile_points + kGroup - 1) / kGroup;
+    return (ng + nw - 1) / nw * kGroup;
 }
 
 static int plan(atac_batch b, const atac_tuning* t) {
@@ -472,16 +517,19 @@ static int plan(atac_batch b, const atac_tuning* t) {
     if (tu.detect_block <= 0) tu.detect_block = 256;
     if (tu.order == 0) tu.order = 1;
     if (tu.points_per_lane <= 0) tu.points_per_lane = 4;
-    if (tu.tile_points > 32768 || tu.tile_points % 64 != 0)
-        return fail(ATAC_ERR_INVALID, "tile_points must be a multiple of 64 and <= 32768");
+    if (tu.tile_points > 32 * kGroup || tu.tile_points % kGroup != 0)
+        return fail(ATAC_ERR_INVALID, "tile_points must be a multiple of " + std::to_string(kGroup) + " and <= " +
+                                          std::to_string(32 * kGroup));
     if (tu.detect_block != 64 && tu.detect_block != 128 && tu.detect_block != 256 && tu.detect_block != 512 &&
         tu.detect_block != 1024)
         return fail(ATAC_ERR_INVALID, "detect_block must be 64, 128, 256, 512 or 1024");
     if (tu.points_per_lane != 1 && tu.points_per_lane != 2 && tu.points_per_lane != 4)
         return fail(ATAC_ERR_INVALID, "points_per_lane must be 1, 2 or 4");
-    if (tu.blocks_per_cu < 0 || tu.order < 0 || tu.order > 3) return fail(ATAC_ERR_INVALID, "bad tuning");
+    if (tu.blocks_per_cu < 0 || tu.order < 0 || tu.order > 4 || tu.group_cull < -1 || tu.group_cull > 1)
+        return fail(ATAC_ERR_INVALID, "bad tuning");
+    if (tu.group_cull == 0) tu.group_cull = 1;
     const int B = tu.detect_block;
-    const int span = wave_span_of(tu.tile_points, B, tu.points_per_lane);
+    const int span = wave_span_of(tu.tile_points, B);
     const size_t lds = detect_lds_per_wave(span, tu.points_per_lane) * (B / 64);
     if (lds + 1024 > b->lds_limit)
         return fail(ATAC_ERR_INVALID, "tile_points " + std::to_string(tu.tile_points) + " with detect_block " +
@@ -526,16 +574,35 @@ static int plan(atac_batch b, const atac_tuning* t) {
     b->chunk_cap = std::min<int64_t>(chunks, b->rec_cap / kChunk + b->tiles.size());
     b->n_tiles = (int32_t)b->tiles.size();
     // tile queues: 1 = one queue in instance order; 2 = one queue grouped by grid;
-    // 3 = grouped by grid and cut into one queue per XCD with balanced point counts
-    b->nq = tu.order == 3 ? kXcds : 1;
+    // 3 = grouped by grid and cut into one queue per XCD with balanced point counts;
+    // 4 = one queue per XCD by the tile's place in its instance's point order: a
+    // point set is Morton-sorted, so the tiles at the same fraction of the instances
+    // of one grid cover about the same part of that grid, and that part's
+    // classification lines and cells stay in one XCD's L2; the eighths are rotated
+    // by the grid index so that every XCD takes a share of every grid
+    b->nq = (tu.order == 3 || tu.order == 4) ? kXcds : 1;
     std::vector<int32_t> order(b->n_tiles);
     std::iota(order.begin(), order.end(), 0);
-    if (tu.order != 1) {
+    std::vector<int32_t> tq(b->nq + 1, b->n_tiles);
+    if (tu.order == 2 || tu.order == 3) {
         auto grid_of_tile = [&](int32_t x) { return b->inst_dev[b->tiles[x].inst].grid; };
         std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return grid_of_tile(x) < grid_of_tile(y); });
     }
-    std::vector<int32_t> tq(b->nq + 1, b->n_tiles);
-    {
+    if (tu.order == 4) {
+        std::vector<int32_t> xcd(b->n_tiles);
+        for (int32_t x = 0; x < b->n_tiles; ++x) {
+            const TileDev& td = b->tiles[x];
+            const InstDev& d = b->inst_dev[td.inst];
+            const double mid = ((double)td.begin + 0.5 * td.count) / (double)d.pt_count;
+            const int eighth = std::min(kXcds - 1, (int)(mid * kXcds));
+            xcd[x] = (eighth + d.grid) % kXcds;
+        }
+        std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return xcd[x] < xcd[y]; });
+        tq[0] = 0;
+        for (int q = 1; q < kXcds; ++q)
+            tq[q] = (int32_t)(std::lower_bound(order.begin(), order.end(), q,
+                                               [&](int32_t x, int q_) { return xcd[x] < q_; }) - order.begin());
+    } else {
         tq[0] = 0;
         double total = 0.0, acc = 0.0;
         for (int32_t it : order) total += (double)b->tiles[it].count;
@@ -550,6 +617,7 @@ static int plan(atac_batch b, const atac_tuning* t) {
     if (!rc) rc = dalloc(&b->d_tq_begin, b->nq + 1);
     if (!rc) rc = dalloc(&b->d_counters, CNT_WORDS);
     if (!rc) rc = dalloc(&b->d_tile_active, b->n_tiles);
+    if (!rc) rc = dalloc(&b->d_tile_live, b->n_tiles);
     if (!rc) rc = dalloc(&b->d_tile_chunk0, b->n_tiles);
     if (!rc) rc = dalloc(&b->d_chunks, (size_t)b->chunk_cap);
     if (rc) return rc;
@@ -642,7 +710,7 @@ int atac_batch_destroy(atac_batch b) {
     for (auto& u : b->ev_used) { (void)hipEventDestroy(u.second.first); (void)hipEventDestroy(u.second.second); }
     for (auto e : b->ev_pool) (void)hipEventDestroy(e);
     dfree(b->d_inst); dfree(b->d_grids);
-    for (auto& ct : b->cls) dfree(ct.d);
+    for (auto& ct : b->cls) { dfree(ct.d); dfree(ct.sat); }
     dfree(b->h_out); dfree(b->h_nact); dfree(b->h_gfm); dfree(b->h_grad);
     if (b->pin) (void)hipHostFree(b->pin);
     if (b->pin_inst) (void)hipHostFree(b->pin_inst);
@@ -739,7 +807,18 @@ static int enqueue_eval(atac_batch b, uint32_t flags, const atac_outputs* out, h
         da.chunk_cap = (int32_t)b->chunk_cap;
         da.tile_active = b->d_tile_active; da.tile_chunk0 = b->d_tile_chunk0; da.chunks = b->d_chunks;
         da.slow = b->d_slow; da.counters = counters(b);
+        // groups whose boxes reach only inactive cells are dropped first (all
+        // points are contacts with ATAC_EVAL_ALL_ACTIVE: nothing is dropped)
+        const bool cull = b->tune.group_cull > 0 && !(flags & ATAC_EVAL_ALL_ACTIVE);
+        da.tile_live = cull ? b->d_tile_live : nullptr;
+        b->last_culled = cull;
         hipEvent_t t0 = stage_begin(b, st);
+        if (cull) {
+            CullArgs cu{};
+            cu.tiles = b->d_tiles; cu.inst = b->d_inst; cu.grids = b->d_grids; cu.tile_live = b->d_tile_live;
+            cu.n_tiles = b->n_tiles;
+            ATAC_HIP(launch_cull(cu, st));
+        }
         ATAC_HIP(launch_detect(da, b->tune.detect_block, b->tune.points_per_lane, b->grid_detect, b->lds_detect, st));
         stage_end(b, ST_DETECT, t0, st);
         SlowArgs sa{};
@@ -1039,6 +1118,25 @@ int atac_batch_workspace(atac_batch b, int64_t* capacity, int64_t* used, int32_t
     if (capacity) *capacity = b->rec_cap;
     if (used) *used = c[CNT_RECS];
     if (overflowed) *overflowed = c[CNT_OVERFLOW] != 0 ? 1 : 0;
+    return ATAC_OK;
+}
+
+int atac_batch_live_points(atac_batch b, int64_t* live_points, int64_t* points) {
+    if (!b) return fail(ATAC_ERR_INVALID, "atac_batch_live_points: null batch");
+    int64_t live = b->n_slots;
+    if (b->last_culled && b->n_tiles > 0) {
+        std::vector<uint32_t> m(b->n_tiles);
+        ATAC_HIP(hipDeviceSynchronize());
+        ATAC_HIP(hipMemcpy(m.data(), b->d_tile_live, sizeof(uint32_t) * m.size(), hipMemcpyDeviceToHost));
+        live = 0;
+        for (int32_t t = 0; t < b->n_tiles; ++t) {
+            const int64_t c = b->tiles[t].count;
+            for (int g = 0; g * (int64_t)kGroup < c; ++g)
+                if ((m[t] >> g) & 1u) live += std::min<int64_t>(kGroup, c - (int64_t)g * kGroup);
+        }
+    }
+    if (live_points) *live_points = live;
+    if (points) *points = b->n_slots;
     return ATAC_OK;
 }
 

@@ -35,11 +35,21 @@ struct DetectArgs {
     int32_t slow_cap;           // entries of the slow list
     int32_t chunk_cap;          // entries of the work-item list
     int32_t pad;
+    const uint32_t* tile_live;  // n_tiles: bit g set when group g of the tile can hold a candidate (null: all)
     int32_t* tile_active;       // n_tiles
     int32_t* tile_chunk0;       // n_tiles: first work item of the tile
     ChunkDev* chunks;           // work items, pushed per tile
     SlowDev* slow;              // slow list
     int32_t* counters;          // CNT_*
+};
+
+struct CullArgs {
+    const TileDev* tiles;
+    const InstDev* inst;
+    const GridDev* grids;
+    uint32_t* tile_live;        // n_tiles
+    int32_t n_tiles;
+    int32_t pad;
 };
 
 struct SlowArgs {
@@ -140,6 +150,9 @@ hipError_t launch_reduce(const ReduceArgs& a, hipStream_t st);
 hipError_t launch_grid_setup(const GridDev& G, float4* cells, hipStream_t st);
 // the cell classification of a grid for one col_th (cls_bytes(G) bytes)
 hipError_t launch_cls_setup(const GridDev& G, double col_th, uint8_t* cls, hipStream_t st);
+// the summed-area table (sat_count(G) entries) of a classification's 2 x 2 x 2 blocks
+hipError_t launch_sat_setup(const GridDev& G, const uint8_t* cls, int32_t* sat, hipStream_t st);
+hipError_t launch_cull(const CullArgs& a, hipStream_t st);
 hipError_t launch_reset(int32_t* c, int n, hipStream_t st);
 hipError_t launch_zero(void* p, int64_t n_words, hipStream_t st);
 hipError_t launch_plain(const PlainArgs& a, int64_t n_points, bool precise, hipStream_t st);

@@ -177,11 +177,25 @@ __global__ __launch_bounds__(BLOCK) void detect_kernel(DetectArgs a) {
         const auto py = as_global(in.py) + t.begin;
         const auto pz = as_global(in.pz) + t.begin;
         uint16_t* q = s_queue + w * span;
-        const int wb = w * span, we = min(t.count, wb + span);
+        // the tile's groups that can hold a candidate (all without the mask); this
+        // wave takes those of ranks [w m / NW, (w + 1) m / NW), in ascending order,
+        // so the waves' output segments follow one another in point order
+        const int ng = (t.count + kGroup - 1) / kGroup;
+        const uint32_t all_groups = ng >= 32 ? 0xffffffffu : ((1u << ng) - 1u);
+        uint32_t mine = a.tile_live ? (as_const(a.tile_live)[tile] & all_groups) : all_groups;
+        {
+            const int wu = __builtin_amdgcn_readfirstlane(w);
+            const int m = __popc(mine), r0 = (wu * m) / NW, r1 = ((wu + 1) * m) / NW;
+            for (int r = 0; r < r0; ++r) mine &= mine - 1u;
+            uint32_t later = mine;
+            for (int r = r0; r < r1; ++r) later &= later - 1u;
+            mine &= ~later;
+        }
         // candidates wait in a ring of CAP entries: [rd, rd + occ) (mod CAP)
         int wcount = 0, rd = 0, occ = 0, npend = 0;   // wave-uniform
         auto ring = [](int i) { return i >= CAP ? i - CAP : i; };   // i < 2 CAP
         const float cth = (float)in.col_th;
+        const ClsConsts kc = cls_consts_vgpr(G, in);
         // A batch: the n <= 64 oldest candidates, one per lane. start() reads
         // them and issues their cell reads; complete() decides them into the
         // output segment. A step starts one at its top, before its own work, and
@@ -213,20 +227,21 @@ __global__ __launch_bounds__(BLOCK) void detect_kernel(DetectArgs a) {
             wcount += __popcll(bal);
             npend = 0;
         };
-        // one step: points (cx, cy, cz) at o0 are tested while the next ones load
-        // into (nx, ny, nz); the loop alternates the two buffers, so no register
-        // copy waits for the loads in flight
-        auto step = [&](int o0, const float (&cx)[PPL], const float (&cy)[PPL], const float (&cz)[PPL],
-                        float (&nx)[PPL], float (&ny)[PPL], float (&nz)[PPL]) {
+        // one step: points (cx, cy, cz) at o0 (valid below we) are tested while
+        // the next step's points (at on, valid below wn) load into (nx, ny, nz);
+        // the loop alternates the two buffers, so no register copy waits for the
+        // loads in flight
+        auto step = [&](int o0, int we, int on, int wn, const float (&cx)[PPL], const float (&cy)[PPL],
+                        const float (&cz)[PPL], float (&nx)[PPL], float (&ny)[PPL], float (&nz)[PPL]) {
             start(occ >= 64 ? 64 : 0);
             unsigned valid = 0;
 #pragma unroll
             for (int k = 0; k < PPL; ++k)
                 if (o0 + 64 * k + lane < we) valid |= 1u << k;
             unsigned op;
-            const unsigned m = detect_candidates<PPL>(G, in, cx, cy, cz, valid, all_active, op, [] {}, [&]() {
+            const unsigned m = detect_candidates<PPL>(G, in, kc, cx, cy, cz, valid, all_active, op, [] {}, [&]() {
                 __builtin_amdgcn_sched_barrier(0);
-                load_points<PPL>(px, py, pz, o0 + 64 * PPL, lane, we, nx, ny, nz);
+                load_points<PPL>(px, py, pz, on, lane, wn, nx, ny, nz);
                 __builtin_amdgcn_sched_barrier(0);
                 complete();   // its cell reads went out before this step's reads
             });
@@ -252,12 +267,38 @@ __global__ __launch_bounds__(BLOCK) void detect_kernel(DetectArgs a) {
                 }
             }
         };
-        float x0[PPL], y0[PPL], z0[PPL], x1[PPL], y1[PPL], z1[PPL];
-        load_points<PPL>(px, py, pz, wb, lane, we, x0, y0, z0);
-        for (int o0 = wb; o0 < we; o0 += 2 * 64 * PPL) {
-            step(o0, x0, y0, z0, x1, y1, z1);
-            if (o0 + 64 * PPL >= we) break;
-            step(o0 + 64 * PPL, x1, y1, z1, x0, y0, z0);
+        // the wave's steps: kGroup / (64 PPL) per group of `mine`, lowest group
+        // first; (g, s) is the current one, advance() moves to the next and says
+        // whether there is one (a step without a next loads point 0, in bounds)
+        if (mine) {
+            constexpr int SP = 64 * PPL, NS = kGroup / SP;
+            auto gend = [&](int gg) { return min(t.count, (gg + 1) * kGroup); };
+            auto advance = [](uint32_t& lv, int& g, int& s) {
+                if (++s < NS) return true;
+                s = 0;
+                lv &= lv - 1u;
+                if (!lv) return false;
+                g = __builtin_ctz(lv);
+                return true;
+            };
+            uint32_t lv = mine;
+            int g = __builtin_ctz(lv), s = 0;
+            float x0[PPL], y0[PPL], z0[PPL], x1[PPL], y1[PPL], z1[PPL];
+            load_points<PPL>(px, py, pz, g * kGroup, lane, gend(g), x0, y0, z0);
+            for (;;) {
+                uint32_t lv1 = lv;
+                int g1 = g, s1 = s;
+                const bool more = advance(lv1, g1, s1);
+                step(g * kGroup + s * SP, gend(g), more ? g1 * kGroup + s1 * SP : 0, more ? gend(g1) : 0, x0, y0, z0,
+                     x1, y1, z1);
+                if (!more) break;
+                lv = lv1; g = g1; s = s1;
+                const bool more2 = advance(lv1, g1, s1);
+                step(g * kGroup + s * SP, gend(g), more2 ? g1 * kGroup + s1 * SP : 0, more2 ? gend(g1) : 0, x1, y1,
+                     z1, x0, y0, z0);
+                if (!more2) break;
+                lv = lv1; g = g1; s = s1;
+            }
         }
         while (occ > 0) {
             start(occ < 64 ? occ : 64);
@@ -661,6 +702,69 @@ hipError_t launch_grid_setup(const GridDev& G, float4* cells, hipStream_t st) {
 
 hipError_t launch_cls_setup(const GridDev& G, double col_th, uint8_t* cls, hipStream_t st) {
     hipLaunchKernelGGL(build_cls_kernel, dim3(1024), dim3(256), 0, st, G, col_th, cls);
+    return hipGetLastError();
+}
+
+// The table's entries: 0 on the lower faces, elsewhere whether block (x-1, y-1, z-1) is open.
+__global__ void sat_mark_kernel(GridDev G, const uint8_t* cls, int32_t* sat) {
+    const uint32_t dx = sat_dim(G, 0), dy = sat_dim(G, 1), dz = sat_dim(G, 2);
+    const int64_t n = (int64_t)dx * dy * dz;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t x = (uint32_t)(e % dx), y = (uint32_t)((e / dx) % dy), z = (uint32_t)(e / ((int64_t)dx * dy));
+        sat[e] = (x && y && z && sat_block_open(G, cls, x - 1, y - 1, z - 1)) ? 1 : 0;
+    }
+}
+
+// Running sums along one axis, one lane per line.
+__global__ void sat_scan_kernel(GridDev G, int32_t* sat, int axis) {
+    const uint32_t d[3] = {sat_dim(G, 0), sat_dim(G, 1), sat_dim(G, 2)};
+    const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+    const int64_t lines = (int64_t)d[a1] * d[a2];
+    const int64_t stride = axis == 0 ? 1 : (axis == 1 ? (int64_t)d[0] : (int64_t)d[0] * d[1]);
+    for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < lines; l += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t c[3];
+        c[axis] = 0;
+        c[a1] = (uint32_t)(l % d[a1]);
+        c[a2] = (uint32_t)(l / d[a1]);
+        int32_t* p = sat + sat_index(G, c[0], c[1], c[2]);
+        int32_t run = 0;
+        for (uint32_t k = 0; k < d[axis]; ++k) {
+            run += p[k * stride];
+            p[k * stride] = run;
+        }
+    }
+}
+
+hipError_t launch_sat_setup(const GridDev& G, const uint8_t* cls, int32_t* sat, hipStream_t st) {
+    hipLaunchKernelGGL(sat_mark_kernel, dim3(512), dim3(256), 0, st, G, cls, sat);
+    for (int axis = 0; axis < 3; ++axis) hipLaunchKernelGGL(sat_scan_kernel, dim3(64), dim3(256), 0, st, G, sat, axis);
+    return hipGetLastError();
+}
+
+// One wavefront per tile, one lane per group of the tile (tiles hold at most 32
+// groups): the tile's live-group mask.
+__global__ __launch_bounds__(256) void cull_kernel(CullArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int tile = (int)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (tile >= a.n_tiles) return;
+    const TileDev t = as_const(a.tiles)[tile];
+    const InstDev in = as_const(a.inst)[t.inst];
+    const GridDev G = as_const(a.grids)[t.grid];
+    const int ng = (t.count + kGroup - 1) / kGroup;
+    bool live = lane < ng;
+    if (live && G.fast && in.cls && in.csat && in.gbox) {
+        const float4* b = reinterpret_cast<const float4*>(in.gbox) + 2 * (t.begin / kGroup + lane);
+        const float4 l = b[0], h = b[1];
+        const float lo[3] = {l.x, l.y, l.z}, hi[3] = {h.x, h.y, h.z};
+        live = group_live(G, in, in.csat, lo, hi);
+    }
+    const uint64_t m = __ballot(live);
+    if (lane == 0) a.tile_live[tile] = (uint32_t)m;
+}
+
+hipError_t launch_cull(const CullArgs& a, hipStream_t st) {
+    if (a.n_tiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(cull_kernel, dim3((a.n_tiles + 3) / 4), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 

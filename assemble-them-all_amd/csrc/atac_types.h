@@ -57,7 +57,11 @@ struct InstDev {
     // per-cell classification of the instance's grid for its col_th (cls_fill;
     // bricked, cls_offset): 0 no active point, 1 only active points, 2 open
     const uint8_t* cls;
-    double dpad[3];
+    // the summed-area table of that classification's blocks (sat_*), and the
+    // boxes of the instance's groups of kGroup points (8 floats each: lo, 0, hi, 0)
+    const int32_t* csat;
+    const float* gbox;
+    double dpad;
     float fpad[4];
 };
 static_assert(sizeof(InstDev) == 704, "InstDev layout");

@@ -492,18 +492,143 @@ ATAC_HD bool detect_interp(const GridDev& G, const InstDev& in, float px, float 
     return cell_value(q[0], q[1], fx, fy, fz, (float)in.col_th) < 0.0f;
 }
 
+// Groups and their culling. A point set's segment is cut into groups of kGroup
+// consecutive (Morton-sorted) points, each with the box of its FP32 coordinates.
+// Before detection, a group whose box can only reach inactive cells of the
+// classification is dropped: every one of its points would read an inactive
+// entry, so the decisions are the same without it. The test counts the
+// non-inactive blocks of 2 x 2 x 2 padded cells over the box's cell range with a
+// summed-area table: S(x, y, z) (x, y, z from 0 to the block counts, x fastest)
+// is the number of such blocks with indices below (x, y, z), zero on the lower faces.
+constexpr int kGroup = 256;
+ATAC_HD uint32_t sat_dim(const GridDev& G, int a) { return ((uint32_t)cls_pad(G, a) + 1u) / 2u + 1u; }
+ATAC_HD size_t sat_count(const GridDev& G) { return (size_t)sat_dim(G, 0) * sat_dim(G, 1) * sat_dim(G, 2); }
+ATAC_HD size_t sat_index(const GridDev& G, uint32_t x, uint32_t y, uint32_t z) {
+    return ((size_t)z * sat_dim(G, 1) + y) * sat_dim(G, 0) + x;
+}
+
+// Is block (bx, by, bz) of the padded grid not inactive (any of its cells)?
+ATAC_HD bool sat_block_open(const GridDev& G, const uint8_t* cls, uint32_t bx, uint32_t by, uint32_t bz) {
+    bool any = false;
+    for (uint32_t c = 0; c < 8; ++c) {
+        const uint32_t i = 2 * bx + (c & 1u), j = 2 * by + ((c >> 1) & 1u), k = 2 * bz + (c >> 2);
+        if (i < (uint32_t)cls_pad(G, 0) && j < (uint32_t)cls_pad(G, 1) && k < (uint32_t)cls_pad(G, 2))
+            any = any || cls[cls_entry(G, i, j, k)] != kClsInactive;
+    }
+    return any;
+}
+
+// Can a point in the box [lo, hi] (body frame) read a classification entry that
+// is not inactive? Over the box, coordinate a of the map c' = A x + bs spans
+// [bs_a + sum_j min(A_aj lo_j, A_aj hi_j), bs_a + sum_j max(A_aj lo_j, A_aj hi_j)];
+// widened by 1e-12 of the terms' magnitude (a point's own FP64 map and this
+// interval are each within a few roundings, under 1e-15 of it, of the exact
+// values), then truncated and clamped to the padded grid as cls_offset does, it
+// holds the cell of every point of the box. The box is live when a block over
+// those cells is not inactive, and whenever an input or the interval is not
+// finite.
+ATAC_HD bool group_block_range(const GridDev& G, const InstDev& in, const float lo[3], const float hi[3],
+                               uint32_t b0[3], uint32_t b1[3]) {
+    for (int a = 0; a < 3; ++a) {
+        double sl = in.bs[a], sh = in.bs[a], mag = fabs(in.bs[a]);
+        for (int j = 0; j < 3; ++j) {
+            if (!(fabs((double)lo[j]) <= 3.4e38) || !(fabs((double)hi[j]) <= 3.4e38)) return false;
+            const double p = in.A[3 * a + j] * (double)lo[j], q = in.A[3 * a + j] * (double)hi[j];
+            sl += p < q ? p : q;
+            sh += p < q ? q : p;
+            mag += fmax(fabs(p), fabs(q));
+        }
+        const double eps = 1e-12 * mag;
+        const double cl = floor(sl - eps), ch = floor(sh + eps);
+        if (!(fabs(cl) <= 1e300) || !(fabs(ch) <= 1e300)) return false;   // NaN or inf
+        const double top = (double)(G.n[a] + 1);
+        const uint32_t i0 = (uint32_t)(cl < 0.0 ? 0.0 : (cl > top ? top : cl));
+        const uint32_t i1 = (uint32_t)(ch < 0.0 ? 0.0 : (ch > top ? top : ch));
+        b0[a] = i0 >> 1;   // blocks [b0, b1)
+        b1[a] = (i1 >> 1) + 1u;
+    }
+    return true;
+}
+
+// Non-inactive blocks in [b0, b1) (per axis) by the summed-area table.
+ATAC_HD int32_t sat_range_count(const GridDev& G, const int32_t* sat, const uint32_t b0[3], const uint32_t b1[3]) {
+    return sat[sat_index(G, b1[0], b1[1], b1[2])] - sat[sat_index(G, b0[0], b1[1], b1[2])] -
+           sat[sat_index(G, b1[0], b0[1], b1[2])] - sat[sat_index(G, b1[0], b1[1], b0[2])] +
+           sat[sat_index(G, b0[0], b0[1], b1[2])] + sat[sat_index(G, b0[0], b1[1], b0[2])] +
+           sat[sat_index(G, b1[0], b0[1], b0[2])] - sat[sat_index(G, b0[0], b0[1], b0[2])];
+}
+
+ATAC_HD bool group_live(const GridDev& G, const InstDev& in, const int32_t* sat, const float lo[3], const float hi[3]) {
+    uint32_t b0[3], b1[3];
+    if (!group_block_range(G, in, lo, hi, b0, b1)) return true;
+    return sat_range_count(G, sat, b0, b1) != 0;
+}
+
+// The values cls_offset takes besides the map's matrix: the map's addends, the
+// padded grid's last cell index per axis and the brick counts along x and y.
+struct ClsConsts {
+    double b[3];
+    int32_t hi[3];
+    uint32_t nb0, nb1;
+};
+
+ATAC_HD ClsConsts cls_consts(const GridDev& G, const InstDev& in) {
+    ClsConsts c;
+    for (int a = 0; a < 3; ++a) {
+        c.b[a] = in.bs[a];
+        c.hi[a] = G.n[a] + 1;
+    }
+    c.nb0 = cls_bricks(G, 0);
+    c.nb1 = cls_bricks(G, 1);
+    return c;
+}
+
+// cls_consts in vector registers, for a loop over many points: a VOP3
+// instruction reads at most one scalar operand, so uniform values left in scalar
+// registers are copied into vector registers at every use.
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ double in_vgpr(double v) {
+    double r;
+    asm volatile("v_mov_b64 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+}
+__device__ __forceinline__ uint32_t in_vgpr(uint32_t v) {
+    uint32_t r;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+}
+__device__ __forceinline__ ClsConsts cls_consts_vgpr(const GridDev& G, const InstDev& in) {
+    ClsConsts c = cls_consts(G, in);
+    for (int a = 0; a < 3; ++a) {
+        c.b[a] = in_vgpr(c.b[a]);
+        c.hi[a] = (int32_t)in_vgpr((uint32_t)c.hi[a]);
+    }
+    c.nb0 = in_vgpr(c.nb0);
+    c.nb1 = in_vgpr(c.nb1);
+    return c;
+}
+#else
+ATAC_HD ClsConsts cls_consts_vgpr(const GridDev& G, const InstDev& in) { return cls_consts(G, in); }
+#endif
+
 // The table byte offset of a point: its grid coordinates by the shifted map
 // c' = A xi + (b + 1), truncated per axis (a negative or NaN coordinate gives
-// 0, the outside layer) and clamped to the padded grid.
-ATAC_HD uint32_t cls_offset(const GridDev& G, const InstDev& in, float px, float py, float pz) {
+// 0, the outside layer) and clamped to the padded grid; the brick layout of
+// cls_entry.
+ATAC_HD uint32_t cls_offset(const InstDev& in, const ClsConsts& kc, float px, float py, float pz) {
     const double x = px, y = py, z = pz;
-    const double c0 = fma(in.A[2], z, fma(in.A[1], y, fma(in.A[0], x, in.bs[0])));
-    const double c1 = fma(in.A[5], z, fma(in.A[4], y, fma(in.A[3], x, in.bs[1])));
-    const double c2 = fma(in.A[8], z, fma(in.A[7], y, fma(in.A[6], x, in.bs[2])));
-    const uint32_t i = (uint32_t)med3_i32(cvt_sat_i32(c0), G.n[0] + 1);
-    const uint32_t j = (uint32_t)med3_i32(cvt_sat_i32(c1), G.n[1] + 1);
-    const uint32_t k = (uint32_t)med3_i32(cvt_sat_i32(c2), G.n[2] + 1);
-    return cls_entry(G, i, j, k);
+    const double c0 = fma(in.A[2], z, fma(in.A[1], y, fma(in.A[0], x, kc.b[0])));
+    const double c1 = fma(in.A[5], z, fma(in.A[4], y, fma(in.A[3], x, kc.b[1])));
+    const double c2 = fma(in.A[8], z, fma(in.A[7], y, fma(in.A[6], x, kc.b[2])));
+    const uint32_t i = (uint32_t)med3_i32(cvt_sat_i32(c0), kc.hi[0]);
+    const uint32_t j = (uint32_t)med3_i32(cvt_sat_i32(c1), kc.hi[1]);
+    const uint32_t k = (uint32_t)med3_i32(cvt_sat_i32(c2), kc.hi[2]);
+    const uint32_t brick = mad24(mad24(k >> 2, kc.nb1, j >> 2), kc.nb0, i >> 2);
+    return (brick << 6) | ((k & 3u) << 4) | ((j & 3u) << 2) | (i & 3u);
+}
+
+ATAC_HD uint32_t cls_offset(const GridDev& G, const InstDev& in, float px, float py, float pz) {
+    return cls_offset(in, cls_consts(G, in), px, py, pz);
 }
 
 // The detection decision detect_value(...) < 0, bit for bit, through the cell
@@ -620,12 +745,12 @@ inline uint32_t cls_load(const uint8_t* tab, uint32_t bytes, uint32_t off) { ret
 // settled candidate and nothing is read; on a grid without the table every valid
 // point is open.
 template <int PPL, typename F0, typename F>
-ATAC_HD unsigned detect_candidates(const GridDev& G, const InstDev& in, const float (&x)[PPL], const float (&y)[PPL],
-                                   const float (&z)[PPL], unsigned valid, bool all_active, unsigned& open,
-                                   F0&& before, F&& issue) {
+ATAC_HD unsigned detect_candidates(const GridDev& G, const InstDev& in, const ClsConsts& kc, const float (&x)[PPL],
+                                   const float (&y)[PPL], const float (&z)[PPL], unsigned valid, bool all_active,
+                                   unsigned& open, F0&& before, F&& issue) {
     uint32_t boff[PPL];
 #pragma unroll
-    for (int k = 0; k < PPL; ++k) boff[k] = ((valid >> k) & 1u) ? cls_offset(G, in, x[k], y[k], z[k]) : 0u;
+    for (int k = 0; k < PPL; ++k) boff[k] = ((valid >> k) & 1u) ? cls_offset(in, kc, x[k], y[k], z[k]) : 0u;
     sched_fence();   // the points are in: the reads below wait for nothing earlier
     before();
     uint32_t cl[PPL];

@@ -39,6 +39,8 @@ ACT_BYTES = 4                                       # active-list entry (point o
 CHUNK_BYTES = 16                                    # work-item entry, one per kChunk active points
 RECORD_BYTES = 48                                   # FP32 active-contact record
 K_CHUNK = 256
+GROUP = 256                                         # points per culling group
+GROUP_BYTES = 32 + 8 * 4                            # its box, and the 8 summed-area entries of its test
 
 
 def log(*a):
@@ -418,8 +420,10 @@ def main():
                     help="skip the mesh -> SDF build and BDF1 stepping measurements (profiling runs)")
     ap.add_argument("--tile-points", type=int, default=0)
     ap.add_argument("--detect-block", type=int, default=0)
-    ap.add_argument("--order", type=int, default=0, help="tile queues: 0 default, 1 instance order, 2 grid-grouped, 3 per XCD")
+    ap.add_argument("--order", type=int, default=0, help="tile queues: 0 default, 1 instance order, 2 grid-grouped, 3 per XCD by grid, "
+                                                                 "4 per XCD by place in the instance")
     ap.add_argument("--points-per-lane", type=int, default=0)
+    ap.add_argument("--group-cull", type=int, default=0, help="0 default (on), -1 detect every point")
     ap.add_argument("--multipart-steps", type=int, default=100, help="timed steps of the multi-part step loop")
     ap.add_argument("--sort-cell", type=float, default=0.0,
                     help="Morton sort granularity of the point sets (0: extent/1024; 1e30: generated order)")
@@ -439,7 +443,7 @@ def main():
         f"{time.perf_counter() - t0:.1f}s")
     t0 = time.perf_counter()
     tuning = {"tile_points": args.tile_points, "detect_block": args.detect_block, "order": args.order,
-              "points_per_lane": args.points_per_lane}
+              "points_per_lane": args.points_per_lane, "group_cull": args.group_cull}
     b = batch_from_workload(w, tuning=tuning, sort_cell=args.sort_cell)
     info = b.info()
     log(f"batch ready in {time.perf_counter() - t0:.1f}s: {info}")
@@ -462,7 +466,9 @@ def main():
     torch.cuda.synchronize()
     ms = start.elapsed_time(stop) / args.steps
     n_active = int(out["n_active"].sum().item())
-    log(f"step {ms:.3f} ms, {n_q / ms * 1e3:.3e} queries/s, active {n_active} ({n_active / n_q:.2%})")
+    n_live, _ = b.live_points()
+    log(f"step {ms:.3f} ms, {n_q / ms * 1e3:.3e} queries/s, active {n_active} ({n_active / n_q:.2%}), "
+        f"points read after group culling {n_live} ({n_live / n_q:.2%})")
 
     # per-kernel durations with HIP events on the library's launch stream (eager launches)
     b.timing(True)
@@ -504,12 +510,17 @@ def main():
     log(f"end to end from host memory: {e2e_ms:.3f} ms per step")
 
     touched = b.touched_values()
-    # detection: every point read once (12 B), every grid value its stencils touch
-    # once (4 B), the instance fields and tile entries read, and the active list
-    # (one offset per active point) and its work items written
+    # detection (the culling pass and the detection kernel): every group's box read
+    # (32 B) and its summed-area counts (8 x 4 B) when culling, every point of the
+    # kept groups read once (12 B), every grid value its stencils touch once (4 B),
+    # the instance fields and tile entries read, and the active list (one offset per
+    # active point) and its work items written
+    n_groups = sum(-(-len(w.point_sets[ins.points]) // GROUP) for ins in w.instances)
+    culled = n_live < n_q or args.group_cull >= 0
     n_chunks_lb = -(-n_active // K_CHUNK)
-    det_bytes = (12 * n_q + 4 * touched + DETECT_INST_BYTES * b.n + TILE_BYTES * info["n_tiles"]
-                 + ACT_BYTES * n_active + CHUNK_BYTES * n_chunks_lb)
+    det_bytes = (12 * n_live + (GROUP_BYTES * n_groups + 4 * info["n_tiles"] if culled else 0) + 4 * touched
+                 + DETECT_INST_BYTES * b.n + TILE_BYTES * info["n_tiles"] + ACT_BYTES * n_active
+                 + CHUNK_BYTES * n_chunks_lb)
     det_ms = stage_ms["detect"]
     achieved = det_bytes / (det_ms * 1e-3) / 1e9
     # the step: detection; the contact pass reads the active list and points and
@@ -537,11 +548,14 @@ def main():
     # value once, per instance its poses, twists and parameters (408 B) and g_fm (96 B)
     # read and its wrench (96 B) and gradients (208 B) written, and the adjoint's
     # checkpoint of d and grad phi (16 B) per active contact
-    step_alg = 12 * n_q + 4 * touched + (408 + 96 + 96 + 208) * b.n + 16 * n_active
+    step_alg = (12 * n_live + (GROUP_BYTES * n_groups if culled else 0) + 4 * touched + (408 + 96 + 96 + 208) * b.n
+                + 16 * n_active)
     step_gbs = step_alg / (ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
-                "kernel": "detect_kernel", "algorithmic_bytes_per_launch": int(det_bytes),
+                "kernel": "cull_kernel + detect_kernel" if culled else "detect_kernel",
+                "algorithmic_bytes_per_launch": int(det_bytes),
+                "points_read": n_live, "groups": n_groups,
                 "touched_grid_values": touched, "launch_ms": round(det_ms, 4),
                 "measured_copy_GBps": round(copy_gbs, 1), "frac_of_measured_copy": round(achieved / copy_gbs, 4),
                 "step_algorithmic_bytes": int(step_alg), "step_achieved": round(step_gbs, 1),
@@ -596,8 +610,9 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": w.name, "instances": b.n, "queries": n_q, "grids": f"{len(w.grids)} x "
                    f"{'x'.join(str(v) for v in w.grids[0].n)} FP32", "active_fraction": round(n_active / n_q, 5),
-                   "step": "detect (cell classification, active list) + slow-path records + contact wrench "
-                           "(records built) + reduce + adjoint (VJP) + reduce, HIP graph replay",
+                   "step": "group culling + detect (cell classification, active list) + slow-path records + "
+                           "contact wrench (records built) + reduce + adjoint (VJP) + reduce, HIP graph replay",
+                   "group_cull": info["group_cull"], "points_read_fraction": round(n_live / n_q, 5),
                    "tile_points": info["tile_points"], "detect_block": info["detect_block"],
                    "points_per_lane": info["points_per_lane"],
                    "persistent_blocks": info["persistent_blocks"], "n_cu": n_cu, "parallelism": "none"},

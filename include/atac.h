@@ -53,18 +53,23 @@ typedef struct {
 } atac_instance;
 
 typedef struct {
-    int32_t tile_points;   /* points per tile (0: 4096); a multiple of 64, at most 32768 */
+    int32_t tile_points;   /* points per tile (0: 4096); a multiple of 256, at most 8192 */
     int32_t detect_block;  /* threads per detection workgroup: 64, 128, 256, 512, 1024 (0: 256) */
     int32_t blocks_per_cu; /* detection: 0 = one workgroup per tile (default); n > 0 = n persistent
                               workgroups per CU (at most the occupancy limit) walking the tiles */
     int32_t order;         /* tile queues: 0 default (1); 1 one queue in instance order;
-                              2 one queue grouped by grid; 3 one queue per XCD, grouped by grid */
+                              2 one queue grouped by grid; 3 one queue per XCD, grouped by grid;
+                              4 one queue per XCD by the tile's place in its instance's Morton-sorted
+                              points (the same part of a grid on one XCD, every grid on every XCD) */
     int32_t points_per_lane; /* points a detection lane takes per step (64 apart): 1, 2, 4 (0: 4) */
     int32_t max_active;    /* bound on the active contacts of one evaluation (the record pool, 48 B each;
                               96 B with ATAC_EVAL_PRECISE): 0 = one per point up to 4M points, a quarter of
                               the points beyond; > 0 that many; < 0 one per point (needed by
                               ATAC_EVAL_ALL_ACTIVE). An evaluation with more contacts gives NaN outputs and
                               n_active -1, and atac_batch_workspace reports the overflow */
+    int32_t group_cull;    /* 0 default (1); 1: before detection, drop the groups of 256 consecutive sorted
+                              points whose bounding box maps only to cells where no point can be active
+                              (the decisions are unchanged); -1: detect every point */
 } atac_tuning;
 
 /* flags of atac_batch_eval / atac_eval_plain */
@@ -163,6 +168,11 @@ int atac_batch_graph_launch(atac_batch b, void* stream);
  * capacity) and overflowed (1: its outputs are NaN, its adjoint refused; plan the
  * batch again with a larger max_active). Any output may be NULL. */
 int atac_batch_workspace(atac_batch b, int64_t* capacity, int64_t* used, int32_t* overflowed);
+
+/* The points the last evaluation's detection read (live_points: those of the groups the
+ * culling pass kept; all of them without culling) and the batch's points (synchronous).
+ * No counterpart in the reference: a diagnostic of this library's group culling. */
+int atac_batch_live_points(atac_batch b, int64_t* live_points, int64_t* points);
 
 /* Per-stage kernel timing with HIP events recorded on the launch stream around
  * each kernel of atac_batch_eval / atac_batch_vjp (not inside captured graphs).

@@ -163,6 +163,67 @@ int hc_detect(const float* vals, const int32_t n[3], const double minb[3], const
     return 0;
 }
 
+// Group culling on the host: the summed-area table built as the device builds it
+// (block marks, then running sums along x, y, z), and per group of `gsize`
+// consecutive points: live[g] = group_live; needed[g] = some point of the group
+// reads a table entry that is not inactive; exact[g] = the table's block count
+// over the group's range equals the count of open blocks found one by one (1 when
+// the range is not finite).
+int hc_group_cull(const float* vals, const int32_t n[3], const double minb[3], const double maxb[3],
+                  const double d[3], const float* px, const float* py, const float* pz, int64_t npts,
+                  const atac_instance* inst, int gsize, uint8_t* live, uint8_t* needed, uint8_t* exact) {
+    HostGrid hg;
+    host_grid(vals, n, minb, maxb, d, hg);
+    const GridDev& G = hg.G;
+    InstDev in;
+    std::memset(&in, 0, sizeof in);
+    compose_instance(*inst, G, px, py, pz, npts, in);
+    host_cls(hg, in);
+    const uint32_t dim[3] = {sat_dim(G, 0), sat_dim(G, 1), sat_dim(G, 2)};
+    std::vector<int32_t> sat(sat_count(G), 0);
+    for (uint32_t z = 1; z < dim[2]; ++z)
+        for (uint32_t y = 1; y < dim[1]; ++y)
+            for (uint32_t x = 1; x < dim[0]; ++x)
+                sat[sat_index(G, x, y, z)] = sat_block_open(G, in.cls, x - 1, y - 1, z - 1) ? 1 : 0;
+    for (uint32_t z = 0; z < dim[2]; ++z)
+        for (uint32_t y = 0; y < dim[1]; ++y)
+            for (uint32_t x = 1; x < dim[0]; ++x) sat[sat_index(G, x, y, z)] += sat[sat_index(G, x - 1, y, z)];
+    for (uint32_t z = 0; z < dim[2]; ++z)
+        for (uint32_t y = 1; y < dim[1]; ++y)
+            for (uint32_t x = 0; x < dim[0]; ++x) sat[sat_index(G, x, y, z)] += sat[sat_index(G, x, y - 1, z)];
+    for (uint32_t z = 1; z < dim[2]; ++z)
+        for (uint32_t y = 0; y < dim[1]; ++y)
+            for (uint32_t x = 0; x < dim[0]; ++x) sat[sat_index(G, x, y, z)] += sat[sat_index(G, x, y, z - 1)];
+    for (int64_t g = 0; g * gsize < npts; ++g) {
+        const int64_t o0 = g * gsize, o1 = std::min<int64_t>(npts, o0 + gsize);
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        bool finite = true, need = false;
+        for (int64_t o = o0; o < o1; ++o) {
+            const float c[3] = {px[o], py[o], pz[o]};
+            for (int a = 0; a < 3; ++a) {
+                finite = finite && std::isfinite(c[a]);
+                lo[a] = std::min(lo[a], c[a]);
+                hi[a] = std::max(hi[a], c[a]);
+            }
+            need = need || in.cls[cls_offset(G, in, px[o], py[o], pz[o])] != kClsInactive;
+        }
+        if (!finite)
+            for (int a = 0; a < 3; ++a) lo[a] = hi[a] = NAN;
+        live[g] = group_live(G, in, sat.data(), lo, hi) ? 1 : 0;
+        needed[g] = need ? 1 : 0;
+        uint32_t b0[3], b1[3];
+        exact[g] = 1;
+        if (group_block_range(G, in, lo, hi, b0, b1)) {
+            int32_t brute = 0;
+            for (uint32_t z = b0[2]; z < b1[2]; ++z)
+                for (uint32_t y = b0[1]; y < b1[1]; ++y)
+                    for (uint32_t x = b0[0]; x < b1[0]; ++x) brute += sat_block_open(G, in.cls, x, y, z) ? 1 : 0;
+            exact[g] = brute == sat_range_count(G, sat.data(), b0, b1) ? 1 : 0;
+        }
+    }
+    return 0;
+}
+
 // The bounded detection decision (block bounds, corner-packed cells) next to the
 // full interpolation's: decided[o] = detect_active, full[o] = detect_value < 0,
 // settled[o] = 1 when the block bound alone settled the point.
@@ -189,7 +250,7 @@ int hc_detect_decisions(const float* vals, const int32_t n[3], const double minb
             if (o0 + k < npts) valid |= 1u << k;
         }
         unsigned open;
-        const unsigned cand = detect_candidates<4>(G, in, x, y, z, valid, false, open, [] {}, [] {});
+        const unsigned cand = detect_candidates<4>(G, in, cls_consts(G, in), x, y, z, valid, false, open, [] {}, [] {});
         for (int k = 0; k < 4 && o0 + k < npts; ++k) {
             bool a = false, s = false;
             if ((cand >> k) & 1u) a = candidate_active(G, in, x[k], y[k], z[k], ((open >> k) & 1u) != 0u, s);

@@ -109,3 +109,18 @@ def detect_decisions(grid, pts_f32, inst):
                               _vp(slow), _vp(slow_ref))
     return (dec.astype(bool), full.astype(bool), settled.astype(bool), batched.astype(bool), slow.astype(bool),
             slow_ref.astype(bool))
+
+
+def group_cull(grid, pts_f32, inst, gsize=256):
+    """Per group of gsize consecutive points: (group_live, a point of it reads a
+    non-inactive table entry, the summed-area count equals the brute-force count)."""
+    vals, n, mn, mx, d = _grid_args(grid)
+    p = np.ascontiguousarray(pts_f32, dtype=np.float32).reshape(-1, 3)
+    px, py, pz = [np.ascontiguousarray(p[:, k]) for k in range(3)]
+    ng = (len(p) + gsize - 1) // gsize
+    live = np.zeros(ng, dtype=np.uint8)
+    needed = np.zeros(ng, dtype=np.uint8)
+    exact = np.zeros(ng, dtype=np.uint8)
+    lib().hc_group_cull(_vp(vals), _vp(n), _vp(mn), _vp(mx), _vp(d), _vp(px), _vp(py), _vp(pz), C.c_int64(len(p)),
+                        C.byref(inst), C.c_int(gsize), _vp(live), _vp(needed), _vp(exact))
+    return live.astype(bool), needed.astype(bool), exact.astype(bool)

@@ -99,3 +99,32 @@ def test_full_size_tuning_invariance(full):
             assert np.all(np.abs(o["fm"] - base["fm"]).max(axis=1) <= 1e-3 * scale + 1e-6)
     finally:
         b.retune({})
+
+
+def test_full_size_group_culling_changes_nothing(full):
+    """Dropping the groups whose boxes reach only inactive cells leaves every output
+    of the evaluation and of its adjoint unchanged, bit for bit (the tiles' active
+    lists are the same, in the same order); on this workload it drops a large share
+    of the points."""
+    w, b = full
+    g = torch.tensor(np.random.default_rng(9).normal(size=(b.n, 12)), device="cuda")
+    on = _np(b.eval(deriv=True))
+    von = _np(b.vjp(g))
+    live, total = b.live_points()
+    ids_on = [b.active_ids(i) for i in range(0, b.n, 97)]
+    try:
+        b.retune({"group_cull": -1})
+        off = _np(b.eval(deriv=True))
+        voff = _np(b.vjp(g))
+        live_off, _ = b.live_points()
+        ids_off = [b.active_ids(i) for i in range(0, b.n, 97)]
+    finally:
+        b.retune({})
+    print(f"  culling read {live} of {total} points ({live / total:.3f})", flush=True)
+    assert live_off == total and 0 < live < 0.9 * total
+    for k in on:
+        assert np.array_equal(on[k], off[k]), k
+    for k in von:
+        assert np.array_equal(von[k], voff[k]), k
+    for a, c in zip(ids_on, ids_off):
+        assert np.array_equal(a, c)

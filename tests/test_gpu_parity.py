@@ -107,10 +107,11 @@ def test_tuning_invariance(peg, peg_batch):
     try:
         for tu in ({"tile_points": 256, "detect_block": 128, "points_per_lane": 1},
                    {"tile_points": 1024, "detect_block": 1024, "points_per_lane": 2},
-                   {"tile_points": 16384, "detect_block": 512, "order": 1}, {"tile_points": 4096, "blocks_per_cu": 1},
+                   {"tile_points": 8192, "detect_block": 512, "order": 1}, {"tile_points": 4096, "blocks_per_cu": 1},
                    {"order": 2, "points_per_lane": 1}, {"order": 3, "tile_points": 2048, "detect_block": 128},
-                   {"tile_points": 16384, "detect_block": 1024},
-                   {"tile_points": 32768, "detect_block": 512},
+                   {"order": 4}, {"order": 4, "tile_points": 1024, "points_per_lane": 2},
+                   {"tile_points": 8192, "detect_block": 1024}, {"tile_points": 768, "detect_block": 64},
+                   {"group_cull": -1}, {"group_cull": 1, "tile_points": 512, "detect_block": 1024},
                    {"tile_points": 2048, "blocks_per_cu": 2}):
             peg_batch.retune(tu)
             o = _np(peg_batch.eval(p6=True))
@@ -121,7 +122,8 @@ def test_tuning_invariance(peg, peg_batch):
                     assert rel_err(o["fm"][i], base["fm"][i], r["scale_fm"]) <= 1e-6
         # a tuning that does not fit is refused and leaves the batch usable
         for bad in ({"tile_points": 100000}, {"tile_points": 1000}, {"points_per_lane": 3},
-                    {"tile_points": 65536}, {"order": 4}):
+                    {"tile_points": 65536}, {"tile_points": 16384}, {"tile_points": 320}, {"order": 5},
+                    {"group_cull": 2}):
             with pytest.raises(atac.AtacError):
                 peg_batch.retune(bad)
         o = _np(peg_batch.eval(p6=True))

@@ -219,3 +219,78 @@ def test_bounded_detection_box_inside_and_beyond_grid():
         g = W.SDFGrid(vals, np.array([lo] * 3), np.array([hi] * 3), np.array([dx] * 3))
         settled = _decisions_agree(g, pts, ins)
         assert 0.0 < settled.mean() < 1.0
+
+
+def _morton_order(p):
+    lo = p.min(0)
+    ext = max(float((p.max(0) - lo).max()), 1e-30)
+    q = np.minimum(((p - lo) / ext * 1024).astype(np.int64), 1023)
+    key = np.zeros(len(p), dtype=np.int64)
+    for b in range(10):
+        for a in range(3):
+            key |= ((q[:, a] >> b) & 1) << (3 * b + a)
+    return np.argsort(key, kind="stable")
+
+
+def _cull_checks(grid, pts, inst, gsize=256):
+    live, needed, exact = H.group_cull(grid, pts, inst, gsize)
+    assert exact.all(), f"{int((~exact).sum())} summed-area counts differ from the block-by-block count"
+    assert not np.any(needed & ~live), f"{int((needed & ~live).sum())} groups dropped with a non-inactive cell"
+    return live, needed
+
+
+def test_group_culling_keeps_every_group_that_can_hold_a_candidate(peg):
+    """Group culling on real poses (Morton-sorted groups of 256, as the library cuts
+    them, and of 16): a group is dropped only when every point of it reads an
+    inactive entry, and the summed-area table's counts are the blocks' own."""
+    w, _ = peg
+    pts = W.round_points_f32(w.point_sets[0]).astype(np.float32)
+    pts = pts[_morton_order(pts)]
+    dropped = []
+    for ins in w.instances[:16]:
+        _cull_checks(w.grids[0], pts, _ai(ins), 256)
+        # the peg lies along the hole's surface: groups of 16 points are needed to drop some
+        live, needed = _cull_checks(w.grids[0], pts, _ai(ins), 16)
+        dropped.append(1.0 - live.mean())
+    print(f"  dropped groups of 16: mean {np.mean(dropped):.3f}")
+    assert 0.0 < np.mean(dropped) < 1.0
+
+
+def test_group_culling_adversarial():
+    """Fields at or within a few ulps of the threshold, non-finite nodes, points on
+    cell faces and box faces, outside the box, and groups holding a non-finite point
+    (never dropped); groups of 1, 7 and 256 points."""
+    rng = np.random.default_rng(7)
+    col_th = 0.01
+    base = np.float32(-col_th)
+    n = 9
+    lo, hi = -1.0, 1.0
+    dxyz = (hi - lo) / (n - 1)
+    zs = np.zeros(6)
+    # points on the cell faces (multiples of the spacing), inside, outside the box
+    faces = (np.round(rng.uniform(lo, hi, (900, 3)) / dxyz) * dxyz)
+    inside = rng.uniform(lo, hi, (1200, 3))
+    outside = rng.uniform(2 * lo, 2 * hi, (600, 3))
+    pts = np.concatenate([faces, inside, outside]).astype(np.float32)
+    pts = pts[_morton_order(pts.astype(np.float64))]
+    bad_pts = pts.copy()
+    bad_pts[5] = np.nan
+    bad_pts[700, 1] = np.inf
+    cases = [np.full((n, n, n), base + np.float32(k) * np.spacing(base), dtype=np.float32) for k in (-3, 0, 3)]
+    mixed = (base + rng.normal(0, 0.3, (n, n, n))).astype(np.float32)
+    cases.append(mixed)
+    far = np.full((n, n, n), 1.0, dtype=np.float32)
+    far[4, 4, 4] = -1.0
+    cases.append(far)
+    bad = mixed.copy()
+    bad[2, 3, 4] = np.nan
+    bad[5, 5, 5] = np.inf
+    cases.append(bad)
+    for vals in cases:
+        g = W.SDFGrid(vals, np.array([lo] * 3), np.array([hi] * 3), np.array([dxyz] * 3))
+        for E in (np.eye(4), W.pose(W.rotation_exp(np.array([0.3, -0.2, 0.5])), np.array([0.05, 0.1, -0.07]))):
+            ins = make_instance(0, 0, 0, E, zs, np.eye(4), zs, col_th, 1e4, 0.0, 0.0, 0.0)
+            for gs in (1, 7, 256):
+                _cull_checks(g, pts, ins, gs)
+            live, _ = _cull_checks(g, bad_pts, ins, 256)
+            assert live[5 // 256] and live[700 // 256]
