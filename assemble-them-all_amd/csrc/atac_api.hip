@@ -797,7 +797,11 @@ static int enqueue_eval(atac_batch b, uint32_t flags, const atac_outputs* out, h
     if (rc) return rc;
     if (b->inst.empty()) return ATAC_OK;
     void* rec = precise ? (void*)b->d_rec64 : (void*)b->d_rec32;
-    ATAC_HIP(launch_reset(b->d_counters, CNT_WORDS, st));
+    // groups whose boxes reach only cells where no point can be active are dropped
+    // first; that pass also clears the counters (all points are contacts with
+    // ATAC_EVAL_ALL_ACTIVE: nothing is dropped)
+    const bool cull = b->tune.group_cull > 0 && !(flags & ATAC_EVAL_ALL_ACTIVE) && b->n_tiles > 0;
+    if (!cull) ATAC_HIP(launch_reset(b->d_counters, CNT_WORDS, st));
     if (b->n_tiles > 0) {
         DetectArgs da{};
         da.q = tile_queues(b);
@@ -807,16 +811,16 @@ static int enqueue_eval(atac_batch b, uint32_t flags, const atac_outputs* out, h
         da.chunk_cap = (int32_t)b->chunk_cap;
         da.tile_active = b->d_tile_active; da.tile_chunk0 = b->d_tile_chunk0; da.chunks = b->d_chunks;
         da.slow = b->d_slow; da.counters = counters(b);
-        // groups whose boxes reach only inactive cells are dropped first (all
-        // points are contacts with ATAC_EVAL_ALL_ACTIVE: nothing is dropped)
-        const bool cull = b->tune.group_cull > 0 && !(flags & ATAC_EVAL_ALL_ACTIVE);
         da.tile_live = cull ? b->d_tile_live : nullptr;
         b->last_culled = cull;
         hipEvent_t t0 = stage_begin(b, st);
         if (cull) {
             CullArgs cu{};
             cu.tiles = b->d_tiles; cu.inst = b->d_inst; cu.grids = b->d_grids; cu.tile_live = b->d_tile_live;
-            cu.n_tiles = b->n_tiles;
+            cu.counters = b->d_counters; cu.n_tiles = b->n_tiles;
+            const int ng = (b->tune.tile_points + kGroup - 1) / kGroup;
+            cu.slots = 4;
+            while (cu.slots < ng) cu.slots *= 2;
             ATAC_HIP(launch_cull(cu, st));
         }
         ATAC_HIP(launch_detect(da, b->tune.detect_block, b->tune.points_per_lane, b->grid_detect, b->lds_detect, st));

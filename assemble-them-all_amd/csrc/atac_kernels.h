@@ -48,8 +48,9 @@ struct CullArgs {
     const InstDev* inst;
     const GridDev* grids;
     uint32_t* tile_live;        // n_tiles
+    int32_t* counters;          // CNT_WORDS, cleared by the pass (the evaluation's reset)
     int32_t n_tiles;
-    int32_t pad;
+    int32_t slots;              // lanes per tile: a power of two from 4 to 32, at least the groups of a tile
 };
 
 struct SlowArgs {

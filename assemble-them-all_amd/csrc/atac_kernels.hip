@@ -741,30 +741,39 @@ hipError_t launch_sat_setup(const GridDev& G, const uint8_t* cls, int32_t* sat, 
     return hipGetLastError();
 }
 
-// One wavefront per tile, one lane per group of the tile (tiles hold at most 32
-// groups): the tile's live-group mask.
+// One lane per group, `slots` lanes per tile (so 64 / slots tiles per
+// wavefront): the tiles' live-group masks. Block 0 also clears the evaluation's
+// counters, which the detection kernel launched after it accumulates into.
 __global__ __launch_bounds__(256) void cull_kernel(CullArgs a) {
-    const int lane = threadIdx.x & 63;
-    const int tile = (int)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    if (tile >= a.n_tiles) return;
-    const TileDev t = as_const(a.tiles)[tile];
-    const InstDev in = as_const(a.inst)[t.inst];
-    const GridDev G = as_const(a.grids)[t.grid];
-    const int ng = (t.count + kGroup - 1) / kGroup;
-    bool live = lane < ng;
-    if (live && G.fast && in.cls && in.csat && in.gbox) {
-        const float4* b = reinterpret_cast<const float4*>(in.gbox) + 2 * (t.begin / kGroup + lane);
-        const float4 l = b[0], h = b[1];
-        const float lo[3] = {l.x, l.y, l.z}, hi[3] = {h.x, h.y, h.z};
-        live = group_live(G, in, in.csat, lo, hi);
+    if (blockIdx.x == 0 && threadIdx.x < CNT_WORDS) a.counters[threadIdx.x] = 0;
+    const int lane = threadIdx.x & 63, S = a.slots;
+    const int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int tile = (int)(gi / S), g = (int)(gi % S);
+    bool live = false;
+    if (tile < a.n_tiles) {
+        const TileDev& t = a.tiles[tile];
+        const int ng = (t.count + kGroup - 1) / kGroup;
+        if (g < ng) {
+            live = true;
+            const InstDev& in = a.inst[t.inst];
+            const GridDev& G = a.grids[t.grid];
+            if (G.fast && in.cls && in.csat && in.gbox) {
+                const float4* b = reinterpret_cast<const float4*>(in.gbox) + 2 * (t.begin / kGroup + g);
+                const float4 l = b[0], h = b[1];
+                const float lo[3] = {l.x, l.y, l.z}, hi[3] = {h.x, h.y, h.z};
+                live = group_live(G, in, in.csat, lo, hi);
+            }
+        }
     }
     const uint64_t m = __ballot(live);
-    if (lane == 0) a.tile_live[tile] = (uint32_t)m;
+    if ((lane & (S - 1)) == 0 && tile < a.n_tiles)
+        a.tile_live[tile] = (uint32_t)((m >> lane) & (S >= 32 ? 0xffffffffull : ((1ull << S) - 1ull)));
 }
 
 hipError_t launch_cull(const CullArgs& a, hipStream_t st) {
     if (a.n_tiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(cull_kernel, dim3((a.n_tiles + 3) / 4), dim3(256), 0, st, a);
+    const int64_t lanes = (int64_t)a.n_tiles * a.slots;
+    hipLaunchKernelGGL(cull_kernel, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 

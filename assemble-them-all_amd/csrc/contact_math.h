@@ -494,9 +494,9 @@ ATAC_HD bool detect_interp(const GridDev& G, const InstDev& in, float px, float 
 
 // Groups and their culling. A point set's segment is cut into groups of kGroup
 // consecutive (Morton-sorted) points, each with the box of its FP32 coordinates.
-// Before detection, a group whose box can only reach inactive cells of the
-// classification is dropped: every one of its points would read an inactive
-// entry, so the decisions are the same without it. The test counts the
+// Before detection, a group whose box can only reach cells where no point is
+// active (cell_may_be_active) is dropped: none of its points would be found
+// active, so the decisions are the same without it. The test counts the
 // non-inactive blocks of 2 x 2 x 2 padded cells over the box's cell range with a
 // summed-area table: S(x, y, z) (x, y, z from 0 to the block counts, x fastest)
 // is the number of such blocks with indices below (x, y, z), zero on the lower faces.
@@ -507,13 +507,27 @@ ATAC_HD size_t sat_index(const GridDev& G, uint32_t x, uint32_t y, uint32_t z) {
     return ((size_t)z * sat_dim(G, 1) + y) * sat_dim(G, 0) + x;
 }
 
+// Can a point in padded cell (i, j, k) be active? Its table entry says so, except
+// in the layer beyond the last node (padded index n) of an axis whose box ends
+// on that node: the table keeps that layer open for its points exactly on the box
+// face, whose values are convex combinations of face nodes of the interior
+// neighbour (padded n - 1), while the rest of the layer lies outside the box; so
+// there the neighbour's entry decides. (Only the culling uses this; detection
+// reads the table itself.)
+ATAC_HD bool cell_may_be_active(const GridDev& G, const uint8_t* cls, uint32_t i, uint32_t j, uint32_t k) {
+    uint32_t c[3] = {i, j, k};
+    for (int a = 0; a < 3; ++a)
+        if (c[a] == (uint32_t)G.n[a] && grid_cmax(G, a) == (double)(G.n[a] - 1)) c[a] -= 1u;
+    return cls[cls_entry(G, c[0], c[1], c[2])] != kClsInactive;
+}
+
 // Is block (bx, by, bz) of the padded grid not inactive (any of its cells)?
 ATAC_HD bool sat_block_open(const GridDev& G, const uint8_t* cls, uint32_t bx, uint32_t by, uint32_t bz) {
     bool any = false;
     for (uint32_t c = 0; c < 8; ++c) {
         const uint32_t i = 2 * bx + (c & 1u), j = 2 * by + ((c >> 1) & 1u), k = 2 * bz + (c >> 2);
         if (i < (uint32_t)cls_pad(G, 0) && j < (uint32_t)cls_pad(G, 1) && k < (uint32_t)cls_pad(G, 2))
-            any = any || cls[cls_entry(G, i, j, k)] != kClsInactive;
+            any = any || cell_may_be_active(G, cls, i, j, k);
     }
     return any;
 }

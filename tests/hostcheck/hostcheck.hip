@@ -166,7 +166,7 @@ int hc_detect(const float* vals, const int32_t n[3], const double minb[3], const
 // Group culling on the host: the summed-area table built as the device builds it
 // (block marks, then running sums along x, y, z), and per group of `gsize`
 // consecutive points: live[g] = group_live; needed[g] = some point of the group
-// reads a table entry that is not inactive; exact[g] = the table's block count
+// is active (the full interpolation's decision); exact[g] = the table's block count
 // over the group's range equals the count of open blocks found one by one (1 when
 // the range is not finite).
 int hc_group_cull(const float* vals, const int32_t n[3], const double minb[3], const double maxb[3],
@@ -205,7 +205,7 @@ int hc_group_cull(const float* vals, const int32_t n[3], const double minb[3], c
                 lo[a] = std::min(lo[a], c[a]);
                 hi[a] = std::max(hi[a], c[a]);
             }
-            need = need || in.cls[cls_offset(G, in, px[o], py[o], pz[o])] != kClsInactive;
+            need = need || detect_value(G, in, px[o], py[o], pz[o]) < 0.0f;
         }
         if (!finite)
             for (int a = 0; a < 3; ++a) lo[a] = hi[a] = NAN;

@@ -235,14 +235,15 @@ def _morton_order(p):
 def _cull_checks(grid, pts, inst, gsize=256):
     live, needed, exact = H.group_cull(grid, pts, inst, gsize)
     assert exact.all(), f"{int((~exact).sum())} summed-area counts differ from the block-by-block count"
-    assert not np.any(needed & ~live), f"{int((needed & ~live).sum())} groups dropped with a non-inactive cell"
+    assert not np.any(needed & ~live), f"{int((needed & ~live).sum())} groups dropped with an active point"
     return live, needed
 
 
 def test_group_culling_keeps_every_group_that_can_hold_a_candidate(peg):
     """Group culling on real poses (Morton-sorted groups of 256, as the library cuts
-    them, and of 16): a group is dropped only when every point of it reads an
-    inactive entry, and the summed-area table's counts are the blocks' own."""
+    them, and of 16): a group is dropped only when none of its points is active
+    (by the full interpolation), and the summed-area table's counts are the
+    blocks' own."""
     w, _ = peg
     pts = W.round_points_f32(w.point_sets[0]).astype(np.float32)
     pts = pts[_morton_order(pts)]
