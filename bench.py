@@ -556,6 +556,9 @@ def main():
                 "kernel": "cull_kernel + detect_kernel" if culled else "detect_kernel",
                 "algorithmic_bytes_per_launch": int(det_bytes),
                 "points_read": n_live, "groups": n_groups,
+                # the same launch priced as if every query read its point (12 B per query, the
+                # per-query rate without culling): what a non-culling detection would have to stream
+                "all_queries_point_bytes_GBps": round(12 * n_q / (det_ms * 1e-3) / 1e9, 1),
                 "touched_grid_values": touched, "launch_ms": round(det_ms, 4),
                 "measured_copy_GBps": round(copy_gbs, 1), "frac_of_measured_copy": round(achieved / copy_gbs, 4),
                 "step_algorithmic_bytes": int(step_alg), "step_achieved": round(step_gbs, 1),
