@@ -541,8 +541,10 @@ ATAC_HD bool sat_block_open(const GridDev& G, const uint8_t* cls, uint32_t bx, u
 // holds the cell of every point of the box. The box is live when a block over
 // those cells is not inactive, and whenever an input or the interval is not
 // finite.
-ATAC_HD bool group_block_range(const GridDev& G, const InstDev& in, const float lo[3], const float hi[3],
-                               uint32_t b0[3], uint32_t b1[3]) {
+// The cells [c0, c1] (per axis, padded indices) a point of the box can fall in;
+// false when an input or the interval is not finite.
+ATAC_HD bool group_cell_range(const GridDev& G, const InstDev& in, const float lo[3], const float hi[3],
+                              uint32_t c0[3], uint32_t c1[3]) {
     for (int a = 0; a < 3; ++a) {
         double sl = in.bs[a], sh = in.bs[a], mag = fabs(in.bs[a]);
         for (int j = 0; j < 3; ++j) {
@@ -556,10 +558,20 @@ ATAC_HD bool group_block_range(const GridDev& G, const InstDev& in, const float 
         const double cl = floor(sl - eps), ch = floor(sh + eps);
         if (!(fabs(cl) <= 1e300) || !(fabs(ch) <= 1e300)) return false;   // NaN or inf
         const double top = (double)(G.n[a] + 1);
-        const uint32_t i0 = (uint32_t)(cl < 0.0 ? 0.0 : (cl > top ? top : cl));
-        const uint32_t i1 = (uint32_t)(ch < 0.0 ? 0.0 : (ch > top ? top : ch));
-        b0[a] = i0 >> 1;   // blocks [b0, b1)
-        b1[a] = (i1 >> 1) + 1u;
+        c0[a] = (uint32_t)(cl < 0.0 ? 0.0 : (cl > top ? top : cl));
+        c1[a] = (uint32_t)(ch < 0.0 ? 0.0 : (ch > top ? top : ch));
+    }
+    return true;
+}
+
+// The summed-area table's blocks [b0, b1) over those cells.
+ATAC_HD bool group_block_range(const GridDev& G, const InstDev& in, const float lo[3], const float hi[3],
+                               uint32_t b0[3], uint32_t b1[3]) {
+    uint32_t c0[3], c1[3];
+    if (!group_cell_range(G, in, lo, hi, c0, c1)) return false;
+    for (int a = 0; a < 3; ++a) {
+        b0[a] = c0[a] >> 1;
+        b1[a] = (c1[a] >> 1) + 1u;
     }
     return true;
 }

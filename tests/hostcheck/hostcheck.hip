@@ -167,8 +167,8 @@ int hc_detect(const float* vals, const int32_t n[3], const double minb[3], const
 // (block marks, then running sums along x, y, z), and per group of `gsize`
 // consecutive points: live[g] = group_live; needed[g] = some point of the group
 // is active (the full interpolation's decision); exact[g] = the table's block count
-// over the group's range equals the count of open blocks found one by one (1 when
-// the range is not finite).
+// over the group's range equals the count of open blocks found one by one and
+// live[g] is that count's decision (1 when the range is not finite).
 int hc_group_cull(const float* vals, const int32_t n[3], const double minb[3], const double maxb[3],
                   const double d[3], const float* px, const float* py, const float* pz, int64_t npts,
                   const atac_instance* inst, int gsize, uint8_t* live, uint8_t* needed, uint8_t* exact) {
@@ -218,7 +218,8 @@ int hc_group_cull(const float* vals, const int32_t n[3], const double minb[3], c
             for (uint32_t z = b0[2]; z < b1[2]; ++z)
                 for (uint32_t y = b0[1]; y < b1[1]; ++y)
                     for (uint32_t x = b0[0]; x < b1[0]; ++x) brute += sat_block_open(G, in.cls, x, y, z) ? 1 : 0;
-            exact[g] = brute == sat_range_count(G, sat.data(), b0, b1) ? 1 : 0;
+            const int32_t count = sat_range_count(G, sat.data(), b0, b1);
+            exact[g] = (brute == count && (live[g] != 0) == (count != 0)) ? 1 : 0;
         }
     }
     return 0;
