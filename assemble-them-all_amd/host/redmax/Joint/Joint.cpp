@@ -38,20 +38,27 @@ HD series(HD s, const double* coef, int n) {
 
 // c = sin(t)/t, a = (1 - cos t)/t^2, b = (t - sin t)/t^3 as functions of s = t^2; Taylor
 // series below t = 0.5 (11 terms: the first omitted term is below 1e-20 of the sum)
-void so3_coeffs(HD s, HD& c, HD& a, HD& b) {
-    if (s.v < 0.25) {
+// 1/(2k+1)!, 1/(2k+2)!, 1/(2k+3)!, k < 11
+struct SeriesCoefs {
+    double fc[11], fa[11], fb[11];
+    SeriesCoefs() {
         double fact[24];
         fact[0] = 1.0;
         for (int n = 1; n < 24; ++n) fact[n] = fact[n - 1] * n;
-        double fc[11], fa[11], fb[11];
         for (int k = 0; k < 11; ++k) {
             fc[k] = 1.0 / fact[2 * k + 1];
             fa[k] = 1.0 / fact[2 * k + 2];
             fb[k] = 1.0 / fact[2 * k + 3];
         }
-        c = series(s, fc, 11);
-        a = series(s, fa, 11);
-        b = series(s, fb, 11);
+    }
+};
+
+void so3_coeffs(HD s, HD& c, HD& a, HD& b) {
+    if (s.v < 0.25) {
+        static const SeriesCoefs k;
+        c = series(s, k.fc, 11);
+        a = series(s, k.fa, 11);
+        b = series(s, k.fb, 11);
     } else {
         const HD t = hsqrt(s), sn = hsin(t), hs = hsin(0.5 * t);
         c = sn / t;
@@ -62,11 +69,13 @@ void so3_coeffs(HD s, HD& c, HD& a, HD& b) {
 
 }  // namespace
 
-void so3_derivatives(const double r[3], So3Derivs& out) {
+void so3_derivatives(const double r[3], So3Derivs& out, bool second) {
+    // one evaluation per pair j <= k of directions (the second derivatives are
+    // symmetric), or per direction j alone without them
     for (int j = 0; j < 3; ++j)
-        for (int k = 0; k < 3; ++k) {
+        for (int k = second ? j : 0; k < (second ? 3 : 1); ++k) {
             HD x[3];
-            for (int i = 0; i < 3; ++i) x[i] = {r[i], i == j ? 1.0 : 0.0, i == k ? 1.0 : 0.0, 0.0};
+            for (int i = 0; i < 3; ++i) x[i] = {r[i], i == j ? 1.0 : 0.0, second && i == k ? 1.0 : 0.0, 0.0};
             const HD s = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
             HD c, a, b;
             so3_coeffs(s, c, a, b);
@@ -83,12 +92,14 @@ void so3_derivatives(const double r[3], So3Derivs& out) {
                     out.R[e] = R.v;
                     out.Jr[e] = J.v;
                 }
-                if (k == 0) {
+                if (k == (second ? j : 0)) {
                     out.dR[j][e] = R.a;
                     out.dJr[j][e] = J.a;
                 }
-                out.d2R[j][k][e] = R.ab;
-                out.d2Jr[j][k][e] = J.ab;
+                if (second) {
+                    out.d2R[j][k][e] = out.d2R[k][j][e] = R.ab;
+                    out.d2Jr[j][k][e] = out.d2Jr[k][j][e] = J.ab;
+                }
             }
         }
 }
@@ -110,7 +121,7 @@ Joint::Joint(std::string name, int id, Type type, const Matrix3& R, const Vector
     update();
 }
 
-void Joint::update() {
+void Joint::update(bool derivs) {
     _Q = Matrix4::Identity();
     if (_type == Type::Translational) {
         for (int i = 0; i < 3; ++i) {
@@ -121,7 +132,7 @@ void Joint::update() {
         // q = [p; r]: Q = [R(r) p; 0 1], phi = [Jr rdot; R^T pdot]
         const double r[3] = {_q(3), _q(4), _q(5)}, rd[3] = {_qdot(3), _qdot(4), _qdot(5)};
         So3Derivs d;
-        so3_derivatives(r, d);
+        so3_derivatives(r, d, derivs);
         double Rdot[9] = {0}, Jdot[9] = {0};
         for (int m = 0; m < 3; ++m)
             for (int e = 0; e < 9; ++e) {
@@ -138,7 +149,7 @@ void Joint::update() {
             }
             _Q(i, 3) = _q(i);
         }
-        for (int m = 0; m < 3; ++m) {
+        for (int m = 0; m < 3 && derivs; ++m) {
             MatrixX& dS = _dSj_dq[3 + m];
             MatrixX& dSd = _dSjdot_dq[3 + m];
             for (int i = 0; i < 3; ++i)

@@ -34,7 +34,9 @@ public:
 
     Joint(std::string name, int id, Type type, const Matrix3& R, const Vector3& p);
 
-    void update();
+    // Q, S_j, S_j_dot, E_0j, phi from q and qdot, and (derivs) the q-derivatives of
+    // S_j and S_j_dot (only those need second derivatives of the rotation)
+    void update(bool derivs = true);
     // free3d-exp: r -> (1 - 2 pi / |r|) r while |r| > 1.5 pi, the rotation unchanged and
     // rdot mapped so that the angular velocity is unchanged (JointSphericalExp.cpp:237-262)
     bool reparam();
@@ -47,11 +49,12 @@ public:
 // SO(3) of exponential coordinates r: R = exp([r]x) and the right Jacobian Jr (the
 // body angular velocity is Jr(r) rdot), with their first and second partial derivatives
 // in r (dR[k] = dR/dr_k, d2R[j][k] = d2R/dr_j dr_k, the same for Jr). Row-major 3 x 3.
+// Without `second` the second derivatives are left unset.
 struct So3Derivs {
     double R[9], Jr[9];
     double dR[3][9], dJr[3][9];
     double d2R[3][3][9], d2Jr[3][3][9];
 };
-void so3_derivatives(const double r[3], So3Derivs& out);
+void so3_derivatives(const double r[3], So3Derivs& out, bool second = true);
 
 }  // namespace redmax

@@ -111,7 +111,13 @@ struct BodyJac {
     int n = 0;
     std::vector<double> J, Jd;
     std::vector<std::vector<double>> dJ, dJd;
+    std::vector<char> dzero;   // per q_k: dJ[k] and dJd[k] are zero (S_j does not depend on q_k)
 };
+bool all_zero(const MatrixX& S) {
+    for (int i = 0; i < S.size(); ++i)
+        if (S.data()[i] != 0.0) return false;
+    return true;
+}
 BodyJac body_jacobian(const Simulation::SimBody& b, bool derivs) {
     BodyJac o;
     const Joint& j = *b.joint;
@@ -132,7 +138,14 @@ BodyJac body_jacobian(const Simulation::SimBody& b, bool derivs) {
     if (derivs) {
         o.dJ.resize(o.n);
         o.dJd.resize(o.n);
+        o.dzero.assign(o.n, 0);
         for (int k = 0; k < o.n; ++k) {
+            if (all_zero(j._dSj_dq[k]) && all_zero(j._dSjdot_dq[k])) {
+                o.dJ[k].assign(6 * o.n, 0.0);
+                o.dJd[k].assign(6 * o.n, 0.0);
+                o.dzero[k] = 1;
+                continue;
+            }
             apply(j._dSj_dq[k], o.dJ[k]);
             apply(j._dSjdot_dq[k], o.dJd[k]);
         }
@@ -151,6 +164,8 @@ VectorX lu_solve(const MatrixX& A_in, const VectorX& b_in) {
     std::vector<double> A(A_in.data(), A_in.data() + (size_t)n * n);
     auto a = [&](int r, int c) -> double& { return A[(size_t)r + (size_t)n * c]; };
     VectorX b = b_in;
+    std::vector<std::pair<int, double>> rows;
+    rows.reserve(n);
     for (int k = 0; k < n; ++k) {
         int p = k;
         for (int r = k + 1; r < n; ++r)
@@ -159,12 +174,24 @@ VectorX lu_solve(const MatrixX& A_in, const VectorX& b_in) {
             for (int c = 0; c < n; ++c) std::swap(a(k, c), a(p, c));
             std::swap(b(k), b(p));
         }
+        // the rows below with a non-zero in column k and their multipliers, then the
+        // update column by column over the pivot row's non-zeros (the bodies' blocks
+        // leave most of both zero; each entry gets the same single update as a plain
+        // row-by-row elimination gives it)
         const double piv = a(k, k);
+        rows.clear();
         for (int r = k + 1; r < n; ++r) {
-            const double l = a(r, k) / piv;
-            if (l == 0.0) continue;
-            for (int c = k + 1; c < n; ++c) a(r, c) -= l * a(k, c);
+            const double ark = a(r, k);
+            if (ark == 0.0) continue;
+            const double l = ark / piv;
+            rows.push_back({r, l});
             b(r) -= l * b(k);
+        }
+        if (rows.empty()) continue;
+        for (int c = k + 1; c < n; ++c) {
+            const double akc = a(k, c);
+            if (akc == 0.0) continue;
+            for (const auto& rl : rows) a(rl.first, c) -= rl.second * akc;
         }
     }
     VectorX x(n);
@@ -441,8 +468,11 @@ void Simulation::set_qdot_init(const VectorX& qdot) {
 
 // Joint::update then Body::update (Joint.cpp:98-140, Body.cpp:138-160): E_0i = E_0j E_ji,
 // phi_i = Ad(E_ij) phi_j
-void Simulation::update_robot(bool) {
-    for (auto& j : _joints) j->update();
+void Simulation::update_robot(bool) { update_kinematics(true); }
+
+// the joints (with the q-derivatives of S_j when derivs), then the bodies' poses and twists
+void Simulation::update_kinematics(bool derivs) {
+    for (auto& j : _joints) j->update(derivs);
     for (auto& b : _bodies) {
         b.body->_E_0i = mul4(b.joint->_E_0j, b.body->_E_ji);
         double A[36];
@@ -808,7 +838,7 @@ void Simulation::contact_forces(bool derivs, VectorX& fm, MatrixX* Km, MatrixX* 
 // ------------------------------------------------------------------ dynamics
 
 void Simulation::computeMatrices(MatrixX& M, VectorX& f) {
-    update_robot();
+    update_kinematics(false);   // M and f need no q-derivatives of S_j
     VectorX fm = VectorX::Zero(_ndof_m);
     body_forces(false, fm, nullptr, nullptr);
     contact_forces(false, fm, nullptr, nullptr);
@@ -878,7 +908,7 @@ void Simulation::assemble(MatrixX& M, VectorX& f, MatrixX& K, MatrixX& D, std::v
         const int n = J.n, o = j._index[0], mo = 6 * bi;
         dJq[bi].assign(6 * n, 0.0);
         for (int k = 0; k < n; ++k)
-            for (int r = 0; r < 6; ++r) {
+            for (int r = 0; r < 6 && !J.dzero[k]; ++r) {
                 double s = 0.0;
                 for (int c = 0; c < n; ++c) s += J.dJ[k][n * r + c] * j._qdot(c);
                 dJq[bi][n * r + k] = s;
@@ -908,6 +938,7 @@ void Simulation::assemble(MatrixX& M, VectorX& f, MatrixX& K, MatrixX& D, std::v
         const size_t ji = (size_t)j._id;
         dMj[ji].assign(n, MatrixX::Zero(n, n));
         for (int k = 0; k < n; ++k) {
+            if (J.dzero[k]) continue;   // dM_dq(k) and the K column below are zero
             // dM_dq(k) = dJ_k^T Mm J + (dJ_k^T Mm J)^T (non-zero only in the joint's block)
             MatrixX& dM = dMj[ji][k];
             for (int a = 0; a < n; ++a)
@@ -932,12 +963,19 @@ void Simulation::assemble(MatrixX& M, VectorX& f, MatrixX& K, MatrixX& D, std::v
             }
         }
     }
-    // J^T Km J + J^T Dm dJdq_qdot, J^T Dm J over the body pairs with non-zero blocks
+    // J^T Km J + J^T Dm dJdq_qdot, J^T Dm J over the body pairs with non-zero blocks:
+    // each body's own, and those of the contact pairs this evaluation gave a force
+    auto nonzero = [&](int a, int b) {
+        for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 6; ++c)
+                if (Km(6 * a + r, 6 * b + c) != 0.0 || Dm(6 * a + r, 6 * b + c) != 0.0) return true;
+        return false;
+    };
     std::set<std::pair<int, int>> blocks;
     for (int bi = 0; bi < nb; ++bi) blocks.insert({bi, bi});
     for (const auto& c : _contacts) {
-        blocks.insert({c.general_body, c.sdf_body});
-        blocks.insert({c.sdf_body, c.general_body});
+        if (nonzero(c.general_body, c.sdf_body)) blocks.insert({c.general_body, c.sdf_body});
+        if (nonzero(c.sdf_body, c.general_body)) blocks.insert({c.sdf_body, c.general_body});
     }
     for (const auto& ab : blocks) {
         const int a = ab.first, b = ab.second;

@@ -162,6 +162,7 @@ private:
     long long _use_clock = 0;
 
     void parse(const std::string& xml_string);
+    void update_kinematics(bool derivs);
     void build_contact_data();
     int body_index(const std::string& name) const;
     const Joint* joint_by_name(const std::string& name) const;
