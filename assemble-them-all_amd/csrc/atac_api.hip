@@ -784,7 +784,9 @@ static QueueDev tile_queues(atac_batch b) {
 
 static int32_t* counters(atac_batch b) { return b->d_counters; }
 
-static int enqueue_eval(atac_batch b, uint32_t flags, const atac_outputs* out, hipStream_t st) {
+// overflow_out (optional) receives the evaluation's overflow flag from the reduction.
+static int enqueue_eval(atac_batch b, uint32_t flags, const atac_outputs* out, hipStream_t st,
+                        int32_t* overflow_out = nullptr) {
     if (flags & ~(ATAC_EVAL_DERIV | ATAC_EVAL_P6 | ATAC_EVAL_CHECKPOINT | ATAC_EVAL_PRECISE | ATAC_EVAL_ALL_ACTIVE))
         return fail(ATAC_ERR_INVALID, "atac_batch_eval: unknown flag");
     const int mode = mode_of(flags);
@@ -849,6 +851,8 @@ static int enqueue_eval(atac_batch b, uint32_t flags, const atac_outputs* out, h
     ra.dst[3] = out ? out->Dm : nullptr;      ra.begin[3] = 168; ra.width[3] = 144;
     ra.n_active = out ? out->n_active : nullptr;
     ra.counters = counters(b);
+    ra.overflow_out = overflow_out;
+    ra.host_dst = overflow_out ? 1 : 0;   // the small-batch graph's outputs are its pinned staging
     hipEvent_t t3 = stage_begin(b, st);
     ATAC_HIP(launch_reduce(ra, st));
     stage_end(b, ST_REDUCE, t3, st);
@@ -1260,14 +1264,14 @@ int atac_batch_eval_instances_host(atac_batch b, const atac_instance* inst, int3
             return fail(ATAC_ERR_NOMEM, std::string("pinned staging: ") + hipGetErrorString(e));
         }
     }
-    double* base = b->h_out;
     double* P = reinterpret_cast<double*>(b->pin);
     int32_t* Pn = reinterpret_cast<int32_t*>(b->pin + n * 312 * sizeof(double));
     const uint32_t key = flags | ((Km || Dm) ? 1u << 8 : 0u) | (dfm_dp6 ? 1u << 9 : 0u);
     atac_batch_s::HostGraph& hg = b->host_graphs[key];
     hipStream_t st = b->cap_stream;
     if (!hg.e || hg.gen != b->buf_gen) {
-        // capture: instance upload, evaluation, result copies (no stage timing inside)
+        // capture: instance upload and evaluation, whose reduction writes the results
+        // and the overflow flag straight into the pinned staging (no stage timing inside)
         if (hg.e) (void)hipGraphExecDestroy(hg.e);
         if (hg.g) (void)hipGraphDestroy(hg.g);
         hg = atac_batch_s::HostGraph{};
@@ -1275,20 +1279,14 @@ int atac_batch_eval_instances_host(atac_batch b, const atac_instance* inst, int3
         const bool timing = b->timing, rc_cap = b->rec_captured, cp = b->cap_precise;
         b->timing = false;
         atac_outputs o{};
-        o.fm = base;
-        o.dfm_dp6 = dfm_dp6 ? base + 12 * n : nullptr;
-        o.Km = (Km || Dm) ? base + 24 * n : nullptr;
-        o.Dm = (Km || Dm) ? base + 168 * n : nullptr;
-        o.n_active = b->h_nact;
+        o.fm = P;
+        o.dfm_dp6 = dfm_dp6 ? P + 12 * n : nullptr;
+        o.Km = (Km || Dm) ? P + 24 * n : nullptr;
+        o.Dm = (Km || Dm) ? P + 168 * n : nullptr;
+        o.n_active = Pn;
         ATAC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         hipError_t e = hipMemcpyAsync(b->d_inst, b->pin_inst, sizeof(InstDev) * n, hipMemcpyHostToDevice, st);
-        if (e == hipSuccess) rc = enqueue_eval(b, flags, &o, st);
-        if (e == hipSuccess && !rc) e = hipMemcpyAsync(P, base, sizeof(double) * (dfm_dp6 ? 24 : 12) * n, hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess && !rc && (Km || Dm))
-            e = hipMemcpyAsync(P + 24 * n, base + 24 * n, sizeof(double) * 288 * n, hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess && !rc) e = hipMemcpyAsync(Pn, b->h_nact, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess && !rc)
-            e = hipMemcpyAsync(Pn + n, b->d_counters + CNT_OVERFLOW, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) rc = enqueue_eval(b, flags, &o, st, Pn + n);
         hipGraph_t g = nullptr;
         const hipError_t e2 = hipStreamEndCapture(st, &g);
         b->timing = timing;

@@ -101,12 +101,13 @@ struct ReduceArgs {
     const int32_t* tile_chunk0;
     const double* partial;
     int32_t nseg;
-    int32_t pad;
+    int32_t host_dst;           // the outputs are in host memory: each wave ends with a system-scope fence
     double* dst[6];
     int32_t begin[6];
     int32_t width[6];
     int32_t* n_active;          // optional: active contacts per instance
     const int32_t* counters;    // CNT_OVERFLOW set: every output is NaN (the records were incomplete)
+    int32_t* overflow_out;      // optional: receives counters[CNT_OVERFLOW] (host-visible memory allowed)
 };
 
 struct PlainArgs {

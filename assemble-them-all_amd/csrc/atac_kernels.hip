@@ -618,6 +618,7 @@ constexpr int kRedGroup = 16;   // partial rows whose reads a reduce wavefront i
 __global__ __launch_bounds__(256) void reduce_kernel(ReduceArgs a) {
     const int lane = threadIdx.x & 63;
     const int i = (int)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (a.overflow_out && blockIdx.x == 0 && threadIdx.x == 0) *a.overflow_out = a.counters[CNT_OVERFLOW];
     if (i >= a.n_inst) return;
     const InstDev in = as_const(a.inst)[i];
     const int tb = in.tile_begin, nt = in.tile_count;
@@ -680,6 +681,7 @@ __global__ __launch_bounds__(256) void reduce_kernel(ReduceArgs a) {
         const int n = (int)wave_sum((double)nact);
         if (lane == 0) a.n_active[i] = ovf ? -1 : n;
     }
+    if (a.host_dst) __threadfence_system();   // the stores reach host memory before the launch completes
 }
 
 // ------------------------------------------------------------------ grid set-up
