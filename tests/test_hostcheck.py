@@ -295,3 +295,31 @@ def test_group_culling_adversarial():
                 _cull_checks(g, pts, ins, gs)
             live, _ = _cull_checks(g, bad_pts, ins, 256)
             assert live[5 // 256] and live[700 // 256]
+
+
+def test_group_culling_box_inside_and_beyond_grid():
+    """Group culling where max_box is below the grid's last node, on it and beyond it
+    (only on it is the layer beyond the last node judged by its interior neighbour),
+    with points in and around the box and on its max faces: no group holding an
+    active point is dropped; groups of 1, 16 and 256 points."""
+    rng = np.random.default_rng(11)
+    n, lo = 11, -1.0
+    dx = 0.2
+    col_th = 0.01
+    zs = np.zeros(6)
+    ins = make_instance(0, 0, 0, np.eye(4), zs, np.eye(4), zs, col_th, 1e4, 0.0, 0.0, 0.0)
+    xs = lo + dx * np.arange(n)
+    X, Y, Z = np.meshgrid(xs, xs, xs, indexing="ij")
+    field = (0.35 * X + 0.2 * Y - 0.3 * Z + 0.05 + 0.02 * rng.normal(size=X.shape)).transpose(2, 1, 0)
+    vals = np.ascontiguousarray(field, dtype=np.float32)
+    for hi in (lo + 6.3 * dx, lo + (n - 1) * dx, lo + (n - 1) * dx + 0.37):
+        p = rng.uniform(lo - 0.3, hi + 0.3, (3000, 3))
+        on_face = rng.uniform(lo, hi, (900, 3))
+        for a in range(3):
+            on_face[300 * a:300 * (a + 1), a] = hi
+        pts = np.concatenate([p, on_face]).astype(np.float32)
+        pts = pts[_morton_order(pts.astype(np.float64))]
+        g = W.SDFGrid(vals, np.array([lo] * 3), np.array([hi] * 3), np.array([dx] * 3))
+        for gs in (1, 16, 256):
+            live, needed = _cull_checks(g, pts, ins, gs)
+            assert needed.any()
