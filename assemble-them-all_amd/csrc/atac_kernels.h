@@ -26,7 +26,7 @@ __host__ __device__ constexpr size_t detect_lds_per_wave(int span, int ppl) {
 struct DetectArgs {
     QueueDev q;                 // tiles
     const TileDev* tiles;
-    int32_t wave_span;          // points of a tile per wavefront (multiple of 64 x points per lane)
+    int32_t wave_span;          // output capacity per wavefront: its share of a tile's groups x 256 points
     int32_t all_active;         // every point is a contact (fixed contact lists)
     const InstDev* inst;
     const GridDev* grids;

@@ -1,9 +1,12 @@
 // gfx950 kernels of the point-vs-SDF contact path.
 //
 // Batched path:
+//   cull_kernel    per group of 256 sorted points: can its box reach a cell where
+//                  a point may be active? (the tiles' live-group masks); it also
+//                  clears the evaluation's counters.
 //   detect_kernel  tiles of Morton-sorted FP32 contact points, taken from one
 //                  queue in instance order (or grid-grouped / per-XCD queues).
-//                  Streams each tile (several points per lane, nontemporal
+//                  Streams each tile's live groups (several points per lane, nontemporal
 //                  loads), maps every point into the grid's padded cell space
 //                  with an FP64 affine map and reads one byte of the grid's cell
 //                  classification table (inactive / active / open). Open cells
@@ -13,7 +16,8 @@
 //                  record slots and work items of the bounded record pool.
 //   slow_kernel    the FP64 FD-stencil samples of the few active points whose
 //                  stencil leaves their cell.
-//   contact_kernel work items of kChunk active points, one wavefront each: builds
+//   contact_kernel work items of kChunk active points, one wavefront each (one
+//                  workgroup each with Km/Dm): builds
 //                  the records (sample of phi, grad phi, Hessian) and sums the
 //                  wrench (optionally the p6 column and Km/Dm tangent columns)
 //                  into the item's partial row.
@@ -138,19 +142,20 @@ __device__ __forceinline__ void load_points(P px, P py, P pz, int o0, int lane, 
     }
 }
 
-// Points of one tile per work item. The tile is streamed: a step takes 64 * PPL
-// consecutive points of the wavefront's contiguous span, lane l the points
-// 64 k + l (each load and block-bound read instruction covers 64 neighbouring
-// points; the next points are loaded behind the block-bound reads of the current
-// ones, so a step waits once), and only the block bounds are read
-// (detect_candidates). The candidates a step finds (settled active or open; a
+// Points of one tile per work item. The tile's live groups are shared among its
+// wavefronts by rank (contiguous ranges, so their outputs follow one another in
+// point order), and each wavefront streams its groups: a step takes 64 * PPL
+// consecutive points of a group, lane l the points 64 k + l (each load and
+// table-read instruction covers 64 neighbouring points; the next step's points
+// are loaded behind the table reads of the current ones, so a step waits once),
+// and only the classification table is read (detect_candidates). The candidates a step finds (settled active or open; a
 // few per cent, near the surface) are appended, with their coordinates, to the
 // wavefront's LDS ring in ascending order (ballots per point slot k); whenever 64
 // are waiting, the next step finishes them one per lane (candidate_locate and
 // candidate_finish: cell read and interpolation of the open ones, FD-stencil
 // test), overlapped with its own work, instead of every stream step branching
 // into that work for a few lanes. The active offsets (bit 15: slow path; tiles
-// hold at most 32768 points) go to a per-wave LDS segment, then to the tile's
+// hold at most 8192 points) go to a per-wave LDS segment, then to the tile's
 // slots of the active list, and the tile pushes its work items for the record,
 // contact and adjoint passes.
 template <int BLOCK, int PPL>
