@@ -636,6 +636,57 @@ PYBIND11_MODULE(_redmax, m) {
                  d["f"] = vec_out(f);
                  return d;
              }, py::arg("derivatives") = false, "M, f (and K = df/dq, D = df/dqdot, dM/dq) at the current state")
+        .def("last_contact_eval", [](const Simulation& s) {
+                 const auto& e = s.last_contact_eval();
+                 const py::ssize_t n = (py::ssize_t)e.pairs.size();
+                 std::vector<std::pair<std::string, std::string>> names;
+                 for (const auto& pr : e.pairs) names.push_back({s._bodies[pr.first].name, s._bodies[pr.second].name});
+                 py::array_t<double> f({n, (py::ssize_t)12});
+                 std::copy(e.f.begin(), e.f.begin() + 12 * n, f.mutable_data());
+                 py::array_t<int32_t> na(n);
+                 std::copy(e.na.begin(), e.na.begin() + n, na.mutable_data());
+                 py::dict d;
+                 d["pairs"] = names;
+                 d["f"] = f;
+                 d["n_active"] = na;
+                 d["derivatives"] = e.derivs;
+                 if (e.derivs) {
+                     py::array_t<double> K({n, (py::ssize_t)12, (py::ssize_t)12}), D({n, (py::ssize_t)12, (py::ssize_t)12});
+                     std::copy(e.K.begin(), e.K.begin() + 144 * n, K.mutable_data());
+                     std::copy(e.D.begin(), e.D.begin() + 144 * n, D.mutable_data());
+                     d["K"] = K;
+                     d["D"] = D;
+                 }
+                 return d;
+             }, "the last contact evaluation: (general body, SDF body) names, wrench f [tau1 f1 tau2 f2] "
+                "per pair, active counts and, with derivatives, K = df/d[q1 q2] and D = df/d[phi1 phi2]")
+        .def("contact_vjp", [](Simulation& s, const arr& g) {
+                 const py::ssize_t n = (py::ssize_t)s.last_contact_eval().pairs.size();
+                 if (g.ndim() != 2 || g.shape(0) != n || g.shape(1) != 12)
+                     throw std::invalid_argument("g must have shape (pairs of the last evaluation, 12)");
+                 std::vector<double> gv(g.data(), g.data() + 12 * n), r;
+                 {
+                     py::gil_scoped_release rel;
+                     r = s.contact_vjp(gv);
+                 }
+                 py::dict d;
+                 const char* keys[4] = {"q1", "q2", "phi1", "phi2"};
+                 for (int k = 0; k < 4; ++k) {
+                     py::array_t<double> a({n, (py::ssize_t)6});
+                     for (py::ssize_t i = 0; i < n; ++i)
+                         for (int c = 0; c < 6; ++c) a.mutable_data()[i * 6 + c] = r[26 * i + 6 * k + c];
+                     d[keys[k]] = a;
+                 }
+                 py::array_t<double> kn(n), kt(n);
+                 for (py::ssize_t i = 0; i < n; ++i) {
+                     kn.mutable_data()[i] = r[26 * i + 24];
+                     kt.mutable_data()[i] = r[26 * i + 25];
+                 }
+                 d["kn"] = kn;
+                 d["kt"] = kt;
+                 return d;
+             }, py::arg("g"), "the adjoint of the last contact evaluation for d(loss)/df per pair: d(loss)/d q1, q2, "
+                "phi1, phi2 (6 each), kn, kt")
         .def("stats", [](const Simulation& s) {
                  const auto& st = s.stats();
                  py::dict d;

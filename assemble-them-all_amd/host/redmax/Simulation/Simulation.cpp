@@ -789,7 +789,39 @@ void Simulation::evaluate_instances(const std::vector<int>& key, std::vector<ata
              "atac_batch_eval_instances_host");
     ++_stats.evaluations;
     if (derivs) ++_stats.derivative_evaluations;
+    _last_batch = it->second.batch;
+    _last_eval.pairs.clear();
+    for (int k : key)
+        if (k >= 0) _last_eval.pairs.push_back({_contacts[k].general_body, _contacts[k].sdf_body});
+    _last_eval.f = out.f;
+    _last_eval.K = out.K;
+    _last_eval.D = out.D;
+    _last_eval.na = out.na;
+    _last_eval.derivs = derivs;
     _stats.eval_ms += ms_since(t0);
+}
+
+std::vector<double> Simulation::contact_vjp(const std::vector<double>& g_fm) {
+    const size_t n = _last_eval.pairs.size();
+    if (!_last_batch || n == 0) throw_error("contact_vjp: no contact evaluation to differentiate");
+    if (g_fm.size() != 12 * n)
+        throw_error("contact_vjp: g_fm needs 12 values per instance of the last evaluation (" + std::to_string(n) + ")");
+    std::vector<double> q1(6 * n), q2(6 * n), p1(6 * n), p2(6 * n), kn(n), kt(n);
+    check_rc(atac_batch_vjp_host(_last_batch, g_fm.data(), q1.data(), q2.data(), p1.data(), p2.data(), kn.data(),
+                                 kt.data()),
+             "atac_batch_vjp_host");
+    std::vector<double> out(26 * n);
+    for (size_t i = 0; i < n; ++i) {
+        for (int c = 0; c < 6; ++c) {
+            out[26 * i + c] = q1[6 * i + c];
+            out[26 * i + 6 + c] = q2[6 * i + c];
+            out[26 * i + 12 + c] = p1[6 * i + c];
+            out[26 * i + 18 + c] = p2[6 * i + c];
+        }
+        out[26 * i + 24] = kn[i];
+        out[26 * i + 25] = kt[i];
+    }
+    return out;
 }
 
 void Simulation::scatter_contacts(const ContactEval& ce, size_t st, VectorX& fm, MatrixX* Km, MatrixX* Dm) {

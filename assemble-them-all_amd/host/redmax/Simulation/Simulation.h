@@ -126,6 +126,23 @@ public:
                             const std::vector<std::vector<Vector6>>& f_external, int num_steps,
                             std::vector<int>* converged = nullptr);
 
+    // The last contact evaluation (of contact_forces, forward or forward_candidates):
+    // per evaluated instance its (general_body, SDF_body) pair, wrench [tau1 f1 tau2 f2],
+    // active count and, when it had derivatives, Km and Dm (12 x 12 row-major, columns
+    // [q1 q2] and [phi1 phi2]).
+    struct LastContactEval {
+        std::vector<std::pair<int, int>> pairs;
+        std::vector<double> f, K, D;
+        std::vector<int32_t> na;
+        bool derivs = false;
+    };
+    const LastContactEval& last_contact_eval() const { return _last_eval; }
+    // The adjoint of that evaluation (ForceGeneralSDFContact's dfm/dq, dfm/dphi and
+    // dfm/dkn contracted with a cotangent, without forming them): g_fm holds 12 values per
+    // evaluated instance; returns 26 per instance: d/dq1 (6), d/dq2 (6), d/dphi1 (6),
+    // d/dphi2 (6), d/dkn, d/dkt (the conventions of atac_batch_vjp)
+    std::vector<double> contact_vjp(const std::vector<double>& g_fm);
+
     // M qddot = f in reduced coordinates, with K = df/dq, D = df/dqdot and dM/dq
     // (Simulation::computeMatrices, Simulation.cpp:379-506) at the current state
     void computeMatrices(MatrixX& M, VectorX& f);
@@ -160,6 +177,8 @@ private:
     };
     std::map<std::vector<int>, CachedBatch> _batches;
     long long _use_clock = 0;
+    LastContactEval _last_eval;
+    atac_batch _last_batch = nullptr;   // the batch of the last evaluation (its records are the adjoint's)
 
     void parse(const std::string& xml_string);
     void update_kinematics(bool derivs);

@@ -236,7 +236,8 @@ def multipart_measurement(cpu, steps):
     contact points and ~253^3 grid nodes each, all 380 ordered pairs declared and the
     AABB broad phase choosing the ones each evaluation runs; BDF1 at dt = 1e-3 with the
     FP64 Newton on the host (FP32 contact arithmetic). `steps` timed steps after 5
-    untimed ones; the 2,000-step loop's time is extrapolated from them."""
+    untimed ones, then `steps` more each followed by the adjoint of its last contact
+    evaluation; the 2,000-step loop's time is extrapolated from them."""
     import tempfile
     from atac import _redmax as R
     from atac import workloads as W
@@ -256,6 +257,16 @@ def multipart_measurement(cpu, steps):
     ms = (time.perf_counter() - t0) * 1e3 / steps
     st = sim.stats()
     ev = max(st["evaluations"], 1)
+    # the same loop with the adjoint every step: after each step, the VJP of its last
+    # contact evaluation for the loss 0.5 |fm|^2 over the evaluated pairs (cotangent fm)
+    t_adj = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.forward(1)
+        t1 = time.perf_counter()
+        sim.contact_vjp(sim.last_contact_eval()["f"])
+        t_adj += time.perf_counter() - t1
+    ms_adj = (time.perf_counter() - t0) * 1e3 / steps
     res = {"parts": len(names), "contact_points": n_points, "pairs_declared": len(names) * (len(names) - 1),
            "grid": "x".join(str(v) for v in sim.get_body_sdf(names[1])["values"].shape[::-1]) + " (part1)",
            "steps_timed": steps, "ms_per_step": round(ms, 3),
@@ -266,6 +277,9 @@ def multipart_measurement(cpu, steps):
            "contact_ms_per_step": round(st["eval_ms"] / steps, 3),
            "converged_last_step": bool(sim.is_converged()),
            "loop_2000_steps_s": round(ms * 2000 / 1e3, 1),
+           "ms_per_step_with_adjoint": round(ms_adj, 3),
+           "adjoint_ms_per_step": round(t_adj * 1e3 / steps, 3),
+           "loop_2000_steps_with_adjoint_s": round(ms_adj * 2000 / 1e3, 1),
            "scene_generation_s": round(t_gen, 1), "scene_build_s": round(t_build, 1)}
     if cpu:
         # the FP64 oracle over the pairs in contact (each free part with the plate, both

@@ -225,10 +225,11 @@ int atac_batch_eval_host(atac_batch b, uint32_t flags, double* fm, double* Km, d
 /* atac_batch_set_instances + atac_batch_eval_host in one call for host loops that
  * evaluate the same force objects at changing poses (Simulation::newton's force
  * evaluations, Simulation.cpp:1081-1117; the reference calls computeForce* per force
- * object each time): the instance upload, the evaluation and the result copies are
- * replayed as one HIP graph with one wait (captured on first use per flag combination
- * and again after the batch's buffers change). Same instance rules, meaning and
- * layout as those two; not for use while the batch's own stream is being captured. */
+ * object each time): the instance upload and the evaluation, whose reduction writes the
+ * results straight into pinned host memory, are replayed as one HIP graph with one wait
+ * (captured on first use per flag combination and again after the batch's buffers
+ * change). Same instance rules, meaning and layout as those two; not for use while the
+ * batch's own stream is being captured. */
 int atac_batch_eval_instances_host(atac_batch b, const atac_instance* inst, int32_t n_inst, uint32_t flags, double* fm,
                                    double* Km, double* Dm, double* dfm_dp6, int32_t* n_active);
 int atac_batch_vjp_host(atac_batch b, const double* g_fm, double* g_q1, double* g_q2, double* g_phi1, double* g_phi2,

@@ -240,3 +240,52 @@ def test_candidates_in_lockstep_match_single_runs(assets):
     print(f"  {K} candidates x 8 steps: {batched['evaluations']} batched evaluations "
           f"({single_evals} one at a time)", flush=True)
     assert batched["evaluations"] < single_evals
+
+
+def test_contact_vjp_of_the_last_evaluation(assets):
+    """Simulation.contact_vjp, the adjoint of the step's last contact evaluation, against
+    that evaluation's own Jacobians (g K for the poses, g D for the velocities) and, for
+    kn and kt, against a central difference in the contact scale (which multiplies both,
+    ForceGeneralSDFContact.cpp:155-156, so d fm/d scale = kn d fm/d kn + kt d fm/d kt)."""
+    sim = R.Simulation(_scene("free3d-exp"), assets, precise=True)
+    with pytest.raises(Exception):
+        sim.contact_vjp(np.zeros((2, 12)))
+    sim.set_state_init(np.zeros(6), np.array([0.02, -0.01, -0.05, 0.1, -0.2, 0.05]))
+    sim.reset()
+    sim.forward(4)
+    sim.computeMatrices(True)
+    e = sim.last_contact_eval()
+    P = len(e["pairs"])
+    assert e["derivatives"] and P >= 1 and e["n_active"].sum() > 0
+    assert set(e["pairs"]) <= {("box", "ground"), ("ground", "box")}
+    g = np.random.default_rng(5).normal(size=(P, 12))
+    r = sim.contact_vjp(g)
+    with pytest.raises(ValueError):
+        sim.contact_vjp(np.zeros((P + 1, 12)))
+    worst = 0.0
+    for i in range(P):
+        if not e["n_active"][i]:
+            assert not np.any(r["q1"][i]) and not np.any(r["phi2"][i])
+            continue
+        gk, gd = g[i] @ e["K"][i], g[i] @ e["D"][i]
+        sk = np.linalg.norm(g[i]) * np.abs(e["K"][i]).max()
+        sd = np.linalg.norm(g[i]) * max(np.abs(e["D"][i]).max(), 1e-30)
+        ek = np.abs(np.concatenate([r["q1"][i], r["q2"][i]]) - gk).max() / sk
+        ed = np.abs(np.concatenate([r["phi1"][i], r["phi2"][i]]) - gd).max() / sd
+        worst = max(worst, ek, ed)
+    # the scale derivative: fm at scale 1 +- h, same state, no derivatives needed
+    h = 1e-4
+    fs = []
+    for s in (1 + h, 1 - h):
+        sim.set_contact_scale(s)
+        sim.computeMatrices(False)
+        fs.append(sim.last_contact_eval()["f"])
+    sim.set_contact_scale(1.0)
+    dfs = (fs[0] - fs[1]) / (2 * h)
+    lhs = np.einsum("pi,pi->p", g, dfs)
+    rhs = KN * r["kn"] + KT * r["kt"]
+    scale = np.abs(g).max() * np.abs(dfs).max()
+    print(f"  {P} pairs, {int(e['n_active'].sum())} active points: VJP vs g K / g D worst {worst:.3g}; "
+          f"kn, kt vs the contact-scale difference {np.abs(lhs - rhs).max() / scale:.3g}", flush=True)
+    assert worst <= 1e-10
+    assert np.abs(lhs - rhs).max() <= 1e-6 * scale
