@@ -1100,6 +1100,7 @@ static PointArgs point_args(atac_grid g, const double E2[16], const double* phi2
     d.col_th = col_th;
     d.contact_scale = 1.0;
     d.kn_chain = 1.0;
+    set_f32_copies(d);   // the FP32 collision arithmetic reads the rounded copies
     return a;
 }
 

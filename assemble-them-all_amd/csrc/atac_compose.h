@@ -8,6 +8,21 @@
 
 namespace atac {
 
+// The FP32 copies of an instance's poses, twists and parameters, which the FP32
+// contact arithmetic loads (make_inst<float>); every InstDev the kernels read
+// needs them, whoever builds it.
+inline void set_f32_copies(InstDev& d) {
+    for (int k = 0; k < 9; ++k) { d.fR1[k] = (float)d.R1[k]; d.fR2[k] = (float)d.R2[k]; }
+    for (int k = 0; k < 3; ++k) {
+        d.fw1[k] = (float)d.w1[k]; d.fv1[k] = (float)d.v1[k];
+        d.fw2[k] = (float)d.w2[k]; d.fv2[k] = (float)d.v2[k];
+    }
+    d.fkn = (float)d.kn; d.fkt = (float)d.kt; d.fmu = (float)d.mu;
+    d.fdamping = (float)d.damping; d.fcontact_scale = (float)d.contact_scale;
+    d.fpad0 = 0.0f;
+    for (float& f : d.fpad) f = 0.0f;
+}
+
 inline void compose_instance(const atac_instance& s, const GridDev& G, const float* px, const float* py, const float* pz,
                              int64_t n_points, InstDev& d) {
     double R1[9], p1[3], R2[9], p2[3];
@@ -50,16 +65,8 @@ inline void compose_instance(const atac_instance& s, const GridDev& G, const flo
     d.pt_count = n_points;
     d.grid = s.grid;
     d.pad = 0;
-    for (int k = 0; k < 9; ++k) { d.fR1[k] = (float)d.R1[k]; d.fR2[k] = (float)d.R2[k]; }
-    for (int k = 0; k < 3; ++k) {
-        d.fw1[k] = (float)d.w1[k]; d.fv1[k] = (float)d.v1[k];
-        d.fw2[k] = (float)d.w2[k]; d.fv2[k] = (float)d.v2[k];
-    }
-    d.fkn = (float)d.kn; d.fkt = (float)d.kt; d.fmu = (float)d.mu;
-    d.fdamping = (float)d.damping; d.fcontact_scale = (float)d.contact_scale;
-    d.fpad0 = 0.0f;
+    set_f32_copies(d);
     d.cls = nullptr;   // set by the batch (its table for this grid and col_th)
-    for (float& f : d.fpad) f = 0.0f;
 }
 
 }  // namespace atac

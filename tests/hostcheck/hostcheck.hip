@@ -129,7 +129,78 @@ static void eval_t(const GridDev& G, const InstDev& in, uint32_t flags, double* 
     if (n_active) *n_active = na;
 }
 
+// collision_kernel's body for point i: BodySDFObj::collision values (11) and,
+// when o2 is given, its 135 derivative values.
+template <typename R>
+static void collision_t(const GridDev& G, const InstDev& in, const double* xw, const double* xwdot, double* v, double* o) {
+    const InstR<R> I = make_inst<R>(in);
+    ContactFwd<R> s;
+    contact_sample<R>(G, in, xw[0], xw[1], xw[2], s);
+    s.xw1dot = mk3<R>((R)xwdot[0], (R)xwdot[1], (R)xwdot[2]);
+    collision_forward<R>(I, s);
+    v[0] = s.d; v[1] = s.n.x; v[2] = s.n.y; v[3] = s.n.z; v[4] = s.ddot;
+    v[5] = s.tdot.x; v[6] = s.tdot.y; v[7] = s.tdot.z; v[8] = s.xi2.x; v[9] = s.xi2.y; v[10] = s.xi2.z;
+    if (!o) return;
+    enum { DD_DXW = 0, DD_DQ2 = 3, DN_DXW = 9, DN_DQ2 = 18, DDD_DXW = 36, DDD_DXWDOT = 39, DDD_DQ2 = 42, DDD_DPHI2 = 48,
+           DT_DXW = 54, DT_DXWDOT = 63, DT_DQ2 = 72, DT_DPHI2 = 90, DXI_DXW = 108, DXI_DQ2 = 117 };
+    for (int c = 0; c < 18; ++c) {
+        v3<R> t6[6];
+        for (auto& t : t6) t = zero3<R>();
+        v3<R>& sl = t6[c / 3];
+        if (c % 3 == 0) sl.x = 1; else if (c % 3 == 1) sl.y = 1; else sl.z = 1;
+        CollTangent<R> t;
+        collision_jvp<R>(I, s, t6[0], t6[1], t6[2], t6[3], t6[4], t6[5], t);
+        const int k = c % 3;
+        const R dn[3] = {t.dn.x, t.dn.y, t.dn.z}, dt[3] = {t.dtdot.x, t.dtdot.y, t.dtdot.z};
+        const R dx[3] = {t.dxi2.x, t.dxi2.y, t.dxi2.z};
+        if (c < 3) {
+            o[DD_DXW + k] = t.dd;
+            o[DDD_DXW + k] = t.dddot;
+            for (int r = 0; r < 3; ++r) { o[DN_DXW + 3 * r + k] = dn[r]; o[DT_DXW + 3 * r + k] = dt[r]; o[DXI_DXW + 3 * r + k] = dx[r]; }
+        } else if (c < 6) {
+            o[DDD_DXWDOT + k] = t.dddot;
+            for (int r = 0; r < 3; ++r) o[DT_DXWDOT + 3 * r + k] = dt[r];
+        } else if (c < 12) {
+            const int col = c - 6;
+            o[DD_DQ2 + col] = t.dd;
+            o[DDD_DQ2 + col] = t.dddot;
+            for (int r = 0; r < 3; ++r) { o[DN_DQ2 + 6 * r + col] = dn[r]; o[DT_DQ2 + 6 * r + col] = dt[r]; o[DXI_DQ2 + 6 * r + col] = dx[r]; }
+        } else {
+            const int col = c - 12;
+            o[DDD_DPHI2 + col] = t.dddot;
+            for (int r = 0; r < 3; ++r) o[DT_DPHI2 + 6 * r + col] = dt[r];
+        }
+    }
+}
+
 extern "C" {
+
+// BodySDFObj::collision of n world points against a grid body at pose E2 with
+// twist phi2 (the instance atac_sdf_collision builds: body 1 the world frame).
+int hc_collision(const float* vals, const int32_t n[3], const double minb[3], const double maxb[3], const double d[3],
+                 const double E2[16], const double phi2[6], double col_th, int precise, const double* xw,
+                 const double* xwdot, int64_t npts, double* values, double* derivs) {
+    HostGrid hg;
+    host_grid(vals, n, minb, maxb, d, hg);
+    InstDev in;
+    std::memset(&in, 0, sizeof in);
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) { in.R1[3 * r + c] = (r == c) ? 1.0 : 0.0; in.R2[3 * r + c] = E2[4 * r + c]; }
+        in.p2[r] = E2[4 * r + 3];
+        in.w2[r] = phi2[r];
+        in.v2[r] = phi2[3 + r];
+    }
+    in.col_th = col_th;
+    in.contact_scale = 1.0;
+    in.kn_chain = 1.0;
+    set_f32_copies(in);
+    for (int64_t i = 0; i < npts; ++i) {
+        double* o = derivs ? derivs + 135 * i : nullptr;
+        if (precise) collision_t<double>(hg.G, in, xw + 3 * i, xwdot + 3 * i, values + 11 * i, o);
+        else collision_t<float>(hg.G, in, xw + 3 * i, xwdot + 3 * i, values + 11 * i, o);
+    }
+    return 0;
+}
 
 // One force object on the host: points are FP32 SoA (as stored on the GPU).
 int hc_eval(const float* vals, const int32_t n[3], const double minb[3], const double maxb[3], const double d[3],

@@ -81,6 +81,20 @@ def eval_instance(grid, pts_f32, inst, deriv=False, precise=False, g_fm=None):
     return out
 
 
+def collision(grid, E2, phi2, col_th, xw, xw_dot, precise=False, derivs=True):
+    """Host run of collision_kernel: values (n, 11) and derivative values (n, 135)."""
+    vals, n, mn, mx, d = _grid_args(grid)
+    x = np.ascontiguousarray(xw, dtype=np.float64).reshape(-1, 3)
+    xd = np.ascontiguousarray(xw_dot, dtype=np.float64).reshape(-1, 3)
+    E = np.ascontiguousarray(E2, dtype=np.float64)
+    ph = np.ascontiguousarray(phi2, dtype=np.float64)
+    out = np.zeros((len(x), 11))
+    der = np.zeros((len(x), 135)) if derivs else None
+    lib().hc_collision(_vp(vals), _vp(n), _vp(mn), _vp(mx), _vp(d), _vp(E), _vp(ph), C.c_double(col_th),
+                       C.c_int(1 if precise else 0), _vp(x), _vp(xd), C.c_int64(len(x)), _vp(out), _vp(der))
+    return out, der
+
+
 def detect_values(grid, pts_f32, inst):
     vals, n, mn, mx, d = _grid_args(grid)
     p = np.ascontiguousarray(pts_f32, dtype=np.float32).reshape(-1, 3)

@@ -269,7 +269,12 @@ def test_distance_api(gpu):
     assert np.max(np.abs(got[inside] - ref[inside])) <= 1e-6
 
 
-def test_collision_api(gpu):
+# BodySDFObj::collision values and derivative blocks against the FP64 oracle, relative to
+# the largest entry: values to 1e-5 (FP64 arithmetic) and 2e-6 (FP32: about 16 units of
+# FP32 rounding; 1.4e-7 measured on the host), blocks to 1e-4 in both (the closed-form
+# cell derivatives against the oracle's FD stencils: 7.9e-7 measured)
+@pytest.mark.parametrize("precise,tol_v,tol_d", [(True, 1e-5, 1e-4), (False, 2e-6, 1e-4)])
+def test_collision_api(gpu, precise, tol_v, tol_d):
     w = W.peg_in_hole(n_points=2000, n_poses=1)
     g = w.grids[0]
     rng = np.random.default_rng(6)
@@ -280,18 +285,22 @@ def test_collision_api(gpu):
     G = O.GridRef.from_grid(g)
     ref = O.collision(G, E2, phi2, 0.01, xw, xd, derivs=True)
     keep = np.abs(ref["d"]) < 1.0          # inside the box: gradient defined
-    vals, der = DeviceGrid.from_grid(g).collision(xw, xd, E2, phi2, 0.01, derivs=True, precise=True)
+    vals, der = DeviceGrid.from_grid(g).collision(xw, xd, E2, phi2, 0.01, derivs=True, precise=precise)
     vals, der = vals.cpu().numpy(), der.cpu().numpy()
     refv = np.concatenate([ref["d"][:, None], ref["n"], ref["ddot"][:, None], ref["tdot"], ref["xi2"]], axis=1)
-    assert np.max(np.abs(vals[keep] - refv[keep])) <= 1e-5 * max(1.0, np.abs(refv[keep]).max())
-    off = 0
+    ev = np.max(np.abs(vals[keep] - refv[keep])) / max(1.0, np.abs(refv[keep]).max())
+    off, worst = 0, {}
     for name, shape in O.COLLISION_DERIV_LAYOUT:
         cnt = int(np.prod(shape))
         a = der[keep, off:off + cnt]
         bref = ref[name][keep].reshape(-1, cnt)
-        scale = max(1.0, np.abs(bref).max())
-        assert np.max(np.abs(a - bref)) <= 1e-4 * scale, name
+        worst[name] = np.max(np.abs(a - bref)) / max(1.0, np.abs(bref).max())
         off += cnt
+    print(f"  precise={precise}: {int(keep.sum())} points, values {ev:.3g}, blocks worst "
+          f"{max(worst.values()):.3g} ({max(worst, key=worst.get)})", flush=True)
+    assert ev <= tol_v
+    for name, e in worst.items():
+        assert e <= tol_d, name
 
 
 def test_nonfinite_nodes(gpu):

@@ -116,6 +116,35 @@ def test_vjp_central_fd_h1e4_multilinear(mu, kt):
     assert n_static > 100
 
 
+@pytest.mark.parametrize("precise,tol_v", [(True, 1e-5), (False, 2e-6)])
+def test_collision_math_matches_oracle(precise, tol_v):
+    """collision_kernel's arithmetic (BodySDFObj::collision, BodySDFObj.cpp:251-376) on the
+    host, through the instance record atac_sdf_collision builds (body 1 the world frame),
+    against the FP64 oracle: values relative to their largest entry to 1e-5 (FP64) and
+    2e-6 (FP32), derivative blocks to 1e-4 (the closed-form cell derivatives against the
+    oracle's FD stencils). A record without its FP32 copies of the poses (R2 = 0 for the
+    FP32 arithmetic) gives errors of order 1."""
+    w = W.peg_in_hole(n_points=2000, n_poses=1)
+    g = w.grids[0]
+    rng = np.random.default_rng(6)
+    xw = rng.uniform(-1.2, 1.2, size=(500, 3))
+    xd = rng.normal(size=(500, 3))
+    E2 = W.pose(W.rotation_exp([0.05, 0.02, -0.1]), [0.01, 0.02, 0.0])
+    phi2 = rng.normal(0, 0.3, 6)
+    ref = O.collision(O.GridRef.from_grid(g), E2, phi2, 0.01, xw, xd, derivs=True)
+    keep = np.abs(ref["d"]) < 1.0
+    refv = np.concatenate([ref["d"][:, None], ref["n"], ref["ddot"][:, None], ref["tdot"], ref["xi2"]], axis=1)
+    vals, der = H.collision(g, E2, phi2, 0.01, xw, xd, precise=precise)
+    assert keep.sum() > 100
+    assert np.max(np.abs(vals[keep] - refv[keep])) <= tol_v * max(1.0, np.abs(refv[keep]).max())
+    off = 0
+    for name, shape in O.COLLISION_DERIV_LAYOUT:
+        cnt = int(np.prod(shape))
+        bref = ref[name][keep].reshape(-1, cnt)
+        assert np.max(np.abs(der[keep, off:off + cnt] - bref)) <= 1e-4 * max(1.0, np.abs(bref).max()), name
+        off += cnt
+
+
 def test_slow_path_near_faces_and_box():
     """Points placed within the FD step of cell faces, cell corners and the box boundary
     take the FP64 stencil path and reproduce the reference's FD blend exactly."""
