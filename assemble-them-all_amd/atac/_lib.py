@@ -19,6 +19,7 @@ EVAL_P6 = 2
 EVAL_CHECKPOINT = 4
 EVAL_PRECISE = 8
 EVAL_ALL_ACTIVE = 16
+EVAL_FUSED_ADJOINT = 32
 
 BVH_NOALLOC_MAX_POINTS = 1048576   # ATAC_BVH_NOALLOC_MAX_POINTS
 
@@ -27,7 +28,7 @@ EXPORTED = [
     "atac_grid_create", "atac_grid_destroy",
     "atac_points_create", "atac_points_destroy", "atac_points_permutation",
     "atac_batch_create", "atac_batch_destroy", "atac_batch_set_instances", "atac_batch_retune", "atac_batch_info",
-    "atac_batch_eval", "atac_batch_vjp", "atac_batch_graph_capture", "atac_batch_graph_launch",
+    "atac_batch_eval", "atac_batch_vjp", "atac_batch_eval_vjp", "atac_batch_graph_capture", "atac_batch_graph_launch",
     "atac_batch_active_ids", "atac_batch_touched_values", "atac_eval_plain", "atac_eval_plain_vjp", "atac_batch_timing",
     "atac_batch_workspace",
     "atac_batch_live_points",
@@ -102,6 +103,7 @@ def lib():
     L.atac_batch_info.argtypes = [vp, C.POINTER(i64), C.POINTER(i32), C.POINTER(i32), vp]
     L.atac_batch_eval.argtypes = [vp, u32, vp, vp]
     L.atac_batch_vjp.argtypes = [vp, vp, vp, vp]
+    L.atac_batch_eval_vjp.argtypes = [vp, u32, vp, vp, vp, vp]
     L.atac_batch_graph_capture.argtypes = [vp, u32, vp, vp, vp]
     L.atac_batch_graph_launch.argtypes = [vp, vp]
     L.atac_batch_active_ids.argtypes = [vp, i32, vp, i64, C.POINTER(i64)]

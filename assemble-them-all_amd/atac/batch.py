@@ -240,6 +240,18 @@ class ContactBatch:
                                     _stream()))
         return out
 
+    def eval_vjp(self, g_fm, precise=False, all_active=False, out=None, grads=None):
+        """eval() then vjp(g_fm) for a cotangent known beforehand, in one fused pass over the
+        contacts (wrench only); it keeps no records, so vjp() needs a new eval() after it."""
+        g_fm = g_fm.to(dtype=torch.float64).contiguous()
+        out = out if out is not None else self.alloc_outputs(False, False)
+        grads = grads if grads is not None else self.alloc_gradients(g_fm.device)
+        o, gs = self._outputs_struct(out), self._grad_struct(grads)
+        self.generation += 1
+        check(lib().atac_batch_eval_vjp(self.handle, self.flags(False, False, False, precise, all_active), C.byref(o),
+                                        _ptr(g_fm), C.byref(gs), _stream()))
+        return out, grads
+
     def eval_plain(self, deriv=False, p6=False, precise=False, out=None):
         out = out if out is not None else self.alloc_outputs(deriv, p6)
         o = self._outputs_struct(out)
@@ -277,10 +289,12 @@ class ContactBatch:
         check(lib().atac_batch_vjp(self.handle, _ptr(g_fm), C.byref(s), _stream()))
         return out
 
-    def graph_capture(self, out, deriv=False, p6=False, checkpoint=False, precise=False, g_fm=None, grads=None):
+    def graph_capture(self, out, deriv=False, p6=False, checkpoint=False, precise=False, g_fm=None, grads=None,
+                      fused=False):
         o = self._outputs_struct(out)
         gs = self._grad_struct(grads) if grads is not None else None
-        check(lib().atac_batch_graph_capture(self.handle, self.flags(deriv, p6, checkpoint, precise), C.byref(o),
+        fl = self.flags(deriv, p6, checkpoint, precise) | (L.EVAL_FUSED_ADJOINT if fused else 0)
+        check(lib().atac_batch_graph_capture(self.handle, fl, C.byref(o),
                                              _ptr(g_fm) if g_fm is not None else None,
                                              C.byref(gs) if gs is not None else None))
 

@@ -120,6 +120,7 @@ struct atac_batch_s {
     size_t lds_detect = 0;
     int wave_span = 0;
     bool rec_valid = false;           // records of the last evaluation are in place
+    bool graph_fused = false;         // the captured graph is a fused pass (it leaves no records)
     bool rec_precise = false;
     bool rec_captured = false;        // the captured graph's evaluation writes records when it is launched
     bool cap_precise = false;
@@ -469,8 +470,10 @@ static void release_plan(atac_batch b) {
 
 // Width of a work item's partial row for an evaluation mode (0 wrench, 1 + p6,
 // 2 + Km/Dm) and for the adjoint (mode 3).
+// partial row per work item: mode 0-2 the evaluation's, 3 the adjoint's, 4 the
+// fused pass's (wrench and adjoint)
 static int64_t part_width_of(int mode) {
-    return mode == 3 ? kVjpOut : (mode == 2 ? 24 + 288 : (mode == 1 ? 24 : 12));
+    return mode == 4 ? kFmOut + kVjpOut : mode == 3 ? kVjpOut : (mode == 2 ? 24 + 288 : (mode == 1 ? 24 : 12));
 }
 
 // Allocates what an evaluation of this mode and precision (or the adjoint,
@@ -789,11 +792,18 @@ static QueueDev tile_queues(atac_batch b) {
 static int32_t* counters(atac_batch b) { return b->d_counters; }
 
 // overflow_out (optional) receives the evaluation's overflow flag from the reduction.
+// g_fused != nullptr: the fused pass (ATAC_EVAL_FUSED_ADJOINT), the wrench and the
+// adjoint of g_fused into out and gg, no records kept.
 static int enqueue_eval(atac_batch b, uint32_t flags, const atac_outputs* out, hipStream_t st,
-                        int32_t* overflow_out = nullptr) {
-    if (flags & ~(ATAC_EVAL_DERIV | ATAC_EVAL_P6 | ATAC_EVAL_CHECKPOINT | ATAC_EVAL_PRECISE | ATAC_EVAL_ALL_ACTIVE))
+                        int32_t* overflow_out = nullptr, const double* g_fused = nullptr,
+                        const atac_gradients* gg = nullptr) {
+    if (flags & ~(ATAC_EVAL_DERIV | ATAC_EVAL_P6 | ATAC_EVAL_CHECKPOINT | ATAC_EVAL_PRECISE | ATAC_EVAL_ALL_ACTIVE |
+                  (g_fused ? ATAC_EVAL_FUSED_ADJOINT : 0u)))
         return fail(ATAC_ERR_INVALID, "atac_batch_eval: unknown flag");
-    const int mode = mode_of(flags);
+    const bool fused = g_fused != nullptr;
+    if (fused && (flags & (ATAC_EVAL_DERIV | ATAC_EVAL_P6)))
+        return fail(ATAC_ERR_INVALID, "ATAC_EVAL_FUSED_ADJOINT computes the wrench only (no ATAC_EVAL_DERIV or ATAC_EVAL_P6)");
+    const int mode = fused ? 4 : mode_of(flags);
     const bool precise = (flags & ATAC_EVAL_PRECISE) != 0;
     if (out && out->dfm_dp6 && mode < 1) return fail(ATAC_ERR_INVALID, "dfm_dp6 requested without ATAC_EVAL_P6 or ATAC_EVAL_DERIV");
     if (out && (out->Km || out->Dm) && mode < 2) return fail(ATAC_ERR_INVALID, "Km/Dm requested without ATAC_EVAL_DERIV");
@@ -841,8 +851,10 @@ static int enqueue_eval(atac_batch b, uint32_t flags, const atac_outputs* out, h
         ca.chunks = b->d_chunks; ca.inst = b->d_inst; ca.grids = b->d_grids; ca.act = b->d_act; ca.rec = rec;
         ca.slow_diag = b->d_slow_diag;
         ca.counters = counters(b); ca.chunk_cap = (int32_t)b->chunk_cap; ca.partial = b->d_part;
+        ca.g_fm = g_fused;
         hipEvent_t t3 = stage_begin(b, st);
-        ATAC_HIP(launch_contact(ca, mode, precise, b->grid_contact[precise ? 1 : 0][mode], st));
+        if (fused) ATAC_HIP(launch_fused(ca, precise, b->grid_vjp[precise ? 1 : 0], st));
+        else ATAC_HIP(launch_contact(ca, mode, precise, b->grid_contact[precise ? 1 : 0][mode], st));
         stage_end(b, ST_CONTACT, t3, st);
     }
     ReduceArgs ra{};
@@ -853,6 +865,18 @@ static int enqueue_eval(atac_batch b, uint32_t flags, const atac_outputs* out, h
     ra.dst[1] = out ? out->dfm_dp6 : nullptr; ra.begin[1] = 12;  ra.width[1] = 12;
     ra.dst[2] = out ? out->Km : nullptr;      ra.begin[2] = 24;  ra.width[2] = 144;
     ra.dst[3] = out ? out->Dm : nullptr;      ra.begin[3] = 168; ra.width[3] = 144;
+    if (fused) {   // fm 12 | q1 6 | q2 6 | phi1 6 | phi2 6 | kn | kt
+        double* gd[6] = {gg ? gg->g_q1 : nullptr, gg ? gg->g_q2 : nullptr, gg ? gg->g_phi1 : nullptr,
+                         gg ? gg->g_phi2 : nullptr, gg ? gg->g_kn : nullptr, gg ? gg->g_kt : nullptr};
+        const int gw[6] = {6, 6, 6, 6, 1, 1};
+        ra.nseg = 7;
+        ra.dst[1] = nullptr; ra.dst[2] = nullptr; ra.dst[3] = nullptr;
+        for (int k = 0, off = 12; k < 6; off += gw[k], ++k) {
+            ra.dst[1 + k] = gd[k];
+            ra.begin[1 + k] = off;
+            ra.width[1 + k] = gw[k];
+        }
+    }
     ra.n_active = out ? out->n_active : nullptr;
     ra.counters = counters(b);
     ra.overflow_out = overflow_out;
@@ -862,7 +886,11 @@ static int enqueue_eval(atac_batch b, uint32_t flags, const atac_outputs* out, h
     stage_end(b, ST_REDUCE, t3, st);
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     ATAC_HIP(hipStreamIsCapturing(st, &cs));
-    if (cs == hipStreamCaptureStatusNone) {
+    if (fused) {
+        // the fast-path records were not written: nothing for atac_batch_vjp
+        b->rec_valid = false;
+        if (cs != hipStreamCaptureStatusNone) b->rec_captured = false;
+    } else if (cs == hipStreamCaptureStatusNone) {
         b->rec_valid = true;
         b->rec_precise = precise;
     } else {
@@ -918,6 +946,13 @@ int atac_batch_eval(atac_batch b, uint32_t flags, const atac_outputs* out, void*
     return enqueue_eval(b, flags, out, (hipStream_t)stream);
 }
 
+int atac_batch_eval_vjp(atac_batch b, uint32_t flags, const atac_outputs* out, const double* g_fm,
+                        const atac_gradients* grads, void* stream) {
+    if (!b) return fail(ATAC_ERR_INVALID, "atac_batch_eval_vjp: null batch");
+    if (!g_fm) return fail(ATAC_ERR_INVALID, "atac_batch_eval_vjp: g_fm is null");
+    return enqueue_eval(b, flags | ATAC_EVAL_FUSED_ADJOINT, out, (hipStream_t)stream, nullptr, g_fm, grads);
+}
+
 int atac_batch_vjp(atac_batch b, const double* g_fm, const atac_gradients* out, void* stream) {
     if (!b) return fail(ATAC_ERR_INVALID, "atac_batch_vjp: null batch");
     return enqueue_vjp(b, g_fm, out, (hipStream_t)stream);
@@ -926,8 +961,10 @@ int atac_batch_vjp(atac_batch b, const double* g_fm, const atac_gradients* out, 
 int atac_batch_graph_capture(atac_batch b, uint32_t flags, const atac_outputs* out, const double* g_fm,
                              const atac_gradients* grads) {
     if (!b) return fail(ATAC_ERR_INVALID, "atac_batch_graph_capture: null batch");
+    const bool fused = (flags & ATAC_EVAL_FUSED_ADJOINT) != 0;
+    if (fused && !g_fm) return fail(ATAC_ERR_INVALID, "atac_batch_graph_capture: ATAC_EVAL_FUSED_ADJOINT needs g_fm");
     // every buffer the captured launches use exists before capture begins
-    int rc = ensure_buffers(b, mode_of(flags), (flags & ATAC_EVAL_PRECISE) != 0);
+    int rc = ensure_buffers(b, fused ? 4 : mode_of(flags), (flags & ATAC_EVAL_PRECISE) != 0);
     if (!rc && g_fm) rc = ensure_buffers(b, 3, (flags & ATAC_EVAL_PRECISE) != 0);
     if (rc) return rc;
     if (b->graph_exec) { (void)hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
@@ -936,8 +973,14 @@ int atac_batch_graph_capture(atac_batch b, uint32_t flags, const atac_outputs* o
     const bool timing = b->timing;
     b->timing = false;
     ATAC_HIP(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
-    rc = enqueue_eval(b, flags, out, b->cap_stream);
-    if (!rc && g_fm) rc = enqueue_vjp(b, g_fm, grads, b->cap_stream);
+    b->rec_captured = false;
+    b->graph_fused = fused;
+    if (fused) {
+        rc = enqueue_eval(b, flags, out, b->cap_stream, nullptr, g_fm, grads);
+    } else {
+        rc = enqueue_eval(b, flags, out, b->cap_stream);
+        if (!rc && g_fm) rc = enqueue_vjp(b, g_fm, grads, b->cap_stream);
+    }
     b->timing = timing;
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
@@ -951,7 +994,9 @@ int atac_batch_graph_capture(atac_batch b, uint32_t flags, const atac_outputs* o
 int atac_batch_graph_launch(atac_batch b, void* stream) {
     if (!b || !b->graph_exec) return fail(ATAC_ERR_STATE, "atac_batch_graph_launch: no captured graph");
     ATAC_HIP(hipGraphLaunch(b->graph_exec, (hipStream_t)stream));
-    if (b->rec_captured) {   // the graph's evaluation leaves its records for atac_batch_vjp
+    if (b->graph_fused) {
+        b->rec_valid = false;   // its pass overwrote the work lists and kept no records
+    } else if (b->rec_captured) {   // the graph's evaluation leaves its records for atac_batch_vjp
         b->rec_valid = true;
         b->rec_precise = b->cap_precise;
     }

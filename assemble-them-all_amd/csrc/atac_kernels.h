@@ -75,7 +75,9 @@ struct ContactArgs {
     int32_t* counters;
     int32_t chunk_cap;
     int32_t pad;
-    double* partial;            // per work item: fm 12 | p6 12 | Km 144 | Dm 144 (as the mode has them)
+    double* partial;            // per work item: fm 12 | p6 12 | Km 144 | Dm 144 (as the mode has them);
+                                // the fused pass: fm 12 | kVjpOut
+    const double* g_fm;         // the fused pass: d(loss)/d(wrench) per instance (n_inst x 12)
 };
 
 struct VjpArgs {
@@ -102,9 +104,9 @@ struct ReduceArgs {
     const double* partial;
     int32_t nseg;
     int32_t host_dst;           // the outputs are in host memory: each wave ends with a system-scope fence
-    double* dst[6];
-    int32_t begin[6];
-    int32_t width[6];
+    double* dst[8];
+    int32_t begin[8];
+    int32_t width[8];
     int32_t* n_active;          // optional: active contacts per instance
     const int32_t* counters;    // CNT_OVERFLOW set: every output is NaN (the records were incomplete)
     int32_t* overflow_out;      // optional: receives counters[CNT_OVERFLOW] (host-visible memory allowed)
@@ -147,6 +149,8 @@ hipError_t launch_slow(const SlowArgs& a, bool precise, int grid, hipStream_t st
 hipError_t launch_contact(const ContactArgs& a, int mode, bool precise, int grid, hipStream_t st);
 const void* contact_kernel_ptr(int mode, bool precise);
 hipError_t launch_vjp(const VjpArgs& a, bool precise, int grid, hipStream_t st);
+// the wrench and the adjoint of a known cotangent in one pass (no records kept)
+hipError_t launch_fused(const ContactArgs& a, bool precise, int grid, hipStream_t st);
 const void* vjp_kernel_ptr(bool precise);
 hipError_t launch_reduce(const ReduceArgs& a, hipStream_t st);
 hipError_t launch_grid_setup(const GridDev& G, float4* cells, hipStream_t st);

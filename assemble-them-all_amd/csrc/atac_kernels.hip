@@ -567,6 +567,74 @@ __global__ __launch_bounds__(256) void contact_kernel(ContactArgs a) {
     }
 }
 
+// ------------------------------------------------------------------ fused wrench and adjoint
+// Work items of kChunk records, one wavefront each, for a cotangent known when
+// the contacts are evaluated (re-evaluation in a backward pass): each record's
+// sample is formed as in contact_kernel (the slow pass's records read), its
+// wrench summed as contact_kernel's mode 0 sums it and its adjoint as
+// vjp_kernel does, per lane over the records 64 apart in turn, then over the
+// lanes: the partial row is fm 12 | kVjpOut. No record is written.
+template <typename R>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 3 : 1))) void fused_kernel(ContactArgs a) {
+    using Rec = typename RecOf<R>::type;
+    __shared__ double s_red[4][64 * kVjpOut];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int total = min(a.counters[CNT_CHUNKS], a.chunk_cap);
+    for (int c = wave_id(); c < total; c += wave_count()) {
+        const ChunkDev ch = as_const(a.chunks)[c];
+        const InstDev in = as_const(a.inst)[ch.inst];
+        const GridDev G = as_const(a.grids)[in.grid];
+        const int n = ch.n;
+        const Rec* rec = static_cast<const Rec*>(a.rec) + ch.rec0;
+        const auto act = as_global(a.act) + ch.rec0;
+        const InstR<R> I = make_inst<R>(in);
+        R gb[12];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) gb[k] = (R)a.g_fm[(int64_t)ch.inst * 12 + k];
+        LaneAcc<R> acc[kFmOut];
+#pragma unroll
+        for (int k = 0; k < kFmOut; ++k) acc[k].zero();
+        R vacc[kVjpOut];
+#pragma unroll
+        for (int k = 0; k < kVjpOut; ++k) vacc[k] = (R)0;
+        for (int j = lane; j < n; j += 64) {
+            const uint32_t e = act[j];
+            Rec r;
+            if (e & kActSlow) {
+                r = load_rec(rec + j);   // written by the slow pass
+            } else {
+                const uint32_t off = e;
+                const float qx = as_global(in.px)[off], qy = as_global(in.py)[off], qz = as_global(in.pz)[off];
+                double cc[3], fd[3];
+                int idx[3];
+                grid_coords(in, qx, qy, qz, cc);
+                cell_of_c(G, cc, idx, fd);
+                const Corners cr = G.fast ? load_cell(G, idx[0], idx[1], idx[2]) : load_corners(G, idx[0], idx[1], idx[2]);
+                r.a[0] = qx; r.a[1] = qy; r.a[2] = qz; r.a[3] = __uint_as_float(e);
+                record_fast(G, in, cr, fd, r);
+            }
+            ContactFwd<R> s;
+            record_decode(r, a.slow_diag, in, s);
+            contact_forward<R>(I, s, nullptr);
+#pragma unroll
+            for (int k = 0; k < kFmOut; ++k) acc[k].add(s.f[k]);
+            contact_vjp<R, true>(I, s, gb, vacc);
+        }
+        double* dst = a.partial + (int64_t)c * (kFmOut + kVjpOut);
+        double lf[kFmOut];
+#pragma unroll
+        for (int k = 0; k < kFmOut; ++k) lf[k] = acc[k].value();
+        const double colf = lds_column_sums<kFmOut>(&s_red[w][0], lf);
+        if (lane < kFmOut) dst[lane] = colf;
+        double lv[kVjpOut];
+#pragma unroll
+        for (int k = 0; k < kVjpOut; ++k) lv[k] = (double)vacc[k];
+        double col = lds_column_sums<kVjpOut>(&s_red[w][0], lv);
+        if (lane == 24 || lane == 25) col *= in.kn_chain;   // d/d kn = kn_chain * d/d kn_eff (ForceGeneralSDFContact.cpp:155-156)
+        if (lane < kVjpOut) dst[kFmOut + lane] = col;
+    }
+}
+
 // ------------------------------------------------------------------ batched adjoint
 // The adjoint of every record of a work item, with d(loss)/d(wrench) of its
 // instance, summed into the item's partial row (kVjpOut).
@@ -999,6 +1067,12 @@ hipError_t launch_contact(const ContactArgs& a, int mode, bool precise, int grid
 hipError_t launch_vjp(const VjpArgs& a, bool precise, int grid, hipStream_t st) {
     if (precise) hipLaunchKernelGGL(vjp_kernel<double>, dim3(grid), dim3(256), 0, st, a);
     else hipLaunchKernelGGL(vjp_kernel<float>, dim3(grid), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_fused(const ContactArgs& a, bool precise, int grid, hipStream_t st) {
+    if (precise) hipLaunchKernelGGL(fused_kernel<double>, dim3(grid), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL(fused_kernel<float>, dim3(grid), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 

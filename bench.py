@@ -578,6 +578,43 @@ def main():
                 "step_algorithmic_bytes": int(step_alg), "step_achieved": round(step_gbs, 1),
                 "step_frac": round(step_gbs / HBM_PEAK_GBS, 4)}
 
+    # the same outputs by the fused pass (atac_batch_eval_vjp): for a cotangent known when the
+    # contacts are evaluated (a backward pass re-evaluating a step's contacts), one pass computes
+    # the wrench and the adjoint without writing and re-reading the records; graph replay
+    b.eval(checkpoint=True, out=out)
+    b.vjp(g, out=grads)
+    fout, fgr = b.alloc_outputs(), b.alloc_gradients()
+    b.graph_capture(fout, g_fm=g, grads=fgr, fused=True)
+    for _ in range(args.warmup):
+        b.graph_launch()
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(args.steps):
+        b.graph_launch()
+    stop.record()
+    torch.cuda.synchronize()
+    ms_f = start.elapsed_time(stop) / args.steps
+    fdiff = 0.0
+    for ref_t, got_t in [(out["fm"], fout["fm"])] + [(grads[k], fgr[k]) for k in grads]:
+        r_, g_ = ref_t.cpu().numpy(), got_t.cpu().numpy()
+        fdiff = max(fdiff, float(np.abs(g_ - r_).max() / max(np.abs(r_).max(), 1e-300)))
+    b.timing(True)
+    for _ in range(args.steps):
+        b.eval_vjp(g, out=fout, grads=fgr)
+    stf = b.timing(False)
+    torch.cuda.synchronize()
+    fused_alg = step_alg - 16 * n_active   # no (d, grad phi) checkpoint to write and read back
+    fused = {"ms_per_step": round(ms_f, 4), "queries_per_s": n_q / ms_f * 1e3,
+             "stage_ms": {k: round(v[0] / v[1], 4) for k, v in stf.items() if v[1]},
+             "max_rel_diff_vs_separate_passes": fdiff,
+             "step_algorithmic_bytes": int(fused_alg),
+             "step_frac": round(fused_alg / (ms_f * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+             "what": "wrench + adjoint of a cotangent given up front, one fused pass over the contacts "
+                     "(no records kept), HIP graph replay; the headline keeps the separate passes a "
+                     "torch.autograd backward needs"}
+    log(f"fused pass: {fused}")
+    b.graph_capture(out, checkpoint=True, g_fm=g, grads=grads)   # the headline graph again
+
     plain = None
     if not args.no_plain:
         po, pg = b.eval_plain_vjp(g)
@@ -638,6 +675,7 @@ def main():
         "err_vs_fp64": err,
         "plain_path": plain,
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "fused_force_adjoint": fused,
         "fp64_contact": {"kernel_ms_per_step": round(ms64, 4), "queries_per_s": n_q / ms64 * 1e3,
                          "stage_ms": {k: round(v, 4) for k, v in stage64.items()},
                          "what": "detection unchanged; records, wrench and adjoint in FP64 (sum of the stage times)"},

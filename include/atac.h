@@ -79,6 +79,8 @@ typedef struct {
 #define ATAC_EVAL_CHECKPOINT 4u  /* accepted for compatibility: every evaluation keeps its active-contact
                                     records (d, grad phi, Hessian) for atac_batch_vjp */
 #define ATAC_EVAL_PRECISE 8u     /* FP64 contact arithmetic (detection stays FP32) */
+#define ATAC_EVAL_FUSED_ADJOINT 32u /* atac_batch_graph_capture with g_fm: the fused pass of
+                                       atac_batch_eval_vjp instead of evaluation + adjoint */
 #define ATAC_EVAL_ALL_ACTIVE 16u /* every point is a contact, as for the reference's contact-list
                                     overloads (ForceGeneralSDFContact.cpp:147, :221, :449) */
 
@@ -155,7 +157,19 @@ int atac_batch_eval(atac_batch b, uint32_t flags, const atac_outputs* out, void*
  * Stream-ordered after that evaluation. */
 int atac_batch_vjp(atac_batch b, const double* g_fm, const atac_gradients* out, void* stream);
 
-/* Captures [reset, detect, slow, contact, reduce] (and the adjoint when g_fm != NULL)
+/* atac_batch_eval then atac_batch_vjp for a cotangent g_fm (n_inst x 12, device) known when the
+ * contacts are evaluated (a backward pass that re-evaluates a step's contact forces,
+ * ForceGeneralSDFContact::computeForce, ForceGeneralSDFContact.cpp:46-144, with the adjoint
+ * of its wrench): one pass over the active contacts computes both, without writing the
+ * records. The same outputs as the two calls (wrench only: flags ATAC_EVAL_PRECISE and
+ * ATAC_EVAL_ALL_ACTIVE; no Km/Dm or p6). No records are kept: atac_batch_vjp then returns
+ * ATAC_ERR_STATE until the next atac_batch_eval. Stream-ordered. No counterpart call in the
+ * reference, which forms Km and Dm instead. */
+int atac_batch_eval_vjp(atac_batch b, uint32_t flags, const atac_outputs* out, const double* g_fm,
+                        const atac_gradients* grads, void* stream);
+
+/* Captures [reset, detect, slow, contact, reduce] (and the adjoint when g_fm != NULL;
+ * with ATAC_EVAL_FUSED_ADJOINT, the fused pass of atac_batch_eval_vjp instead)
  * into a HIP graph bound to these outputs; atac_batch_graph_launch replays it.
  * Capturing runs nothing: atac_batch_vjp returns ATAC_ERR_STATE until the graph
  * (or an eager evaluation) has been launched. */
